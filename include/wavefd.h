@@ -1,0 +1,185 @@
+/* wavefd — C ABI of the MI355X (gfx950) finite-difference scalar-wave solver.
+ *
+ * Plain pointers and sizes only.  Every device pointer is a HIP device allocation owned by the
+ * caller (typically a torch tensor's data_ptr()); the library allocates device memory only
+ * inside wfd_plan objects.  Every launch is stream-ordered on the caller's hipStream_t (passed as
+ * void*, NULL = the default stream) and the library never synchronises the host except where a
+ * function says so.  Functions return WFD_OK (0) or a positive status; wfd_last_error() gives a
+ * message (thread-local).
+ *
+ * Field layout ("halo layout").  A field of a grid with nodes n[0..ndim-1] (slowest axis first,
+ * boundary ring included) is stored as
+ *     rows of `ld` elements (ld >= n_fast + 4, a multiple of 32),
+ *     `plane` rows per plane in 3D (plane >= n[1] + 4),
+ *     4 zero halo rows (2D) / planes (3D) before and after the nodes,
+ * so that every neighbour of an interior node within 4 nodes along any axis is an in-bounds
+ * element that is either a node or zero.  `offset` is the element index of node (0,..,0) and
+ * `elems` the element count of one field.  Batches of `nshots` fields are contiguous with stride
+ * `elems`.  The outside of the node box (halo, padding) must be zero and stays zero; boundary
+ * nodes stay zero (homogeneous Dirichlet, as step2D_wave.cu:70 never writes them).
+ *
+ * Update (per interior node x, step n; K = order/2, kc = v^2 dt^2, b = sponge damping):
+ *     lap   = wc*u[x];  for each axis d (slowest first), m = 1..K:
+ *             lap = fma(w[d][m-1], u[x - m e_d] + u[x + m e_d], lap)
+ *     t     = 2*u[x] - (1 - b)*u_prev[x]            (t = 2*u[x] - u_prev[x] without sponge)
+ *     num   = fma(kc, lap, t);   for each source s at x: num = fma(kc, sig[n][s], num)
+ *     u_next[x] = num / (1 + b)                     (num without sponge)
+ *     trace[n][r] = u_next[rec_r]
+ * with w[d][m-1] = (T)(a_m / h_d^2), wc = (T)(a_0 * sum_d 1/h_d^2), a_k the central
+ * second-derivative weights of the order.  This extends the reference update
+ * u2 = -u0 + 2u1 + dt^2 c^2 (Lx + Ly) of projects/18_simple_wave_fd/step2D_wave.cu:72-79.
+ */
+#ifndef WAVEFD_H
+#define WAVEFD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WFD_OK 0
+#define WFD_EINVAL 1      /* bad shape, order, CFL violation, aliasing, bad point list */
+#define WFD_EHIP 2        /* a HIP runtime call or launch failed */
+#define WFD_ENOMEM 3      /* plan allocation failed or exceeds the caller's byte budget */
+#define WFD_ENONFINITE 4  /* a medium field holds NaN/Inf or a non-positive coefficient */
+
+#define WFD_F32 0
+#define WFD_F64 1
+
+typedef void* wfd_stream_t; /* hipStream_t */
+
+/* Grid descriptor.  Fill with wfd_grid_init; treat the layout fields as read-only. */
+typedef struct wfd_grid {
+    int32_t ndim;   /* 1, 2 or 3 */
+    int32_t order;  /* 2, 4, 6 or 8 */
+    int32_t dtype;  /* WFD_F32 or WFD_F64 */
+    int32_t nshots; /* independent fields per launch (batch), >= 1 */
+    int64_t n[3];   /* nodes per axis incl. the boundary ring, slowest first (unused = 1) */
+    double h[3];    /* node spacing per axis */
+    double dt;      /* time step */
+    int64_t ld;     /* elements per row */
+    int64_t plane;  /* rows per plane (3D), 1 otherwise */
+    int64_t offset; /* element index of node (0,...,0) */
+    int64_t elems;  /* elements per field */
+} wfd_grid;
+
+/* Medium: kc = v^2 dt^2 per node (a field in the halo layout, shared by all shots when
+ * coef_shot_stride == 0) or the constant kconst when coef == NULL; optional sponge profiles
+ * beta = concatenation of one per-axis profile (n[d] values, slowest axis first), the damping
+ * of node x being the sum of its axes' entries. */
+typedef struct wfd_medium {
+    const void* coef;
+    double kconst;
+    int64_t coef_shot_stride; /* 0 (shared) or elems */
+    const void* beta;         /* NULL = no sponge */
+} wfd_medium;
+
+/* Point list (sources or receivers) in device memory, CSR over rows: a row is every axis but the
+ * fastest (n[0] rows in 2D, n[0]*n[1] in 3D, 1 in 1D).  Build it with wfd_points_csr.
+ * `data` is the signal (sources: read) or the traces (receivers: written), per shot
+ * [nsteps][width], shots `shot_stride` elements apart, dtype of the grid. */
+typedef struct wfd_points {
+    int32_t count;          /* points per shot (0 = none) */
+    int32_t nrows;          /* CSR rows per shot */
+    const int32_t* row_ptr; /* [nshots][nrows + 1], offsets into this shot's col/id */
+    const int32_t* col;     /* [nshots][count], fastest-axis index, ascending within a row */
+    const int32_t* id;      /* [nshots][count], column of the point in `data` */
+    void* data;
+    int64_t width;          /* entries per time step in data */
+    int64_t shot_stride;    /* elements between shots in data */
+} wfd_points;
+
+/* --------------------------------------------------------------------------- host-only */
+const char* wfd_last_error(void);
+const char* wfd_version(void);
+
+/* Fill the descriptor and its layout.  Replaces the reference's manual padding
+ * (NY_PADDED = ceil32(NY), projects/18_simple_wave_fd/2D_wave_cuda_padding.py:43-47) and
+ * cuwave's ScalarWave(Nx, dx, N, dt, ..., space_order) grid (projects/18_fwi/helper.py:63-72). */
+int wfd_grid_init(wfd_grid* g, int ndim, const int64_t* n, const double* h, double dt, int order,
+                  int dtype, int nshots);
+
+/* Largest stable dt for max speed vmax: vmax dt <= 2 / sqrt(sum_d S / h_d^2),
+ * S = |a_0| + 2 sum |a_k|.  Order 2 in 2D gives the reference's min(dx,dy)/c/sqrt(2)
+ * (2D_wave_cupy.py:38); replaces cuwave.wave.stable_dt (projects/18_fwi/helper.py:52). */
+double wfd_stable_dt(int ndim, const double* h, double vmax, int order);
+
+/* The stencil weights the kernels use, as doubles (rounded to the grid dtype first):
+ * wc and w[d*K + m-1]. */
+int wfd_stencil_weights(const wfd_grid* g, double* wc, double* w);
+
+/* Sort `npts` logical flat node indices (C order over n) into a CSR point list of this grid.
+ * Host arrays: row_ptr[nrows+1], col[npts], id[npts] (id = the point's position in `nodes`).
+ * Points must be interior nodes (WFD_EINVAL otherwise).  Replaces cuwave's point_source /
+ * Sensors node lookup (projects/18_fwi/plate.py:51). */
+int wfd_points_csr(const wfd_grid* g, int32_t npts, const int64_t* nodes, int32_t* row_ptr,
+                   int32_t* col, int32_t* id);
+int32_t wfd_points_nrows(const wfd_grid* g);
+
+/* ------------------------------------------------------------------------------ device */
+/* One step of the update above: u_next = step(u_prev, u_cur).  u_next may alias u_prev (the
+ * in-place leapfrog of 2D_wave_cuda_padding.py:84), no other aliasing.  `src`/`rec` may be NULL.
+ * `step` indexes the signal and trace rows.  Replaces padded_step (step2D_wave.cu:59-81) and the
+ * per-step work inside cuwave.wave.simulate. */
+int wfd_step(const wfd_grid* g, const void* u_prev, const void* u_cur, void* u_next,
+             const wfd_medium* med, const wfd_points* src, const wfd_points* rec, int64_t step,
+             wfd_stream_t stream);
+
+/* nsteps steps from (u_a = u^{n-1}, u_b = u^n), n = step0, rotating in place.  On return the
+ * newest level is in u_b when nsteps is even and in u_a when it is odd.  Replaces the host loop
+ * of 2D_wave_cuda_padding.py:83-85 and cuwave.wave.simulate/measure. */
+int wfd_forward(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med,
+                const wfd_points* src, const wfd_points* rec, int64_t step0, int64_t nsteps,
+                wfd_stream_t stream);
+
+/* Same as wfd_forward, replayed from a captured hipGraph of `chunk` steps (chunk even, the
+ * reference's 20-step graph of 2D_wave_cuda_graph.py:73-79,94-95).  The step index travels in
+ * device memory, so one graph serves every chunk; the remainder runs as plain launches. */
+int wfd_forward_graph(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med,
+                      const wfd_points* src, const wfd_points* rec, int64_t step0,
+                      int64_t nsteps, int32_t chunk, wfd_stream_t stream);
+
+/* One adjoint step with imaging: mu_next = step(mu_prev, mu_cur) with the adjoint sources
+ * `adj` (signal = -residual, injected like sources) and recording `rec` (e.g. mu at the forward
+ * sources), then g[x] = fma(mu_next[x], (L u_fwd)[x], g[x]) on interior nodes. */
+int wfd_adjoint_step(const wfd_grid* g, const void* mu_prev, const void* mu_cur, void* mu_next,
+                     const wfd_medium* med, const wfd_points* adj, const wfd_points* rec,
+                     int64_t step, const void* u_fwd, void* g_partial, wfd_stream_t stream);
+
+/* G[x] += sum over shots of g_partial[shot][x] (G float64, one field), then g_partial = 0. */
+int wfd_grad_flush(const wfd_grid* g, void* g_partial, double* G, wfd_stream_t stream);
+
+/* Write 1 to *flag (device int) if any coefficient / beta entry is NaN/Inf or kc <= 0.
+ * wfd_check_medium is the host-synchronising variant: it returns WFD_ENONFINITE itself. */
+int wfd_check_medium_async(const wfd_grid* g, const wfd_medium* med, int32_t* flag,
+                           wfd_stream_t stream);
+int wfd_check_medium(const wfd_grid* g, const wfd_medium* med, wfd_stream_t stream);
+
+/* ------------------------------------------------- gradient plan (sqrt(n) checkpointing) */
+/* A plan owns the checkpoint store, the segment buffer and the working fields for a forward
+ * run of `nsteps` steps from rest and its adjoint.  seg = steps per segment (0: chosen to
+ * minimise 2*ceil(nsteps/seg) + seg fields); max_bytes caps the allocation (0: no cap) and seg
+ * shrinks the store to fit.  Replaces cuwave.utils.misfit_gradient's internal memory strategy
+ * (projects/18_fwi/2D_fwi.py:40) with checkpoint + recompute. */
+typedef struct wfd_plan wfd_plan;
+int wfd_plan_create(wfd_plan** plan, const wfd_grid* g, int64_t nsteps, int64_t seg,
+                    size_t max_bytes);
+int wfd_plan_destroy(wfd_plan* plan);
+int wfd_plan_info(const wfd_plan* plan, int64_t* seg, int64_t* nckpt, size_t* bytes);
+/* Forward from rest over all nsteps, recording `rec` and saving checkpoints.  The final two
+ * levels are copied to u_last (optional, 2 fields: u^{N-1} then u^N). */
+int wfd_plan_forward(wfd_plan* plan, const wfd_medium* med, const wfd_points* src,
+                     const wfd_points* rec, void* u_last, wfd_stream_t stream);
+/* Backward: recompute each segment from its checkpoint, run the adjoint with `adj` injected,
+ * `src_rec` recording mu at the source nodes, and accumulate G = sum_n mu^{n+1} (L u^n)
+ * (float64, one field summed over shots, overwritten). */
+int wfd_plan_backward(wfd_plan* plan, const wfd_medium* med, const wfd_points* src,
+                      const wfd_points* adj, const wfd_points* src_rec, double* G,
+                      wfd_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WAVEFD_H */

@@ -1,0 +1,595 @@
+// Host side of the wavefd C ABI: validation, layout, point lists, launch dispatch, hipGraph
+// replay and the checkpointed gradient plan.  No exceptions cross the ABI; errors are returned
+// as status codes with a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "wfd_internal.h"
+
+using namespace wfd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return fail(WFD_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                          \
+    do {                                             \
+        hipError_t e_ = (expr);                      \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+int64_t gcd64(int64_t a, int64_t b)
+{
+    a = a < 0 ? -a : a;
+    b = b < 0 ? -b : b;
+    while (b) {
+        int64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+// a_0..a_K, each rounded once from the exact rational (see wavefd.h).
+void fd_coeffs(int order, double* a)
+{
+    const int K = order / 2;
+    int64_t f[17];
+    f[0] = 1;
+    for (int i = 1; i <= 16; i++) f[i] = f[i - 1] * i;
+    int64_t sn = 0, sd = 1;
+    for (int k = 1; k <= K; k++) {
+        int64_t num = 2 * f[K] * f[K];
+        int64_t den = (int64_t)k * k * f[K - k] * f[K + k];
+        int64_t q = gcd64(num, den);
+        num /= q;
+        den /= q;
+        if (k % 2 == 0) num = -num;
+        a[k] = (double)num / (double)den;
+        const int64_t nn = sn * den + num * sd, nd = sd * den;
+        q = gcd64(nn, nd);
+        sn = nn / q;
+        sd = nd / q;
+    }
+    a[0] = (double)(-2 * sn) / (double)sd;
+}
+
+bool valid_order(int order) { return order == 2 || order == 4 || order == 6 || order == 8; }
+
+int check_grid(const wfd_grid* g)
+{
+    if (!g) return fail(WFD_EINVAL, "grid is NULL");
+    if (g->ndim < 1 || g->ndim > 3) return fail(WFD_EINVAL, "ndim must be 1, 2 or 3 (got %d)", g->ndim);
+    if (!valid_order(g->order)) return fail(WFD_EINVAL, "order must be 2, 4, 6 or 8 (got %d)", g->order);
+    if (g->dtype != WFD_F32 && g->dtype != WFD_F64) return fail(WFD_EINVAL, "dtype must be WFD_F32 or WFD_F64");
+    if (g->nshots < 1) return fail(WFD_EINVAL, "nshots must be >= 1");
+    if (g->ld <= 0 || g->elems <= 0) return fail(WFD_EINVAL, "grid layout not initialised (wfd_grid_init)");
+    return WFD_OK;
+}
+
+// axis extents as (planes, rows, columns)
+void extents(const wfd_grid* g, int64_t& np, int64_t& nr, int64_t& nc)
+{
+    if (g->ndim == 3) {
+        np = g->n[0];
+        nr = g->n[1];
+        nc = g->n[2];
+    } else if (g->ndim == 2) {
+        np = 1;
+        nr = g->n[0];
+        nc = g->n[1];
+    } else {
+        np = 1;
+        nr = 1;
+        nc = g->n[0];
+    }
+}
+
+size_t esize(const wfd_grid* g) { return g->dtype == WFD_F64 ? 8 : 4; }
+
+template <typename T>
+void fill_weights(const wfd_grid* g, T& wc, T (*w)[4])
+{
+    double a[5];
+    fd_coeffs(g->order, a);
+    const int K = g->order / 2;
+    double s = 0.0;
+    for (int d = 0; d < g->ndim; d++) s += 1.0 / (g->h[d] * g->h[d]);
+    wc = (T)(a[0] * s);
+    for (int d = 0; d < g->ndim; d++)
+        for (int m = 1; m <= 4; m++) w[d][m - 1] = m <= K ? (T)(a[m] / (g->h[d] * g->h[d])) : T(0);
+}
+
+template <typename T>
+int fill_points(PtsDev<T>& p, const wfd_points* src, int32_t nrows, const char* what)
+{
+    std::memset(&p, 0, sizeof p);
+    if (!src || src->count == 0) return WFD_OK;
+    if (src->count < 0) return fail(WFD_EINVAL, "%s: negative point count", what);
+    if (src->nrows != nrows) return fail(WFD_EINVAL, "%s: CSR has %d rows, grid needs %d", what, src->nrows, nrows);
+    if (!src->row_ptr || !src->col || !src->id || !src->data)
+        return fail(WFD_EINVAL, "%s: NULL CSR or data pointer", what);
+    p.row_ptr = src->row_ptr;
+    p.col = src->col;
+    p.id = src->id;
+    p.data = static_cast<T*>(src->data);
+    p.width = src->width;
+    p.shot_stride = src->shot_stride;
+    p.count = src->count;
+    p.nrows = src->nrows;
+    return WFD_OK;
+}
+
+int nrows_of(const wfd_grid* g)
+{
+    if (g->ndim == 3) return (int)(g->n[0] * g->n[1]);
+    if (g->ndim == 2) return (int)g->n[0];
+    return 1;
+}
+
+struct Tuned {
+    int ndim, order, dtype;
+    int64_t n[3];
+    LaunchCfg cfg;
+};
+std::mutex g_tune_mu;
+std::vector<Tuned> g_tuned;
+
+LaunchCfg default_cfg(const wfd_grid* g)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (const Tuned& t : g_tuned)
+            if (t.ndim == g->ndim && t.order == g->order && t.dtype == g->dtype && t.n[0] == g->n[0] &&
+                t.n[1] == g->n[1] && t.n[2] == g->n[2])
+                return t.cfg;
+    }
+    LaunchCfg c{};
+    if (g->ndim == 3) {
+        c.ri = 32;
+    } else if (g->ndim == 2) {
+        // enough waves to cover the chip: about 8 per SIMD over 1024 SIMDs
+        const int64_t chunks = ((g->n[1] + 255) / 256) * (int64_t)g->nshots;
+        const int64_t rows = g->n[0] - 2;
+        int ri = 8;
+        while (ri > 1 && chunks * ((rows + ri - 1) / ri) < 8192) ri /= 2;
+        c.ri = ri;
+    } else {
+        c.ri = 1;
+    }
+    return c;
+}
+
+struct Launch {
+    const wfd_grid* g;
+    const void* up;
+    const void* uc;
+    void* un;
+    const wfd_medium* med;
+    const wfd_points* src;
+    const wfd_points* rec;
+    int64_t step;
+    const int64_t* step_dev;
+    const void* uf;
+    void* gpart;
+    bool img;
+    LaunchCfg cfg;
+};
+
+template <typename T>
+int launch_t(const Launch& L, hipStream_t s)
+{
+    const wfd_grid* g = L.g;
+    StepArgs<T> a;
+    std::memset(&a, 0, sizeof a);
+    const int64_t off = g->offset;
+    a.up = static_cast<const T*>(L.up) + off;
+    a.uc = static_cast<const T*>(L.uc) + off;
+    a.un = static_cast<T*>(L.un) + off;
+    const wfd_medium* m = L.med;
+    const bool varc = m && m->coef;
+    const bool sponge = m && m->beta;
+    if (varc) {
+        a.coef = static_cast<const T*>(m->coef) + off;
+        a.coef_shot_stride = m->coef_shot_stride;
+    }
+    a.kconst = m ? (T)m->kconst : T(0);
+    extents(g, a.np, a.nr, a.nc);
+    if (sponge) {
+        const T* b = static_cast<const T*>(m->beta);
+        if (g->ndim == 3) {
+            a.beta_p = b;
+            a.beta_r = b + a.np;
+            a.beta_c = b + a.np + a.nr;
+        } else if (g->ndim == 2) {
+            a.beta_r = b;
+            a.beta_c = b + a.nr;
+        } else {
+            a.beta_c = b;
+        }
+    }
+    if (L.img) {
+        a.uf = static_cast<const T*>(L.uf) + off;
+        a.g = static_cast<T*>(L.gpart) + off;
+    }
+    a.shot_stride = g->elems;
+    a.ld = g->ld;
+    a.ps = g->plane * g->ld;
+    if (g->ndim == 1) {
+        a.row_lo = 0;
+        a.row_hi = 1;
+    } else {
+        a.row_lo = 1;
+        a.row_hi = a.nr - 1;
+    }
+    T w[3][4];
+    fill_weights<T>(g, a.wc, w);
+    // weights by role: the marching/plane axis, the row axis, the column (fastest) axis
+    const int d_c = g->ndim - 1;
+    for (int mm = 0; mm < 4; mm++) {
+        a.wcol[mm] = w[d_c][mm];
+        a.wr[mm] = g->ndim >= 2 ? w[d_c - 1][mm] : T(0);
+        a.wp[mm] = g->ndim == 3 ? w[0][mm] : T(0);
+    }
+    const int32_t nrows = nrows_of(g);
+    int rc = fill_points<T>(a.src, L.src, nrows, L.img ? "adjoint sources" : "sources");
+    if (rc) return rc;
+    rc = fill_points<T>(a.rec, L.rec, nrows, "receivers");
+    if (rc) return rc;
+    a.step = L.step;
+    a.step_dev = L.step_dev;
+    hipError_t e;
+    if (g->ndim == 3)
+        e = launch_step3d<T>(a, g->order, varc, sponge, L.img, std::max(1, L.cfg.ri), g->nshots, s);
+    else
+        e = launch_step2d<T>(a, g->ndim == 2, g->order, varc, sponge, L.img, L.cfg.ri, g->nshots, s);
+    if (e != hipSuccess) return hip_fail(e, "step kernel launch");
+    return WFD_OK;
+}
+
+int do_launch(const Launch& L, hipStream_t s)
+{
+    return L.g->dtype == WFD_F64 ? launch_t<double>(L, s) : launch_t<float>(L, s);
+}
+
+int check_step_args(const wfd_grid* g, const void* up, const void* uc, void* un, const wfd_medium* med)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!up || !uc || !un) return fail(WFD_EINVAL, "NULL field pointer");
+    if (uc == un) return fail(WFD_EINVAL, "u_next may alias u_prev only, not u_cur");
+    if (up == uc) return fail(WFD_EINVAL, "u_prev and u_cur must be distinct fields");
+    if (!med) return fail(WFD_EINVAL, "medium is NULL");
+    if (med->coef && med->coef_shot_stride != 0 && med->coef_shot_stride != g->elems)
+        return fail(WFD_EINVAL, "coef_shot_stride must be 0 or elems");
+    if (!med->coef && !(med->kconst > 0.0 && std::isfinite(med->kconst)))
+        return fail(WFD_ENONFINITE, "constant coefficient must be finite and > 0");
+    const uintptr_t align = 4 * esize(g);
+    if (((uintptr_t)up | (uintptr_t)uc | (uintptr_t)un) % align ||
+        (med->coef && (uintptr_t)med->coef % align))
+        return fail(WFD_EINVAL, "fields must be %zu-byte aligned", (size_t)align);
+    return WFD_OK;
+}
+
+// one device counter per device for graph replay
+std::mutex g_ctr_mu;
+std::vector<int64_t*> g_ctr;
+
+int device_counter(int64_t** out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(g_ctr_mu);
+    if ((int)g_ctr.size() <= dev) g_ctr.resize(dev + 1, nullptr);
+    if (!g_ctr[dev]) HIP_TRY(hipMalloc(&g_ctr[dev], 64), "hipMalloc(step counter)");
+    *out = g_ctr[dev];
+    return WFD_OK;
+}
+
+}  // namespace
+
+int wfd::set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
+// ======================================================================== exported C ABI
+extern "C" {
+
+const char* wfd_last_error(void) { return g_err.c_str(); }
+const char* wfd_version(void) { return "wavefd 0.1.0 gfx950"; }
+
+int wfd_grid_init(wfd_grid* g, int ndim, const int64_t* n, const double* h, double dt, int order, int dtype,
+                  int nshots)
+{
+    if (!g || !n || !h) return fail(WFD_EINVAL, "NULL argument");
+    std::memset(g, 0, sizeof *g);
+    if (ndim < 1 || ndim > 3) return fail(WFD_EINVAL, "ndim must be 1, 2 or 3 (got %d)", ndim);
+    if (!valid_order(order)) return fail(WFD_EINVAL, "order must be 2, 4, 6 or 8 (got %d)", order);
+    if (dtype != WFD_F32 && dtype != WFD_F64) return fail(WFD_EINVAL, "dtype must be WFD_F32 or WFD_F64");
+    if (nshots < 1) return fail(WFD_EINVAL, "nshots must be >= 1");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(WFD_EINVAL, "dt must be finite and > 0");
+    for (int d = 0; d < ndim; d++) {
+        if (n[d] < 3) return fail(WFD_EINVAL, "axis %d has %lld nodes; at least 3 needed", d, (long long)n[d]);
+        if (n[d] > (int64_t)1 << 30) return fail(WFD_EINVAL, "axis %d too long", d);
+        if (!(h[d] > 0.0) || !std::isfinite(h[d])) return fail(WFD_EINVAL, "spacing of axis %d must be > 0", d);
+    }
+    g->ndim = ndim;
+    g->order = order;
+    g->dtype = dtype;
+    g->nshots = nshots;
+    for (int d = 0; d < 3; d++) {
+        g->n[d] = d < ndim ? n[d] : 1;
+        g->h[d] = d < ndim ? h[d] : 1.0;
+    }
+    g->dt = dt;
+    const int64_t nfast = n[ndim - 1];
+    g->ld = (nfast + HALO + 31) / 32 * 32;
+    if (ndim == 3) {
+        g->plane = n[1] + HALO;
+        g->offset = (int64_t)HALO * g->plane * g->ld;
+        g->elems = (n[0] + 2 * HALO) * g->plane * g->ld;
+    } else {
+        g->plane = 1;
+        const int64_t rows = ndim == 2 ? n[0] : 1;
+        g->offset = (int64_t)HALO * g->ld;
+        g->elems = (rows + 2 * HALO) * g->ld;
+    }
+    if (nrows_of(g) > (1 << 30)) return fail(WFD_EINVAL, "grid too large for 32-bit point rows");
+    return WFD_OK;
+}
+
+double wfd_stable_dt(int ndim, const double* h, double vmax, int order)
+{
+    if (ndim < 1 || ndim > 3 || !h || !valid_order(order) || !(vmax > 0.0)) {
+        fail(WFD_EINVAL, "wfd_stable_dt: bad arguments");
+        return 0.0;
+    }
+    double a[5];
+    fd_coeffs(order, a);
+    double S = std::fabs(a[0]);
+    for (int k = 1; k <= order / 2; k++) S += 2.0 * std::fabs(a[k]);
+    double q = 0.0;
+    for (int d = 0; d < ndim; d++) q += S / (h[d] * h[d]);
+    return 2.0 / std::sqrt(q) / vmax;
+}
+
+int wfd_stencil_weights(const wfd_grid* g, double* wc, double* w)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!wc || !w) return fail(WFD_EINVAL, "NULL output");
+    const int K = g->order / 2;
+    if (g->dtype == WFD_F64) {
+        double c, ww[3][4];
+        fill_weights<double>(g, c, ww);
+        *wc = c;
+        for (int d = 0; d < g->ndim; d++)
+            for (int m = 0; m < K; m++) w[d * K + m] = ww[d][m];
+    } else {
+        float c, ww[3][4];
+        fill_weights<float>(g, c, ww);
+        *wc = c;
+        for (int d = 0; d < g->ndim; d++)
+            for (int m = 0; m < K; m++) w[d * K + m] = ww[d][m];
+    }
+    return WFD_OK;
+}
+
+int32_t wfd_points_nrows(const wfd_grid* g) { return check_grid(g) ? -1 : nrows_of(g); }
+
+int wfd_points_csr(const wfd_grid* g, int32_t npts, const int64_t* nodes, int32_t* row_ptr, int32_t* col,
+                   int32_t* id)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (npts < 0 || (npts > 0 && (!nodes || !col || !id)) || !row_ptr) return fail(WFD_EINVAL, "bad point arrays");
+    int64_t np, nr, nc;
+    extents(g, np, nr, nc);
+    const int32_t nrows = nrows_of(g);
+    std::vector<int64_t> key(npts);
+    for (int32_t p = 0; p < npts; p++) {
+        const int64_t x = nodes[p];
+        if (x < 0 || x >= np * nr * nc) return fail(WFD_EINVAL, "point %d: node %lld outside the grid", p, (long long)x);
+        const int64_t j = x % nc, r = x / nc, i = r % nr, k = r / nr;
+        const bool inner = j >= 1 && j <= nc - 2 && (g->ndim < 2 || (i >= 1 && i <= nr - 2)) &&
+                           (g->ndim < 3 || (k >= 1 && k <= np - 2));
+        if (!inner) return fail(WFD_EINVAL, "point %d: node %lld is not an interior node", p, (long long)x);
+        key[p] = x;  // flat node order = (row, column) order
+    }
+    std::vector<int32_t> perm(npts);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+    std::fill(row_ptr, row_ptr + nrows + 1, 0);
+    for (int32_t q = 0; q < npts; q++) {
+        const int32_t p = perm[q];
+        const int64_t row = key[p] / nc;
+        col[q] = (int32_t)(key[p] % nc);
+        id[q] = p;
+        row_ptr[row + 1]++;
+    }
+    for (int32_t r = 0; r < nrows; r++) row_ptr[r + 1] += row_ptr[r];
+    return WFD_OK;
+}
+
+int wfd_step(const wfd_grid* g, const void* u_prev, const void* u_cur, void* u_next, const wfd_medium* med,
+             const wfd_points* src, const wfd_points* rec, int64_t step, wfd_stream_t stream)
+{
+    int rc = check_step_args(g, u_prev, u_cur, u_next, med);
+    if (rc) return rc;
+    Launch L{g, u_prev, u_cur, u_next, med, src, rec, step, nullptr, nullptr, nullptr, false, default_cfg(g)};
+    return do_launch(L, (hipStream_t)stream);
+}
+
+int wfd_forward(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med, const wfd_points* src,
+                const wfd_points* rec, int64_t step0, int64_t nsteps, wfd_stream_t stream)
+{
+    int rc = check_step_args(g, u_a, u_b, u_a, med);
+    if (rc) return rc;
+    if (nsteps < 0) return fail(WFD_EINVAL, "nsteps must be >= 0");
+    const LaunchCfg cfg = default_cfg(g);
+    void* p = u_a;
+    void* c = u_b;
+    for (int64_t k = 0; k < nsteps; k++) {
+        Launch L{g, p, c, p, med, src, rec, step0 + k, nullptr, nullptr, nullptr, false, cfg};
+        rc = do_launch(L, (hipStream_t)stream);
+        if (rc) return rc;
+        std::swap(p, c);
+    }
+    return WFD_OK;
+}
+
+int wfd_forward_graph(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med, const wfd_points* src,
+                      const wfd_points* rec, int64_t step0, int64_t nsteps, int32_t chunk, wfd_stream_t stream)
+{
+    int rc = check_step_args(g, u_a, u_b, u_a, med);
+    if (rc) return rc;
+    if (nsteps < 0) return fail(WFD_EINVAL, "nsteps must be >= 0");
+    if (chunk < 2 || chunk % 2) return fail(WFD_EINVAL, "chunk must be even and >= 2 (got %d)", chunk);
+    const int64_t reps = nsteps / chunk;
+    hipStream_t st = (hipStream_t)stream;
+    if (reps > 0) {
+        int64_t* ctr = nullptr;
+        rc = device_counter(&ctr);
+        if (rc) return rc;
+        const LaunchCfg cfg = default_cfg(g);
+        hipStream_t cap;
+        HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            void* p = u_a;
+            void* c = u_b;
+            for (int k = 0; k < chunk && rc == WFD_OK; k++) {
+                Launch L{g, p, c, p, med, src, rec, k, ctr, nullptr, nullptr, false, cfg};
+                rc = do_launch(L, cap);
+                std::swap(p, c);
+            }
+            if (rc == WFD_OK) {
+                e = launch_add_i64(ctr, chunk, cap);
+                if (e != hipSuccess) rc = hip_fail(e, "counter kernel");
+            }
+            hipError_t e2 = hipStreamEndCapture(cap, &graph);
+            if (rc == WFD_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
+        } else {
+            rc = hip_fail(e, "hipStreamBeginCapture");
+        }
+        if (rc == WFD_OK) {
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+        }
+        if (rc == WFD_OK) {
+            // counter = step0 (a kernel, stream-ordered), then replay
+            e = launch_set_i64(ctr, step0, st);
+            for (int64_t r = 0; e == hipSuccess && r < reps; r++) e = hipGraphLaunch(exec, st);
+            if (e != hipSuccess) rc = hip_fail(e, "graph replay");
+        }
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        hipStreamDestroy(cap);
+        if (rc) return rc;
+    }
+    const int64_t rem = nsteps - reps * chunk;
+    // after an even number of replayed steps the roles of u_a / u_b are unchanged
+    return wfd_forward(g, u_a, u_b, med, src, rec, step0 + reps * chunk, rem, stream);
+}
+
+int wfd_adjoint_step(const wfd_grid* g, const void* mu_prev, const void* mu_cur, void* mu_next,
+                     const wfd_medium* med, const wfd_points* adj, const wfd_points* rec, int64_t step,
+                     const void* u_fwd, void* g_partial, wfd_stream_t stream)
+{
+    int rc = check_step_args(g, mu_prev, mu_cur, mu_next, med);
+    if (rc) return rc;
+    if (!u_fwd || !g_partial) return fail(WFD_EINVAL, "NULL forward field or gradient partial");
+    if (u_fwd == mu_next || g_partial == mu_next || g_partial == mu_cur || g_partial == u_fwd)
+        return fail(WFD_EINVAL, "adjoint buffers must not alias");
+    Launch L{g, mu_prev, mu_cur, mu_next, med, adj, rec, step, nullptr, u_fwd, g_partial, true, default_cfg(g)};
+    return do_launch(L, (hipStream_t)stream);
+}
+
+int wfd_grad_flush(const wfd_grid* g, void* g_partial, double* G, wfd_stream_t stream)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!g_partial || !G) return fail(WFD_EINVAL, "NULL pointer");
+    hipError_t e = g->dtype == WFD_F64
+                       ? launch_flush<double>((double*)g_partial, G, g->elems, g->nshots, (hipStream_t)stream)
+                       : launch_flush<float>((float*)g_partial, G, g->elems, g->nshots, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "flush kernel");
+    return WFD_OK;
+}
+
+int wfd_check_medium_async(const wfd_grid* g, const wfd_medium* med, int32_t* flag, wfd_stream_t stream)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!med || !flag) return fail(WFD_EINVAL, "NULL argument");
+    int64_t np, nr, nc;
+    extents(g, np, nr, nc);
+    const int64_t nbeta = (g->ndim == 3 ? np + nr : g->ndim == 2 ? nr : 0) + nc;
+    const int64_t ps = g->plane * g->ld;
+    hipError_t e = hipSuccess;
+    const int nsets = med->coef && med->coef_shot_stride ? g->nshots : 1;
+    for (int s = 0; s < nsets && e == hipSuccess; s++) {
+        const int64_t so = (int64_t)s * med->coef_shot_stride + g->offset;
+        if (g->dtype == WFD_F64)
+            e = launch_check<double>(med->coef ? (const double*)med->coef + so : nullptr, np, nr, nc, g->ld, ps,
+                                     s == 0 ? (const double*)med->beta : nullptr, nbeta, flag, (hipStream_t)stream);
+        else
+            e = launch_check<float>(med->coef ? (const float*)med->coef + so : nullptr, np, nr, nc, g->ld, ps,
+                                    s == 0 ? (const float*)med->beta : nullptr, nbeta, flag, (hipStream_t)stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "check kernel");
+    return WFD_OK;
+}
+
+int wfd_check_medium(const wfd_grid* g, const wfd_medium* med, wfd_stream_t stream)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (!med) return fail(WFD_EINVAL, "NULL medium");
+    if (!med->coef && !(med->kconst > 0.0 && std::isfinite(med->kconst)))
+        return fail(WFD_ENONFINITE, "constant coefficient must be finite and > 0");
+    int32_t* flag = nullptr;
+    HIP_TRY(hipMalloc(&flag, sizeof(int32_t)), "hipMalloc(flag)");
+    hipStream_t st = (hipStream_t)stream;
+    int32_t host = 0;
+    hipError_t e = hipMemcpyAsync(flag, &host, sizeof host, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        rc = wfd_check_medium_async(g, med, flag, stream);
+        if (rc == WFD_OK) {
+            e = hipMemcpyAsync(&host, flag, sizeof host, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+        }
+    }
+    hipFree(flag);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "medium check");
+    if (host) return fail(WFD_ENONFINITE, "medium holds a NaN/Inf, a non-positive coefficient or negative damping");
+    return WFD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,304 @@
+// 1D / 2D leapfrog step kernels for gfx950 (wave64).
+//
+// Mapping: a lane owns a chunk of 4 consecutive nodes along the fastest axis (one 16-byte
+// vector load per field and row for fp32), a wave owns 256 consecutive columns of RI
+// consecutive rows and marches down them, keeping the 2K+RI rows of u_cur its stencil needs
+// in registers (each row of u_cur is loaded from memory once per wave).  The column
+// neighbours of the centre row come from the adjacent chunks (same cache lines the
+// neighbouring lanes load, so they are cache hits).  A block is 4 waves stacked along the
+// rows.  The halo layout makes every neighbour read an in-bounds element, so there are no
+// bounds checks on loads; stores mask the boundary ring and the row padding to zero.
+//
+// The arithmetic is the update documented in wavefd.h, in exactly that operation order.
+#include "wfd_internal.h"
+
+namespace wfd {
+
+template <typename T>
+struct C4 {
+    T e[4];
+};
+
+template <typename T>
+__device__ __forceinline__ C4<T> ld4(const T* p);
+template <>
+__device__ __forceinline__ C4<float> ld4<float>(const float* p)
+{
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    return {{v.x, v.y, v.z, v.w}};
+}
+template <>
+__device__ __forceinline__ C4<double> ld4<double>(const double* p)
+{
+    const double2 a = *reinterpret_cast<const double2*>(p);
+    const double2 b = *reinterpret_cast<const double2*>(p + 2);
+    return {{a.x, a.y, b.x, b.y}};
+}
+
+template <typename T>
+__device__ __forceinline__ void st4(T* p, const T (&v)[4]);
+template <>
+__device__ __forceinline__ void st4<float>(float* p, const float (&v)[4])
+{
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+template <>
+__device__ __forceinline__ void st4<double>(double* p, const double (&v)[4])
+{
+    *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+    *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
+}
+
+template <typename T>
+__device__ __forceinline__ C4<T> zero4()
+{
+    return {{T(0), T(0), T(0), T(0)}};
+}
+
+// value at column offset j in [-4, 8) of a chunk, from the left / centre / right chunks
+template <typename T>
+__device__ __forceinline__ T pick(const C4<T>& L, const C4<T>& C, const C4<T>& R, int j)
+{
+    return j < 0 ? L.e[4 + j] : (j < 4 ? C.e[j] : R.e[j - 4]);
+}
+
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// First index in [lo, hi) of a sorted column list with col >= c.
+__device__ __forceinline__ int lower_bound_col(const int32_t* col, int lo, int hi, int64_t c)
+{
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (col[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// num[e] = fma(kc[e], sig, num[e]) for every source in this row that falls in the chunk,
+// in list order (the CSR keeps the caller's order within a node).
+template <typename T>
+__device__ __forceinline__ void inject4(const PtsDev<T>& p, int shot, int64_t row, int64_t c,
+                                        int64_t step, const T (&kc)[4], T (&num)[4])
+{
+    const int32_t* rp = p.row_ptr + (int64_t)shot * (p.nrows + 1);
+    const int lo = rp[row], hi = rp[row + 1];
+    if (lo >= hi) return;
+    const int32_t* col = p.col + (int64_t)shot * p.count;
+    const int32_t* id = p.id + (int64_t)shot * p.count;
+    const T* sig = p.data + (int64_t)shot * p.shot_stride + step * p.width;
+    for (int q = lower_bound_col(col, lo, hi, c); q < hi; q++) {
+        const int64_t cc = col[q];
+        if (cc >= c + 4) break;
+        const T s = sig[id[q]];
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            if (cc == c + e) num[e] = fmaT(kc[e], s, num[e]);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void record4(const PtsDev<T>& p, int shot, int64_t row, int64_t c,
+                                        int64_t step, const T (&val)[4])
+{
+    const int32_t* rp = p.row_ptr + (int64_t)shot * (p.nrows + 1);
+    const int lo = rp[row], hi = rp[row + 1];
+    if (lo >= hi) return;
+    const int32_t* col = p.col + (int64_t)shot * p.count;
+    const int32_t* id = p.id + (int64_t)shot * p.count;
+    T* tr = p.data + (int64_t)shot * p.shot_stride + step * p.width;
+    for (int q = lower_bound_col(col, lo, hi, c); q < hi; q++) {
+        const int64_t cc = col[q];
+        if (cc >= c + 4) break;
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            if (cc == c + e) tr[id[q]] = val[e];
+    }
+}
+
+// Laplacian of chunk element e: centre row `q` of window `win`, column chunks L / R.
+template <typename T, int K, bool VROW>
+__device__ __forceinline__ T lap_elem(const T wc, const T* wr, const T* wcol, const C4<T>* win,
+                                      int q, const C4<T>& L, const C4<T>& R, int e)
+{
+    constexpr int KR = VROW ? K : 0;
+    const C4<T>& C = win[q + KR];
+    T lap = wc * C.e[e];
+    if constexpr (VROW) {
+#pragma unroll
+        for (int m = 1; m <= K; m++) lap = fmaT(wr[m - 1], win[q + KR - m].e[e] + win[q + KR + m].e[e], lap);
+    }
+#pragma unroll
+    for (int m = 1; m <= K; m++) lap = fmaT(wcol[m - 1], pick(L, C, R, e - m) + pick(L, C, R, e + m), lap);
+    return lap;
+}
+
+template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI>
+__global__ __launch_bounds__(256) void step2d_kernel(const StepArgs<T> a)
+{
+    constexpr int KR = VROW ? K : 0;
+    constexpr int NW = RI + 2 * KR;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int shot = blockIdx.z;
+    const int64_t c = ((int64_t)blockIdx.x * 64 + lane) * 4;
+    const int64_t r0 = a.row_lo + ((int64_t)blockIdx.y * 4 + wave) * RI;
+    if (r0 >= a.row_hi || c >= a.nc) return;
+
+    const int64_t sb = (int64_t)shot * a.shot_stride;
+    const T* __restrict__ uc = a.uc + sb;
+    const T* up = a.up + sb;  // may alias un (in-place leapfrog): same element, same lane
+    T* un = a.un + sb;
+    const T* __restrict__ coef = VARC ? a.coef + (int64_t)shot * a.coef_shot_stride : nullptr;
+    const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
+
+    C4<T> win[NW];
+#pragma unroll
+    for (int t = 0; t < NW; t++) {
+        const int64_t r = r0 - KR + t;
+        win[t] = (r < a.row_hi + KR) ? ld4(uc + r * a.ld + c) : zero4<T>();
+    }
+    C4<T> wf[IMG ? NW : 1];
+    if constexpr (IMG) {
+        const T* __restrict__ uf = a.uf + sb;
+#pragma unroll
+        for (int t = 0; t < NW; t++) {
+            const int64_t r = r0 - KR + t;
+            wf[t] = (r < a.row_hi + KR) ? ld4(uf + r * a.ld + c) : zero4<T>();
+        }
+    }
+
+    bool valid[4];
+    T bc[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        valid[e] = (c + e >= 1) && (c + e <= a.nc - 2);
+        bc[e] = T(0);
+        if constexpr (SPONGE) bc[e] = (c + e < a.nc) ? a.beta_c[c + e] : T(0);
+    }
+
+#pragma unroll
+    for (int q = 0; q < RI; q++) {
+        const int64_t r = r0 + q;
+        if (r >= a.row_hi) break;
+        const int64_t o = r * a.ld + c;
+        const C4<T> L = ld4(uc + o - 4);
+        const C4<T> R = ld4(uc + o + 4);
+        const C4<T> P = ld4(up + o);
+        T kc[4];
+        if constexpr (VARC) {
+            const C4<T> kv = ld4(coef + o);
+#pragma unroll
+            for (int e = 0; e < 4; e++) kc[e] = kv.e[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++) kc[e] = a.kconst;
+        }
+        T b[4];
+        if constexpr (SPONGE) {
+            const T br = VROW ? a.beta_r[r] : T(0);
+#pragma unroll
+            for (int e = 0; e < 4; e++) b[e] = VROW ? br + bc[e] : bc[e];
+        }
+        T val[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const T ctr = win[q + KR].e[e];
+            const T lap = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, win, q, L, R, e);
+            T t;
+            if constexpr (SPONGE)
+                t = T(2) * ctr - (T(1) - b[e]) * P.e[e];
+            else
+                t = T(2) * ctr - P.e[e];
+            val[e] = fmaT(kc[e], lap, t);
+        }
+        const int64_t prow = VROW ? r : 0;  // CSR row of the point lists
+        if (a.src.count) inject4(a.src, shot, prow, c, step, kc, val);
+        if constexpr (SPONGE) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) val[e] = val[e] / (T(1) + b[e]);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            if (!valid[e]) val[e] = T(0);
+        st4(un + o, val);
+        if (a.rec.count) record4(a.rec, shot, prow, c, step, val);
+        if constexpr (IMG) {
+            const T* __restrict__ uf = a.uf + sb;
+            T* gp = a.g + sb;
+            const C4<T> Lf = ld4(uf + o - 4);
+            const C4<T> Rf = ld4(uf + o + 4);
+            const C4<T> G = ld4(gp + o);
+            T gv[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const T lapu = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, wf, q, Lf, Rf, e);
+                gv[e] = valid[e] ? fmaT(val[e], lapu, G.e[e]) : T(0);
+            }
+            st4(gp + o, gv);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------ dispatch
+template <typename T, int K, bool VARC, bool SPONGE, bool IMG>
+static hipError_t launch2d_k(const StepArgs<T>& a, bool vrow, int ri, int nshots, hipStream_t s)
+{
+    const dim3 block(256);
+    const int64_t gx = (a.nc + 255) / 256;
+    const int64_t rows = a.row_hi - a.row_lo;
+    if (!vrow) {
+        hipLaunchKernelGGL((step2d_kernel<T, K, false, VARC, SPONGE, IMG, 1>),
+                           dim3((unsigned)gx, 1, nshots), block, 0, s, a);
+        return hipGetLastError();
+    }
+    auto grid = [&](int r) { return dim3((unsigned)gx, (unsigned)((rows + 4 * r - 1) / (4 * r)), nshots); };
+    if constexpr (sizeof(T) == 4) {
+        switch (ri) {
+            case 1: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 1>), grid(1), block, 0, s, a); break;
+            case 2: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 2>), grid(2), block, 0, s, a); break;
+            case 8: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 8>), grid(8), block, 0, s, a); break;
+            default: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4>), grid(4), block, 0, s, a); break;
+        }
+    } else {
+        hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4>), grid(4), block, 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+template <typename T, int K>
+static hipError_t launch2d_flags(const StepArgs<T>& a, bool vrow, bool varc, bool sponge, bool img,
+                                 int ri, int nshots, hipStream_t s)
+{
+    const int f = (varc ? 4 : 0) | (sponge ? 2 : 0) | (img ? 1 : 0);
+    switch (f) {
+        case 0: return launch2d_k<T, K, false, false, false>(a, vrow, ri, nshots, s);
+        case 1: return launch2d_k<T, K, false, false, true>(a, vrow, ri, nshots, s);
+        case 2: return launch2d_k<T, K, false, true, false>(a, vrow, ri, nshots, s);
+        case 3: return launch2d_k<T, K, false, true, true>(a, vrow, ri, nshots, s);
+        case 4: return launch2d_k<T, K, true, false, false>(a, vrow, ri, nshots, s);
+        case 5: return launch2d_k<T, K, true, false, true>(a, vrow, ri, nshots, s);
+        case 6: return launch2d_k<T, K, true, true, false>(a, vrow, ri, nshots, s);
+        default: return launch2d_k<T, K, true, true, true>(a, vrow, ri, nshots, s);
+    }
+}
+
+template <typename T>
+hipError_t launch_step2d(const StepArgs<T>& a, bool vrow, int order, bool varc, bool sponge, bool img,
+                         int ri, int nshots, hipStream_t s)
+{
+    switch (order) {
+        case 2: return launch2d_flags<T, 1>(a, vrow, varc, sponge, img, ri, nshots, s);
+        case 4: return launch2d_flags<T, 2>(a, vrow, varc, sponge, img, ri, nshots, s);
+        case 6: return launch2d_flags<T, 3>(a, vrow, varc, sponge, img, ri, nshots, s);
+        default: return launch2d_flags<T, 4>(a, vrow, varc, sponge, img, ri, nshots, s);
+    }
+}
+
+template hipError_t launch_step2d<float>(const StepArgs<float>&, bool, int, bool, bool, bool, int, int, hipStream_t);
+template hipError_t launch_step2d<double>(const StepArgs<double>&, bool, int, bool, bool, bool, int, int, hipStream_t);
+
+}  // namespace wfd

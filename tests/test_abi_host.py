@@ -1,0 +1,161 @@
+"""CPU tests of the C ABI: the library loads and exports every declared symbol; host-only
+functions (layout, weights, stability limit, point lists, argument validation) behave; the
+host-side layout helpers round-trip.  No kernel is launched here."""
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from neuralmech_amd import _lib, solver
+from oracle import c_oracle, slice_ref
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared():
+    text = (ROOT / "include" / "wavefd.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(wfd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = _declared()
+    assert len(names) >= 18
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, f"{name} not bound in _lib.SIGNATURES"
+    assert b"gfx950" in lib.wfd_version()
+
+
+def _grid(shape, h, order=8, dtype=_lib.WFD_F32, nshots=1, dt=0.1):
+    g = _lib.Grid()
+    n = (ctypes.c_int64 * 3)(*shape, *([1] * (3 - len(shape))))
+    hh = (ctypes.c_double * 3)(*h, *([1.0] * (3 - len(h))))
+    rc = _lib.load().wfd_grid_init(ctypes.byref(g), len(shape), n, hh, dt, order, dtype, nshots)
+    return rc, g
+
+
+@pytest.mark.parametrize("shape", [(2048, 2048), (1024, 1024), (513, 77), (512, 512, 512), (21, 19, 35), (101,)])
+def test_layout(shape):
+    rc, g = _grid(shape, [1.0] * len(shape))
+    assert rc == 0
+    nf = shape[-1]
+    assert g.ld % 32 == 0 and g.ld >= nf + 4
+    if len(shape) == 3:
+        assert g.plane >= shape[1] + 4
+        assert g.offset == 4 * g.plane * g.ld and g.elems == (shape[0] + 8) * g.plane * g.ld
+    else:
+        rows = shape[0] if len(shape) == 2 else 1
+        assert g.offset == 4 * g.ld and g.elems == (rows + 8) * g.ld
+
+
+@pytest.mark.parametrize("bad", [dict(order=3), dict(order=10), dict(shape=(2, 10)), dict(dt=0.0),
+                                 dict(dtype=7), dict(nshots=0)])
+def test_grid_init_rejects(bad):
+    kw = dict(shape=(10, 10), h=(1.0, 1.0), order=4, dtype=_lib.WFD_F32, nshots=1, dt=0.1)
+    kw.update(bad)
+    rc, _ = _grid(kw["shape"], kw["h"], kw["order"], kw["dtype"], kw["nshots"], kw["dt"])
+    assert rc == _lib.WFD_EINVAL
+    assert _lib.last_error()
+
+
+@pytest.mark.parametrize("ndim,order", [(1, 2), (2, 4), (2, 8), (3, 6), (3, 8)])
+@pytest.mark.parametrize("dt_code", [_lib.WFD_F32, _lib.WFD_F64])
+def test_stencil_weights_match_oracle(ndim, order, dt_code):
+    h = [0.7, 1.3, 2.1][:ndim]
+    rc, g = _grid([9] * ndim, h, order, dt_code)
+    assert rc == 0
+    wc = ctypes.c_double()
+    w = (ctypes.c_double * 12)()
+    assert _lib.load().wfd_stencil_weights(ctypes.byref(g), ctypes.byref(wc), w) == 0
+    npdt = np.float32 if dt_code == _lib.WFD_F32 else np.float64
+    owc, ow = c_oracle.weights(ndim, h, order, npdt)
+    assert wc.value == float(owc)
+    assert np.array_equal(np.array(w[: ndim * (order // 2)]), ow.reshape(-1).astype(np.float64))
+
+
+@pytest.mark.parametrize("ndim,order", [(2, 2), (2, 8), (3, 8), (1, 4)])
+def test_stable_dt_matches_closed_form(ndim, order):
+    h = [10.0, 12.0, 9.0][:ndim]
+    assert solver.stable_dt(h, 4500.0, order) == pytest.approx(slice_ref.stable_dt(h, 4500.0, order), rel=1e-14)
+
+
+def _csr(g, nodes):
+    nodes = np.asarray(nodes, np.int64)
+    nrows = _lib.load().wfd_points_nrows(ctypes.byref(g))
+    rp = np.zeros(nrows + 1, np.int32)
+    col = np.zeros(max(len(nodes), 1), np.int32)
+    ids = np.zeros(max(len(nodes), 1), np.int32)
+    rc = _lib.load().wfd_points_csr(ctypes.byref(g), len(nodes), nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                    rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                    col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                    ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return rc, rp, col[: len(nodes)], ids[: len(nodes)]
+
+
+def test_points_csr_sorted_and_stable():
+    rc, g = _grid((20, 30), (1.0, 1.0))
+    pts = [(5, 7), (3, 9), (5, 2), (5, 7), (18, 28), (3, 1)]
+    flat = [i * 30 + j for i, j in pts]
+    rc, rp, col, ids = _csr(g, flat)
+    assert rc == 0 and rp[-1] == len(pts)
+    for r in range(20):
+        seg = slice(rp[r], rp[r + 1])
+        assert all(flat[k] // 30 == r for k in ids[seg])
+        assert list(col[seg]) == sorted(col[seg])
+    # duplicates keep the caller's order
+    dup = [k for k in ids if flat[k] == 5 * 30 + 7]
+    assert dup == [0, 3]
+    # row ranges and columns reproduce the points
+    rebuilt = sorted((r, int(c), int(k)) for r in range(20) for c, k in zip(col[rp[r]:rp[r + 1]], ids[rp[r]:rp[r + 1]]))
+    assert sorted((p[0], p[1], k) for k, p in enumerate(pts)) == rebuilt
+
+
+def test_points_csr_3d_rows():
+    rc, g = _grid((6, 7, 8), (1.0, 1.0, 1.0))
+    pts = [(2, 3, 4), (1, 1, 1), (4, 5, 6)]
+    flat = [np.ravel_multi_index(p, (6, 7, 8)) for p in pts]
+    rc, rp, col, ids = _csr(g, flat)
+    assert rc == 0 and len(rp) == 6 * 7 + 1
+    assert rp[2 * 7 + 3 + 1] - rp[2 * 7 + 3] == 1 and col[rp[2 * 7 + 3]] == 4
+
+
+@pytest.mark.parametrize("node", [(0, 5), (19, 5), (5, 0), (5, 29), (25, 3)])
+def test_points_csr_rejects_non_interior(node):
+    rc, g = _grid((20, 30), (1.0, 1.0))
+    flat = node[0] * 30 + node[1] if node[0] < 20 else 20 * 30 + 3
+    rc, *_ = _csr(g, [flat])
+    assert rc == _lib.WFD_EINVAL
+
+
+def test_step_argument_validation_needs_no_gpu():
+    lib = _lib.load()
+    assert lib.wfd_step(None, None, None, None, None, None, None, 0, None) == _lib.WFD_EINVAL
+    rc, g = _grid((20, 30), (1.0, 1.0))
+    med = _lib.Medium()
+    med.kconst = 0.1
+    buf = (ctypes.c_float * 4)()
+    # u_next aliasing u_cur is rejected before any launch
+    p = ctypes.addressof(buf)
+    assert lib.wfd_step(ctypes.byref(g), p + 64, p, p, ctypes.byref(med), None, None, 0, None) == _lib.WFD_EINVAL
+    med.kconst = float("nan")
+    assert lib.wfd_step(ctypes.byref(g), p, p + 64, p, ctypes.byref(med), None, None, 0, None) == _lib.WFD_ENONFINITE
+    assert lib.wfd_plan_create(ctypes.byref(ctypes.c_void_p()), ctypes.byref(g), 0, 0, 0) == _lib.WFD_EINVAL
+
+
+@pytest.mark.parametrize("shape", [(17, 23), (9, 11, 13), (31,)])
+def test_layout_views_round_trip_on_cpu(shape):
+    g = solver.WaveGrid(shape, [1.0] * len(shape), 0.1, 4, torch.float64, nshots=2, device="cpu")
+    a = torch.randn(2, *shape, dtype=torch.float64)
+    f = g.from_nodes(a)
+    assert f.shape == (2, g.elems)
+    assert torch.equal(g.nodes(f), a)
+    mask = torch.ones_like(f, dtype=torch.bool)
+    g.nodes(mask).fill_(False)
+    assert (f[mask] == 0).all()
+    assert g.interior_cells == int(np.prod([n - 2 for n in shape]))

@@ -1,0 +1,138 @@
+"""GPU adjoint and gradient: bit-exact imaging sum against the C oracle, independence from the
+checkpoint spacing, the dot-product test, central finite differences and torch autograd through
+the CPU restatement."""
+
+import numpy as np
+import pytest
+
+from oracle import c_oracle, torch_ref
+from tests.wave_cases import gpu_objects, make_case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _plan_grad(case, tdt, seg=0, obs=None, nshots=1):
+    from neuralmech_amd import solver
+
+    g, med, src, rec = gpu_objects(case, tdt, nshots=nshots)
+    plan = solver.GradientPlan(g, case["nsteps"], seg=seg)
+    tr = plan.forward(med, src, case["sig"], rec)
+    if obs is None:
+        obs = torch.zeros_like(tr)
+    resid = tr - torch.as_tensor(obs, device=tr.device, dtype=tr.dtype)
+    grad = plan.backward(med, src, rec, resid)
+    torch.cuda.synchronize()
+    return plan, tr, resid, grad
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+@pytest.mark.parametrize("shape,order", [((31, 43), 4), ((40, 36), 8), ((14, 17, 23), 8), ((60,), 2)])
+def test_imaging_sum_bitexact_vs_oracle(shape, order, dtype):
+    print(f"\n[adjoint] shape={shape} order={order} {dtype}", flush=True)
+    npdt, tdt = (np.float32, torch.float32) if dtype == "f32" else (np.float64, torch.float64)
+    case = make_case(shape, order, seed=21, nsteps=70, nrec=5)
+    obs = np.random.default_rng(2).standard_normal((1, 70, 5)) * 1e-3
+    plan, tr, resid, grad = _plan_grad(case, tdt, seg=9, obs=obs)
+    kc = case["kc"].astype(npdt)
+    tr_o, hist, _, _ = c_oracle.history(kc, order, case["h"], 70, beta_profiles=case["prof"],
+                                        src_nodes=case["src"], src_sig=case["sig"].astype(npdt),
+                                        rec_nodes=case["rec"])
+    assert np.array_equal(tr[0].cpu().numpy(), tr_o)
+    r = resid[0].cpu().numpy()
+    G, mu_src = c_oracle.adjoint(kc, order, case["h"], 70, -r, hist, seg=32, beta_profiles=case["prof"],
+                                 rec_nodes=case["rec"], src_nodes=case["src"])
+    assert np.array_equal(plan.last_mu_src[0].cpu().numpy(), mu_src)
+    assert np.array_equal(plan.last_G.cpu().numpy(), G)
+    for s, node in enumerate(case["src"]):
+        G[node] += float(np.dot(mu_src[:, s].astype(np.float64), case["sig"][:, s].astype(npdt).astype(np.float64)))
+    inner = tuple(slice(1, n - 1) for n in shape)
+    ref = np.zeros_like(G)
+    ref[inner] = -G[inner] / kc[inner].astype(np.float64)
+    assert rel_l2(grad.cpu().numpy(), ref) < 1e-13
+
+
+def test_gradient_independent_of_checkpoint_spacing():
+    case = make_case((47, 39), 8, seed=4, nsteps=61)
+    obs = np.zeros((1, 61, len(case["rec"])))
+    grads = [_plan_grad(case, torch.float32, seg=s, obs=obs)[3] for s in (1, 2, 7, 13, 61, 0)]
+    for gr in grads[1:]:
+        assert torch.equal(gr, grads[0])
+
+
+def test_adjoint_dot_product_fp64():
+    """<A s, y> == <s, A^T y> for the signal-to-trace map A, A^T y = -mu at the sources."""
+    case = make_case((41, 52), 6, seed=9, nsteps=80, nsrc=3, nrec=7)
+    rng = np.random.default_rng(3)
+    y = rng.standard_normal((1, 80, 7))
+    from neuralmech_amd import solver
+
+    g, med, src, rec = gpu_objects(case, torch.float64)
+    plan = solver.GradientPlan(g, 80, seg=10)
+    tr = plan.forward(med, src, case["sig"], rec)
+    plan.backward(med, src, rec, torch.as_tensor(y))
+    lhs = float((tr.cpu().double() * torch.as_tensor(y)).sum())
+    ATy = -plan.last_mu_src[0].cpu().numpy()
+    rhs = float(np.sum(case["sig"] * ATy))
+    assert abs(lhs - rhs) <= 1e-11 * max(abs(lhs), 1.0), (lhs, rhs)
+
+
+def _misfit_v(case, v, obs, tdt=torch.float64):
+    from neuralmech_amd import solver
+
+    c = dict(case, kc=(v * case["dt"]) ** 2)
+    g, med, src, rec = gpu_objects(c, tdt)
+    tr, _ = solver.run(g, med, case["nsteps"], src=src, sig=case["sig"], rec=rec)
+    r = tr[0].cpu().double().numpy() - obs
+    return 0.5 * float(np.sum(r * r))
+
+
+def test_gradient_vs_central_finite_differences_fp64():
+    case = make_case((36, 44), 8, seed=13, nsteps=90, nrec=8)
+    rng = np.random.default_rng(33)
+    obs = rng.standard_normal((90, 8)) * 1e-2
+    _, _, _, grad_kc = _plan_grad(case, torch.float64, obs=obs[None])
+    v = case["v"]
+    grad_v = grad_kc.cpu().numpy() * 2 * v * case["dt"] ** 2
+    cells = [(int(rng.integers(2, 34)), int(rng.integers(2, 42))) for _ in range(10)]
+    for cell in cells:
+        hstep = 1e-4 * v[cell]
+        vp = v.copy()
+        vp[cell] += hstep
+        vm = v.copy()
+        vm[cell] -= hstep
+        fd = (_misfit_v(case, vp, obs) - _misfit_v(case, vm, obs)) / (2 * hstep)
+        assert abs(fd - grad_v[cell]) <= 1e-6 * np.max(np.abs(grad_v)) + 1e-7 * abs(fd), (cell, fd, grad_v[cell])
+
+
+def test_gradient_vs_torch_autograd_on_cpu_restatement():
+    case = make_case((24, 29), 4, seed=17, nsteps=50, nrec=5)
+    obs = np.random.default_rng(5).standard_normal((50, 5)) * 1e-2
+    _, _, _, grad_kc = _plan_grad(case, torch.float64, obs=obs[None])
+    grad_v = grad_kc.cpu().numpy() * 2 * case["v"] * case["dt"] ** 2
+    from oracle import slice_ref
+
+    beta = torch.as_tensor(slice_ref.beta_field(case["prof"], np.float64))
+    _, gt = torch_ref.misfit_grad_v(torch.as_tensor(case["v"]), case["dt"], 4, case["h"], 50, torch.as_tensor(obs),
+                                    beta=beta, src_nodes=case["src"], src_sig=case["sig"], rec_nodes=case["rec"])
+    assert rel_l2(grad_v, gt.numpy()) < 1e-10
+
+
+def test_batched_gradient_is_sum_of_shot_gradients():
+    from neuralmech_amd import solver
+
+    case = make_case((33, 51), 8, seed=23, nsteps=40, nrec=6)
+    shots = [[(5, 10)], [(5, 25)], [(5, 40)]]
+    sig = np.random.default_rng(6).standard_normal((3, 40, 1))
+    g, med, _, _ = gpu_objects(case, torch.float64, nshots=3)
+    src = solver.PointSet(g, shots, per_shot=True)
+    rec = solver.PointSet(g, case["rec"])
+    plan = solver.GradientPlan(g, 40)
+    tr = plan.forward(med, src, torch.as_tensor(sig), rec)
+    total = plan.backward(med, src, rec, tr)
+    acc = torch.zeros_like(total)
+    for s in range(3):
+        c1 = dict(case, src=shots[s], sig=sig[s])
+        acc += _plan_grad(c1, torch.float64, obs=np.zeros((1, 40, 6)))[3]
+    assert rel_l2(total.cpu().numpy(), acc.cpu().numpy()) < 1e-13
