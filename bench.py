@@ -1,0 +1,197 @@
+"""Benchmark of the 2D forward single shot: 2048 x 2048 nodes, 8th-order Laplacian, seeded
+smoothed random velocity (1500-4500 m/s) plus three layers, dx = 10 m, 40-node quadratic sponge,
+Ricker 10 Hz source, 512 surface receivers, fp32.
+
+A bench "step" is one leapfrog time step of that shot (one launch of the step kernel over the
+whole grid, source injection and receiver sampling fused).  The default times the full 4000-step
+shot.  Prints one JSON line with Gcell-updates/s, the HBM roofline of the step kernel and the CPU
+oracle timed on the host cores on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def workload(n=2048, nsteps=4000, dtype=None):
+    import numpy as np
+    import torch
+
+    from neuralmech_amd import media, solver
+
+    dtype = dtype or torch.float32
+    h = (10.0, 10.0)
+    v = media.forward_2d_model(n, seed=2)
+    dt = 0.9 * solver.stable_dt(h, float(v.max()), 8)
+    g = solver.WaveGrid((n, n), h, dt, 8, dtype)
+    beta = [media.sponge_profile(n, 40, 0.05) for _ in range(2)]
+    med = solver.Medium(g, velocity=torch.as_tensor(v), beta_profiles=beta)
+    src = solver.PointSet(g, [(50, n // 2)])
+    rec = solver.PointSet(g, [(45, int(j)) for j in np.round(np.linspace(1, n - 2, 512)).astype(int)])
+    sig = media.ricker(np.arange(nsteps) * dt, 10.0) / (h[0] * h[1])
+    return dict(grid=g, medium=med, src=src, rec=rec, sig=sig, v=v, dt=dt, h=h, beta=beta, n=n, nsteps=nsteps)
+
+
+def cpu_baseline(w, budget_s=12.0):
+    """C oracle (scalar restatement, OpenMP over rows) on the same model, bounded number of steps."""
+    import numpy as np
+
+    from oracle import c_oracle
+
+    threads = os.cpu_count() or 1
+    kc = ((w["v"] * w["dt"]) ** 2).astype(np.float32)
+    rec = [(45, int(j)) for j in np.round(np.linspace(1, w["n"] - 2, 512)).astype(int)]
+    kw = dict(beta_profiles=w["beta"], src_nodes=[(50, w["n"] // 2)], rec_nodes=rec, nthreads=threads)
+    sig = w["sig"][:, None].astype(np.float32)
+    t0 = time.perf_counter()
+    c_oracle.forward(kc, 8, w["h"], 2, src_sig=sig[:2], **kw)
+    probe = time.perf_counter() - t0
+    steps = int(max(4, min(400, budget_s / max(probe / 2, 1e-4))))
+    t0 = time.perf_counter()
+    c_oracle.forward(kc, 8, w["h"], steps, src_sig=sig[:steps], **kw)
+    el = time.perf_counter() - t0
+    cells = (w["n"] - 2) ** 2 * steps
+    return {"value": cells / el / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"C oracle, {steps} of {w['nsteps']} steps of the same 2048^2 order-8 shot "
+                      f"(sponge, source, 512 receivers), {threads} OpenMP threads, {el:.1f} s"}
+
+
+def fwi_main(args):
+    """Batched forward + adjoint FWI gradient (16 shots, 1024^2, order 8, 3000 steps) against the
+    plain per-shot torch autograd form; s/shot for both, gradient error against the fp64 run."""
+    import torch
+
+    from neuralmech_amd import fwi
+
+    nsteps = args.fwi_steps
+    w = fwi.setup(nsteps=nsteps)
+    obs = fwi.observed(w)
+    torch.cuda.synchronize()
+    fwi.batched_gradient(w, obs)  # warm-up (allocations, code objects)
+    (J, grad), t_b = fwi.timed(fwi.batched_gradient, w, obs)
+    w64 = fwi.setup(nsteps=nsteps, dtype=torch.float64)
+    (J64, grad64), _ = fwi.timed(fwi.batched_gradient, w64, obs.double())
+    err64 = float((grad - grad64).norm() / grad64.norm())
+    plain_shots = max(1, args.plain_shots)
+    t_p = 0.0
+    for s in range(plain_shots):
+        _, dt_s = fwi.timed(fwi.plain_shot_gradient, w, obs[s], s)
+        t_p += dt_s
+    s_b = t_b / w["nshots"]
+    s_p = t_p / plain_shots
+    cells = (w["n"] - 2) ** 2 * nsteps
+    out = {
+        "metric": "FWI gradient s/shot (batched forward+adjoint)",
+        "value": round(s_b, 5),
+        "unit": "s/shot",
+        "n_gpus": 1,
+        "steps": nsteps,
+        "warmup": 1,
+        "ms_per_step": round(t_b / nsteps * 1e3, 5),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered model +-10 %, smoothed start, Ricker shots)",
+        "config": {"workload": "2D FWI gradient, 16 shots batched: 1024x1024 nodes, order 8, 201 receivers, "
+                               f"{nsteps} steps, sqrt(n) checkpointing", "shots": w["nshots"]},
+        "plain_s_per_shot": round(s_p, 4),
+        "speedup_vs_plain": round(s_p / s_b, 2),
+        "plain_sample": f"{plain_shots} shot(s) of torch-on-GPU slice-form step + autograd",
+        "grad_rel_l2_vs_fp64": err64,
+        "misfit": J,
+        "gcell_updates_per_s_gradient": round(3 * cells * w["nshots"] / t_b / 1e9, 2),
+    }
+    print(json.dumps(out))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--graph", type=int, default=0, help="replay a hipGraph of this many steps (even)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-tune", action="store_true", help="use the default launch shape")
+    ap.add_argument("--workload", choices=["forward", "fwi"], default="forward")
+    ap.add_argument("--fwi-steps", type=int, default=3000)
+    ap.add_argument("--plain-shots", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: the shot runs on a single GPU and this build has no "
+              "multi-GPU path; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "fwi":
+        fwi_main(args)
+        return
+
+    import torch
+
+    from neuralmech_amd import solver
+
+    nsteps = args.warmup + args.steps
+    w = workload(nsteps=max(nsteps, 4000))
+    g, med, src, rec = w["grid"], w["medium"], w["src"], w["rec"]
+    tuned = g.tune(med, steps=40) if not args.no_tune else None
+    state = (g.zeros(), g.zeros())
+    # warm-up steps advance the same shot; the timed steps continue it
+    solver.run(g, med, args.warmup, src=src, sig=w["sig"], rec=rec, state=state, graph_chunk=args.graph)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    t_start, t_end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_start.record(stream)
+    traces, _ = solver.run(g, med, args.steps, src=src, sig=w["sig"], rec=rec, state=state, step0=args.warmup,
+                           graph_chunk=args.graph)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    ms = t_start.elapsed_time(t_end)
+    if not torch.isfinite(traces).all():
+        raise RuntimeError("non-finite traces")
+    cells = g.interior_cells
+    ms_step = ms / args.steps
+    value = cells * args.steps / (ms / 1e3) / 1e9
+    bytes_per_launch = 16 * cells  # u_prev r, u_cur r, kc r, u_next w (fp32)
+    achieved = bytes_per_launch / (ms_step / 1e3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "step2d_order8_2048_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    out = {
+        "metric": "Gcell-updates/s (2D forward single shot, fp32)",
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded smoothed random velocity + layers, Ricker source)",
+        "config": {"workload": "2D forward single shot: 2048x2048 nodes, order 8, dx 10 m, 40-node sponge, "
+                               "Ricker 10 Hz, 512 receivers, 4000 steps",
+                   "grid": [w["n"], w["n"]], "order": 8, "receivers": 512, "graph_chunk": args.graph,
+                   "launch": {"variant": g.launch()[0], "rows": g.launch()[1], "tuned": tuned}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "step2d_kernel<float, K=4, VARC, SPONGE>", "bytes_per_cell": 16},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -157,6 +157,20 @@ int wfd_check_medium_async(const wfd_grid* g, const wfd_medium* med, int32_t* fl
                            wfd_stream_t stream);
 int wfd_check_medium(const wfd_grid* g, const wfd_medium* med, wfd_stream_t stream);
 
+/* ---------------------------------------------------------------------- launch shapes */
+/* The step kernels come in variants (2D: 0 = halo chunks from cache, 1 = halo chunks exchanged
+ * between lanes) and march `rows` rows (2D) or planes (3D) per wave / block.  The choice is kept
+ * per grid shape (ndim, order, dtype, n, nshots) in a process-wide table.  Results do not depend
+ * on it (every variant evaluates the same expression in the same order).
+ * wfd_tune times every candidate for `steps` steps on the caller's scratch fields u_a, u_b
+ * (overwritten; host-synchronising) and records the fastest; the reference's fixed launch
+ * shapes (2D_wave_cuda_padding.py:22,62-63) are what it replaces.
+ * wfd_set_config forces a shape (rows = 0 forgets it); wfd_get_config reports the one in use. */
+int wfd_tune(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med, int32_t steps,
+             int32_t* variant, int32_t* rows, double* us_per_step, wfd_stream_t stream);
+int wfd_set_config(const wfd_grid* g, int32_t variant, int32_t rows);
+int wfd_get_config(const wfd_grid* g, int32_t* variant, int32_t* rows);
+
 /* ------------------------------------------------- gradient plan (sqrt(n) checkpointing) */
 /* A plan owns the checkpoint store, the segment buffer and the working fields for a forward
  * run of `nsteps` steps from rest and its adjoint.  seg = steps per segment (0: chosen to
@@ -178,6 +192,37 @@ int wfd_plan_forward(wfd_plan* plan, const wfd_medium* med, const wfd_points* sr
 int wfd_plan_backward(wfd_plan* plan, const wfd_medium* med, const wfd_points* src,
                       const wfd_points* adj, const wfd_points* src_rec, double* G,
                       wfd_stream_t stream);
+
+
+/* ------------------------------------- reference-layout steps (projects/18_simple_wave_fd) */
+/* Fields are N rows of M_padded elements with the boundary ring inside the array (node (i, j)
+ * at i*M_padded + j); only interior nodes are written.  The update is the slice form of
+ * 2D_wave_cupy.py:54-62 in fp32 with IEEE division and no contraction, coef = f32(dt^2 c^2),
+ * divisors f32(dx^2), f32(dy^2): bit-identical to NumPy evaluating fd_step on float32 arrays.
+ * u2 may alias u0 (the in-place loop of 2D_wave_cuda_padding.py:83-85). */
+
+/* Replaces padded_step (step2D_wave.cu:59-81; M_padded = ceil32(M)), basic_step / coalesced_step
+ * (:8-55; M_padded = M) and, with variant = 1 (LDS-staged tile), shared_step (:187-240). */
+int wfd_ref_step2d(const float* u0, const float* u1, float* u2, double wavespeed, double dt, double dx,
+                   double dy, int32_t N, int32_t M, int32_t M_padded, int32_t variant, wfd_stream_t stream);
+/* Replaces step1D (step1D_wave.cu:4-17). */
+int wfd_ref_step1d(const float* u0, const float* u1, float* u2, double wavespeed, double dt, double dx,
+                   int32_t N, wfd_stream_t stream);
+/* `steps` (2, 4 or 8) steps per launch on LDS tiles, writing the older / newer level to (v0, v1);
+ * bit-identical to `steps` single steps.  Replaces temporal_step (step2D_wave.cu:244-304). */
+int wfd_ref_temporal(const float* u0, const float* u1, float* v0, float* v1, double wavespeed, double dt,
+                     double dx, double dy, int32_t N, int32_t M, int32_t M_padded, int32_t steps,
+                     wfd_stream_t stream);
+/* fp16 storage (IEEE binary16 bit patterns), fp32 arithmetic, round-to-nearest-even stores;
+ * M_padded even and >= M + 2.  Replaces mixed_step (step2D_wave.cu:85-124). */
+int wfd_ref_mixed_step(const void* u0, const void* u1, void* u2, double wavespeed, double dt, double dx,
+                       double dy, int32_t N, int32_t M, int32_t M_padded, wfd_stream_t stream);
+/* int16 values with one fp32 scale per 32 nodes of a row (s has M_padded/32 entries per row,
+ * M_padded % 128 == 0); re-encoded every step by the block maximum.  Replaces scaled_step
+ * (step2D_wave.cu:128-183). */
+int wfd_ref_scaled_step(const int16_t* q0, const float* s0, const int16_t* q1, const float* s1, int16_t* q2,
+                        float* s2, double wavespeed, double dt, double dx, double dy, int32_t N, int32_t M,
+                        int32_t M_padded, wfd_stream_t stream);
 
 #ifdef __cplusplus
 }

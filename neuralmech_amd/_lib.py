@@ -88,11 +88,27 @@ SIGNATURES = {
     "wfd_grad_flush": (ctypes.c_int, [_GP, _P, _P, _P]),
     "wfd_check_medium_async": (ctypes.c_int, [_GP, _MP, _P, _P]),
     "wfd_check_medium": (ctypes.c_int, [_GP, _MP, _P]),
+    "wfd_tune": (ctypes.c_int, [_GP, _P, _P, _MP, ctypes.c_int32, _I32P, _I32P, _DP, _P]),
+    "wfd_set_config": (ctypes.c_int, [_GP, ctypes.c_int32, ctypes.c_int32]),
+    "wfd_get_config": (ctypes.c_int, [_GP, _I32P, _I32P]),
     "wfd_plan_create": (ctypes.c_int, [ctypes.POINTER(_P), _GP, _I64, _I64, ctypes.c_size_t]),
     "wfd_plan_destroy": (ctypes.c_int, [_P]),
     "wfd_plan_info": (ctypes.c_int, [_P, _I64P, _I64P, ctypes.POINTER(ctypes.c_size_t)]),
     "wfd_plan_forward": (ctypes.c_int, [_P, _MP, _PP, _PP, _P, _P]),
     "wfd_plan_backward": (ctypes.c_int, [_P, _MP, _PP, _PP, _PP, _P, _P]),
+    "wfd_ref_step2d": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.c_int32, _P]),
+    "wfd_ref_step1d": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int32, _P]),
+    "wfd_ref_temporal": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_int32, _P]),
+    "wfd_ref_mixed_step": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P]),
+    "wfd_ref_scaled_step": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_int32, _P]),
 }
 
 _lib = None

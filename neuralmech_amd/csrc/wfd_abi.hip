@@ -153,34 +153,61 @@ int nrows_of(const wfd_grid* g)
 }
 
 struct Tuned {
-    int ndim, order, dtype;
+    int ndim, order, dtype, nshots;
     int64_t n[3];
     LaunchCfg cfg;
 };
 std::mutex g_tune_mu;
 std::vector<Tuned> g_tuned;
 
+bool same_shape(const Tuned& t, const wfd_grid* g)
+{
+    return t.ndim == g->ndim && t.order == g->order && t.dtype == g->dtype && t.nshots == g->nshots &&
+           t.n[0] == g->n[0] && t.n[1] == g->n[1] && t.n[2] == g->n[2];
+}
+
+void remember_cfg(const wfd_grid* g, const LaunchCfg& c)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (Tuned& t : g_tuned)
+        if (same_shape(t, g)) {
+            t.cfg = c;
+            return;
+        }
+    Tuned t{g->ndim, g->order, g->dtype, g->nshots, {g->n[0], g->n[1], g->n[2]}, c};
+    g_tuned.push_back(t);
+}
+
+void forget_cfg(const wfd_grid* g)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.erase(std::remove_if(g_tuned.begin(), g_tuned.end(), [&](const Tuned& t) { return same_shape(t, g); }),
+                  g_tuned.end());
+}
+
+// Launch shape used when nothing was tuned or set for this grid: enough waves to give every
+// SIMD (1024 on MI355X) several, then as many rows per wave as that allows.
 LaunchCfg default_cfg(const wfd_grid* g)
 {
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         for (const Tuned& t : g_tuned)
-            if (t.ndim == g->ndim && t.order == g->order && t.dtype == g->dtype && t.n[0] == g->n[0] &&
-                t.n[1] == g->n[1] && t.n[2] == g->n[2])
-                return t.cfg;
+            if (same_shape(t, g)) return t.cfg;
     }
     LaunchCfg c{};
     if (g->ndim == 3) {
         c.ri = 32;
+        c.tile = 0;
     } else if (g->ndim == 2) {
-        // enough waves to cover the chip: about 8 per SIMD over 1024 SIMDs
-        const int64_t chunks = ((g->n[1] + 255) / 256) * (int64_t)g->nshots;
+        c.tile = 1;
+        const int64_t chunks = ((g->n[1] + 247) / 248) * (int64_t)g->nshots;
         const int64_t rows = g->n[0] - 2;
         int ri = 8;
-        while (ri > 1 && chunks * ((rows + ri - 1) / ri) < 8192) ri /= 2;
-        c.ri = ri;
+        while (ri > 1 && chunks * ((rows + ri - 1) / ri) < 4096) ri /= 2;
+        c.ri = g->dtype == WFD_F64 ? 4 : ri;
     } else {
         c.ri = 1;
+        c.tile = 0;
     }
     return c;
 }
@@ -267,7 +294,7 @@ int launch_t(const Launch& L, hipStream_t s)
     if (g->ndim == 3)
         e = launch_step3d<T>(a, g->order, varc, sponge, L.img, std::max(1, L.cfg.ri), g->nshots, s);
     else
-        e = launch_step2d<T>(a, g->ndim == 2, g->order, varc, sponge, L.img, L.cfg.ri, g->nshots, s);
+        e = launch_step2d<T>(a, g->ndim == 2, g->order, varc, sponge, L.img, L.cfg.tile, L.cfg.ri, g->nshots, s);
     if (e != hipSuccess) return hip_fail(e, "step kernel launch");
     return WFD_OK;
 }
@@ -589,6 +616,96 @@ int wfd_check_medium(const wfd_grid* g, const wfd_medium* med, wfd_stream_t stre
     if (rc) return rc;
     if (e != hipSuccess) return hip_fail(e, "medium check");
     if (host) return fail(WFD_ENONFINITE, "medium holds a NaN/Inf, a non-positive coefficient or negative damping");
+    return WFD_OK;
+}
+
+// ------------------------------------------------------------------------- launch tuning
+int wfd_set_config(const wfd_grid* g, int32_t variant, int32_t rows)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    if (rows == 0) {
+        forget_cfg(g);
+        return WFD_OK;
+    }
+    const bool ok2d = g->ndim == 2 && (variant == 0 || variant == 1) &&
+                      (g->dtype == WFD_F64 ? rows == 4 : (rows == 1 || rows == 2 || rows == 4 || rows == 8));
+    const bool ok3d = g->ndim == 3 && variant == 0 && rows >= 1 && rows <= 1024;
+    const bool ok1d = g->ndim == 1 && variant == 0 && rows == 1;
+    if (!(ok1d || ok2d || ok3d)) return fail(WFD_EINVAL, "unsupported launch config (variant %d, rows %d)", variant, rows);
+    LaunchCfg c{};
+    c.tile = variant;
+    c.ri = rows;
+    remember_cfg(g, c);
+    return WFD_OK;
+}
+
+int wfd_get_config(const wfd_grid* g, int32_t* variant, int32_t* rows)
+{
+    int rc = check_grid(g);
+    if (rc) return rc;
+    const LaunchCfg c = default_cfg(g);
+    if (variant) *variant = c.tile;
+    if (rows) *rows = c.ri;
+    return WFD_OK;
+}
+
+int wfd_tune(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med, int32_t steps, int32_t* variant,
+             int32_t* rows, double* us_per_step, wfd_stream_t stream)
+{
+    int rc = check_step_args(g, u_a, u_b, u_a, med);
+    if (rc) return rc;
+    if (steps < 2) return fail(WFD_EINVAL, "wfd_tune: steps must be >= 2");
+    std::vector<LaunchCfg> cands;
+    if (g->ndim == 2) {
+        const int f32rows[] = {1, 2, 4, 8};
+        for (int v = 0; v < 2; v++) {
+            if (g->dtype == WFD_F64) {
+                cands.push_back({4, v});
+            } else {
+                for (int r : f32rows) cands.push_back({r, v});
+            }
+        }
+    } else if (g->ndim == 3) {
+        for (int r : {4, 8, 16, 32, 64}) cands.push_back({r, 0});
+    } else {
+        cands.push_back({1, 0});
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "hipEventCreate");
+    HIP_TRY(hipEventCreate(&e1), "hipEventCreate");
+    float best_ms = 1e30f;
+    LaunchCfg best = cands[0];
+    for (const LaunchCfg& c : cands) {
+        for (int k = 0; k < steps + 2 && rc == WFD_OK; k++) {
+            if (k == 2) hipEventRecord(e0, st);
+            void* p = (k % 2) ? u_b : u_a;
+            void* q = (k % 2) ? u_a : u_b;
+            Launch L{g, p, q, p, med, nullptr, nullptr, k, nullptr, nullptr, nullptr, false, c};
+            rc = do_launch(L, st);
+        }
+        if (rc) break;
+        hipEventRecord(e1, st);
+        hipError_t e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "wfd_tune timing");
+            break;
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    remember_cfg(g, best);
+    if (variant) *variant = best.tile;
+    if (rows) *rows = best.ri;
+    if (us_per_step) *us_per_step = 1e3 * best_ms / steps;
     return WFD_OK;
 }
 

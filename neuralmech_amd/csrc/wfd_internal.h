@@ -63,7 +63,7 @@ hipError_t launch_check(const T* origin, int64_t np, int64_t nr, int64_t nc, int
 
 template <typename T>
 hipError_t launch_step2d(const StepArgs<T>& a, bool vrow, int order, bool varc, bool sponge, bool img,
-                         int ri, int nshots, hipStream_t s);
+                         int variant, int ri, int nshots, hipStream_t s);
 template <typename T>
 hipError_t launch_step3d(const StepArgs<T>& a, int order, bool varc, bool sponge, bool img, int rk,
                          int nshots, hipStream_t s);

@@ -45,7 +45,8 @@ int plan_fail(int code, const char* msg) { return wfd::set_error(code, msg); }
 size_t fields_needed(int64_t N, int64_t C)
 {
     const int64_t nck = (N + C - 1) / C;
-    const int64_t ck_fields = C == 1 ? nck : 2 * nck;
+    // C == 1: checkpoint k's older level is checkpoint k-1's newer one; only u^{-1} is extra
+    const int64_t ck_fields = C == 1 ? nck + 1 : 2 * nck;
     return (size_t)(ck_fields + (C - 1) + 2 + 2 + 1);
 }
 
@@ -104,6 +105,14 @@ int wfd_plan_create(wfd_plan** out, const wfd_grid* g, int64_t nsteps, int64_t s
     if (e != hipSuccess) {
         delete p;
         return plan_fail(WFD_ENOMEM, "wfd_plan_create: hipMalloc of the checkpoint store failed");
+    }
+    // The kernels write interior nodes only: halo, padding and boundary nodes of every field in
+    // the store must start (and so stay) zero.
+    e = hipMemset(p->mem, 0, p->bytes);
+    if (e != hipSuccess) {
+        hipFree(p->mem);
+        delete p;
+        return plan_fail(WFD_EHIP, "wfd_plan_create: clearing the checkpoint store failed");
     }
     char* q = p->mem;
     auto take = [&]() {
@@ -167,8 +176,10 @@ int wfd_plan_forward(wfd_plan* p, const wfd_medium* med, const wfd_points* src, 
             D = p->ck_prev[(m + 1) / C];
         else if (m % C == 0 && m / C < p->nckpt)
             D = p->ck_cur[m / C];
+        else if (P == p->work[0] || P == p->work[1])
+            D = P;  // in place over the oldest level
         else
-            D = (p->work[0] != P && p->work[0] != Cu) ? p->work[0] : p->work[1];
+            D = Cu == p->work[0] ? p->work[1] : p->work[0];
         int rc = wfd_step(&p->g, P, Cu, D, med, src, rec, n, stream);
         if (rc) return rc;
         P = Cu;

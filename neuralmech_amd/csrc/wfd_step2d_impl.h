@@ -10,6 +10,7 @@
 // bounds checks on loads; stores mask the boundary ring and the row padding to zero.
 //
 // The arithmetic is the update documented in wavefd.h, in exactly that operation order.
+#pragma once
 #include "wfd_internal.h"
 
 namespace wfd {
@@ -243,62 +244,207 @@ __global__ __launch_bounds__(256) void step2d_kernel(const StepArgs<T> a)
     }
 }
 
+// Variant with the horizontal halo exchanged between lanes: the waves of a row band overlap by
+// one chunk on each side (lane 0 and lane 63 only load), so the left / right neighbour chunks of
+// an owning lane are the centre chunks of lanes l-1 / l+1, moved with cross-lane shuffles instead
+// of extra cache loads.  A wave owns 62 chunks = 248 columns.  Inside the sponge-free interior
+// the damping terms are skipped (b = 0 makes them exact no-ops, so results are identical).
+constexpr int XL_COLS = 248;
+
+template <typename T>
+__device__ __forceinline__ C4<T> from_lane_left(const C4<T>& v)
+{
+    C4<T> r;
+#pragma unroll
+    for (int e = 0; e < 4; e++) r.e[e] = __shfl_up(v.e[e], 1);
+    return r;
+}
+template <typename T>
+__device__ __forceinline__ C4<T> from_lane_right(const C4<T>& v)
+{
+    C4<T> r;
+#pragma unroll
+    for (int e = 0; e < 4; e++) r.e[e] = __shfl_down(v.e[e], 1);
+    return r;
+}
+
+template <typename T, int K, bool VARC, bool SPONGE, bool IMG, int RI>
+__global__ __launch_bounds__(256) void step2d_xl_kernel(const StepArgs<T> a)
+{
+    constexpr int NW = RI + 2 * K;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int shot = blockIdx.z;
+    const int64_t c = (int64_t)blockIdx.x * XL_COLS + lane * 4 - 4;
+    const int64_t r0 = a.row_lo + ((int64_t)blockIdx.y * 4 + wave) * RI;
+    if (r0 >= a.row_hi) return;  // whole wave
+    const bool load_ok = c < a.ld;  // lanes beyond the row load nothing (no wrap into other rows)
+    const bool own = lane >= 1 && lane <= 62 && c < a.nc;
+
+    const int64_t sb = (int64_t)shot * a.shot_stride;
+    const T* __restrict__ uc = a.uc + sb;
+    const T* up = a.up + sb;  // may alias un (in-place leapfrog): same element, same lane
+    T* un = a.un + sb;
+    const T* __restrict__ coef = VARC ? a.coef + (int64_t)shot * a.coef_shot_stride : nullptr;
+    const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
+
+    C4<T> win[NW];
+#pragma unroll
+    for (int t = 0; t < NW; t++) {
+        const int64_t r = r0 - K + t;
+        win[t] = (r < a.row_hi + K && load_ok) ? ld4(uc + r * a.ld + c) : zero4<T>();
+    }
+    C4<T> wf[IMG ? NW : 1];
+    if constexpr (IMG) {
+        const T* __restrict__ uf = a.uf + sb;
+#pragma unroll
+        for (int t = 0; t < NW; t++) {
+            const int64_t r = r0 - K + t;
+            wf[t] = (r < a.row_hi + K && load_ok) ? ld4(uf + r * a.ld + c) : zero4<T>();
+        }
+    }
+
+    bool valid[4];
+    T bc[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        valid[e] = (c + e >= 1) && (c + e <= a.nc - 2);
+        bc[e] = T(0);
+        if constexpr (SPONGE) bc[e] = (c + e >= 0 && c + e < a.nc) ? a.beta_c[c + e] : T(0);
+    }
+
+#pragma unroll
+    for (int q = 0; q < RI; q++) {
+        const int64_t r = r0 + q;
+        if (r >= a.row_hi) break;
+        const int64_t o = r * a.ld + c;
+        const C4<T> L = from_lane_left(win[q + K]);
+        const C4<T> R = from_lane_right(win[q + K]);
+        C4<T> Lf, Rf;
+        if constexpr (IMG) {
+            Lf = from_lane_left(wf[q + K]);
+            Rf = from_lane_right(wf[q + K]);
+        }
+        if (!own) continue;
+        const C4<T> P = ld4(up + o);
+        T kc[4];
+        if constexpr (VARC) {
+            const C4<T> kv = ld4(coef + o);
+#pragma unroll
+            for (int e = 0; e < 4; e++) kc[e] = kv.e[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++) kc[e] = a.kconst;
+        }
+        T b[4] = {T(0), T(0), T(0), T(0)};
+        bool damp = false;
+        if constexpr (SPONGE) {
+            const T br = a.beta_r[r];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                b[e] = br + bc[e];
+                damp |= b[e] != T(0);
+            }
+        }
+        T val[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const T ctr = win[q + K].e[e];
+            const T lap = lap_elem<T, K, true>(a.wc, a.wr, a.wcol, win, q, L, R, e);
+            const T t = damp ? T(2) * ctr - (T(1) - b[e]) * P.e[e] : T(2) * ctr - P.e[e];
+            val[e] = fmaT(kc[e], lap, t);
+        }
+        if (a.src.count) inject4(a.src, shot, r, c, step, kc, val);
+        if (damp) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) val[e] = val[e] / (T(1) + b[e]);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            if (!valid[e]) val[e] = T(0);
+        st4(un + o, val);
+        if (a.rec.count) record4(a.rec, shot, r, c, step, val);
+        if constexpr (IMG) {
+            T* gp = a.g + sb;
+            const C4<T> G = ld4(gp + o);
+            T gv[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const T lapu = lap_elem<T, K, true>(a.wc, a.wr, a.wcol, wf, q, Lf, Rf, e);
+                gv[e] = valid[e] ? fmaT(val[e], lapu, G.e[e]) : T(0);
+            }
+            st4(gp + o, gv);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------ dispatch
 template <typename T, int K, bool VARC, bool SPONGE, bool IMG>
-static hipError_t launch2d_k(const StepArgs<T>& a, bool vrow, int ri, int nshots, hipStream_t s)
+static hipError_t launch2d_k(const StepArgs<T>& a, bool vrow, int variant, int ri, int nshots, hipStream_t s)
 {
     const dim3 block(256);
-    const int64_t gx = (a.nc + 255) / 256;
     const int64_t rows = a.row_hi - a.row_lo;
     if (!vrow) {
         hipLaunchKernelGGL((step2d_kernel<T, K, false, VARC, SPONGE, IMG, 1>),
-                           dim3((unsigned)gx, 1, nshots), block, 0, s, a);
+                           dim3((unsigned)((a.nc + 255) / 256), 1, nshots), block, 0, s, a);
         return hipGetLastError();
     }
-    auto grid = [&](int r) { return dim3((unsigned)gx, (unsigned)((rows + 4 * r - 1) / (4 * r)), nshots); };
+    auto grid = [&](int64_t cols, int r) {
+        return dim3((unsigned)((a.nc + cols - 1) / cols), (unsigned)((rows + 4 * r - 1) / (4 * r)), nshots);
+    };
+    if (variant == 1) {
+        if constexpr (sizeof(T) == 4) {
+            switch (ri) {
+                case 1: hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 1>), grid(XL_COLS, 1), block, 0, s, a); break;
+                case 2: hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 2>), grid(XL_COLS, 2), block, 0, s, a); break;
+                case 8: hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 8>), grid(XL_COLS, 8), block, 0, s, a); break;
+                default: hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4>), grid(XL_COLS, 4), block, 0, s, a); break;
+            }
+        } else {
+            hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4>), grid(XL_COLS, 4), block, 0, s, a);
+        }
+        return hipGetLastError();
+    }
     if constexpr (sizeof(T) == 4) {
         switch (ri) {
-            case 1: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 1>), grid(1), block, 0, s, a); break;
-            case 2: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 2>), grid(2), block, 0, s, a); break;
-            case 8: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 8>), grid(8), block, 0, s, a); break;
-            default: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4>), grid(4), block, 0, s, a); break;
+            case 1: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 1>), grid(256, 1), block, 0, s, a); break;
+            case 2: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 2>), grid(256, 2), block, 0, s, a); break;
+            case 8: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 8>), grid(256, 8), block, 0, s, a); break;
+            default: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4>), grid(256, 4), block, 0, s, a); break;
         }
     } else {
-        hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4>), grid(4), block, 0, s, a);
+        hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4>), grid(256, 4), block, 0, s, a);
     }
     return hipGetLastError();
 }
 
 template <typename T, int K>
 static hipError_t launch2d_flags(const StepArgs<T>& a, bool vrow, bool varc, bool sponge, bool img,
-                                 int ri, int nshots, hipStream_t s)
+                                 int variant, int ri, int nshots, hipStream_t s)
 {
     const int f = (varc ? 4 : 0) | (sponge ? 2 : 0) | (img ? 1 : 0);
     switch (f) {
-        case 0: return launch2d_k<T, K, false, false, false>(a, vrow, ri, nshots, s);
-        case 1: return launch2d_k<T, K, false, false, true>(a, vrow, ri, nshots, s);
-        case 2: return launch2d_k<T, K, false, true, false>(a, vrow, ri, nshots, s);
-        case 3: return launch2d_k<T, K, false, true, true>(a, vrow, ri, nshots, s);
-        case 4: return launch2d_k<T, K, true, false, false>(a, vrow, ri, nshots, s);
-        case 5: return launch2d_k<T, K, true, false, true>(a, vrow, ri, nshots, s);
-        case 6: return launch2d_k<T, K, true, true, false>(a, vrow, ri, nshots, s);
-        default: return launch2d_k<T, K, true, true, true>(a, vrow, ri, nshots, s);
+        case 0: return launch2d_k<T, K, false, false, false>(a, vrow, variant, ri, nshots, s);
+        case 1: return launch2d_k<T, K, false, false, true>(a, vrow, variant, ri, nshots, s);
+        case 2: return launch2d_k<T, K, false, true, false>(a, vrow, variant, ri, nshots, s);
+        case 3: return launch2d_k<T, K, false, true, true>(a, vrow, variant, ri, nshots, s);
+        case 4: return launch2d_k<T, K, true, false, false>(a, vrow, variant, ri, nshots, s);
+        case 5: return launch2d_k<T, K, true, false, true>(a, vrow, variant, ri, nshots, s);
+        case 6: return launch2d_k<T, K, true, true, false>(a, vrow, variant, ri, nshots, s);
+        default: return launch2d_k<T, K, true, true, true>(a, vrow, variant, ri, nshots, s);
     }
 }
 
 template <typename T>
 hipError_t launch_step2d(const StepArgs<T>& a, bool vrow, int order, bool varc, bool sponge, bool img,
-                         int ri, int nshots, hipStream_t s)
+                         int variant, int ri, int nshots, hipStream_t s)
 {
     switch (order) {
-        case 2: return launch2d_flags<T, 1>(a, vrow, varc, sponge, img, ri, nshots, s);
-        case 4: return launch2d_flags<T, 2>(a, vrow, varc, sponge, img, ri, nshots, s);
-        case 6: return launch2d_flags<T, 3>(a, vrow, varc, sponge, img, ri, nshots, s);
-        default: return launch2d_flags<T, 4>(a, vrow, varc, sponge, img, ri, nshots, s);
+        case 2: return launch2d_flags<T, 1>(a, vrow, varc, sponge, img, variant, ri, nshots, s);
+        case 4: return launch2d_flags<T, 2>(a, vrow, varc, sponge, img, variant, ri, nshots, s);
+        case 6: return launch2d_flags<T, 3>(a, vrow, varc, sponge, img, variant, ri, nshots, s);
+        default: return launch2d_flags<T, 4>(a, vrow, varc, sponge, img, variant, ri, nshots, s);
     }
 }
-
-template hipError_t launch_step2d<float>(const StepArgs<float>&, bool, int, bool, bool, bool, int, int, hipStream_t);
-template hipError_t launch_step2d<double>(const StepArgs<double>&, bool, int, bool, bool, bool, int, int, hipStream_t);
 
 }  // namespace wfd

@@ -114,6 +114,26 @@ class WaveGrid:
             return np.ravel_multi_index(tuple(a.T), self.shape).astype(np.int64)
         return a.reshape(-1)
 
+    # ------------------------------------------------------------------ launch shapes
+    def set_launch(self, variant: int, rows: int) -> None:
+        """Force the kernel variant and rows (2D) / planes (3D) per wave for this grid shape;
+        ``rows = 0`` returns to the tuned or default shape."""
+        check(_lib.load().wfd_set_config(ctypes.byref(self.c), int(variant), int(rows)))
+
+    def launch(self):
+        v, r = ctypes.c_int32(), ctypes.c_int32()
+        check(_lib.load().wfd_get_config(ctypes.byref(self.c), ctypes.byref(v), ctypes.byref(r)))
+        return v.value, r.value
+
+    def tune(self, medium: "Medium", steps: int = 20) -> dict:
+        """Time every launch shape on scratch fields and keep the fastest for this grid shape."""
+        a, b = self.zeros(), self.zeros()
+        v, r, us = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        check(_lib.load().wfd_tune(ctypes.byref(self.c), a.data_ptr(), b.data_ptr(), ctypes.byref(medium.c),
+                                   int(steps), ctypes.byref(v), ctypes.byref(r), ctypes.byref(us),
+                                   _stream_ptr(self.device)))
+        return {"variant": v.value, "rows": r.value, "us_per_step": us.value}
+
     def stencil_weights(self):
         """``(wc, w[ndim, K])`` as the kernels use them (rounded to the grid dtype)."""
         K = self.order // 2

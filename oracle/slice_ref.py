@@ -249,3 +249,66 @@ def adjoint_gradient(kc, order, h, nsteps, observed, beta=None, src_nodes=(), sr
                 G[node] += mu[node] * sig[m - 1, s]
         mu_nn, mu_n = mu_n, mu
     return J, -G / kc
+
+
+# ------------------------------------------------------- storage-compressed reference steps
+def fd_step_reference_f32(u0, u1, u2, dt, dx, dy, wavespeed, N, M):
+    """``fd_step`` on the ``(N, M)`` nodes of fp32 arrays of any row stride (``step2D_wave.cu:59-81``
+    semantics: the boundary ring and the row padding are never written)."""
+    fd_step_reference(u0[:N, :M], u1[:N, :M], u2[:N, :M], dt, dx, dy, wavespeed)
+    return u2
+
+
+def mixed_step_reference(u0, u1, u2, dt, dx, dy, wavespeed, N, M):
+    """``mixed_step`` (``step2D_wave.cu:85-124``): fp16 storage, the fp32 slice-form update, and
+    round-to-nearest-even back to fp16 on the interior."""
+    a0 = u0[:N, :M].astype(np.float32)
+    a1 = u1[:N, :M].astype(np.float32)
+    a2 = a0.copy()
+    fd_step_reference(a0, a1, a2, dt, dx, dy, wavespeed)
+    u2[1:N - 1, 1:M - 1] = a2[1:-1, 1:-1].astype(np.float16)
+    return u2
+
+
+def scaled_decode(q, s):
+    """``u = s[block] * q`` with one fp32 scale per 32 consecutive entries of a row."""
+    return (np.repeat(s, 32, axis=1) * q.astype(np.float32)).astype(np.float32)
+
+
+def scaled_step_reference(q0, s0, q1, s1, q2, s2, dt, dx, dy, wavespeed, N, M):
+    """``scaled_step`` (``step2D_wave.cu:128-183``): decode, fp32 slice-form update on the interior
+    (boundary columns and padding of interior rows encode zero), re-encode each 32-node block
+    with its largest magnitude m: ``q = rint(u * (32767/m))``, ``s = m / 32767``, with ``q = 0``
+    where ``32767/m`` is not finite (the reference divides by zero there).  Rows 0 and N-1 are
+    left untouched."""
+    u0 = scaled_decode(q0, s0)
+    u1 = scaled_decode(q1, s1)
+    u2 = np.zeros_like(u1)
+    fd_step_reference(u0[:N, :M], u1[:N, :M], u2[:N, :M], dt, dx, dy, wavespeed)
+    rows = slice(1, N - 1)
+    v = u2[rows]
+    nb = v.shape[1] // 32
+    blk = np.abs(v).reshape(v.shape[0], nb, 32).max(axis=2)
+    inv = _block_inverse(blk)
+    q = np.rint(v * np.repeat(inv, 32, axis=1)).astype(np.int16)
+    q2[rows] = q
+    s2[rows] = (blk / np.float32(32767.0)).astype(np.float32)
+    return q2, s2
+
+
+def scaled_encode(u, M):
+    """Block-scaled encoding of a node array (same rule as ``scaled_step_reference``), all rows."""
+    v = u.astype(np.float32)
+    nb = v.shape[1] // 32
+    blk = np.abs(v).reshape(v.shape[0], nb, 32).max(axis=2)
+    inv = _block_inverse(blk)
+    q = np.rint(v * np.repeat(inv, 32, axis=1)).astype(np.int16)
+    return q, (blk / np.float32(32767.0)).astype(np.float32)
+
+
+def _block_inverse(blk):
+    """``32767 / m`` in fp32, zero where it is not finite (all-zero or near-denormal blocks)."""
+    with np.errstate(divide="ignore", over="ignore"):
+        inv = (np.float32(32767.0) / blk.astype(np.float32)).astype(np.float32)
+    inv[~np.isfinite(inv)] = 0
+    return inv

@@ -136,3 +136,21 @@ def test_batched_gradient_is_sum_of_shot_gradients():
         c1 = dict(case, src=shots[s], sig=sig[s])
         acc += _plan_grad(c1, torch.float64, obs=np.zeros((1, 40, 6)))[3]
     assert rel_l2(total.cpu().numpy(), acc.cpu().numpy()) < 1e-13
+
+
+@pytest.mark.parametrize("variant,rows", [(0, 2), (1, 1), (1, 8)])
+def test_adjoint_launch_shapes_bitexact(variant, rows):
+    case = make_case((45, 300), 8, seed=29, nsteps=30, nrec=5)
+    from neuralmech_amd import solver
+
+    base = _plan_grad(case, torch.float32, seg=6)
+    g, med, src, rec = gpu_objects(case, torch.float32)
+    g.set_launch(variant, rows)
+    try:
+        plan = solver.GradientPlan(g, 30, seg=6)
+        tr = plan.forward(med, src, case["sig"], rec)
+        grad = plan.backward(med, src, rec, tr)
+        torch.cuda.synchronize()
+    finally:
+        g.set_launch(0, 0)
+    assert torch.equal(tr, base[1]) and torch.equal(grad, base[3])

@@ -114,8 +114,11 @@ def test_in_place_and_separate_output_agree():
     case = make_case((40, 50), 6, seed=8, nsteps=1, nsrc=0, nrec=0)
     g, med, _, _ = gpu_objects(case, torch.float32)
     rng = np.random.default_rng(1)
-    up = g.from_nodes(torch.as_tensor(rng.standard_normal(case["shape"])).float())
-    uc = g.from_nodes(torch.as_tensor(rng.standard_normal(case["shape"])).float())
+    a, b = rng.standard_normal((2, *case["shape"]))
+    for x in (a, b):  # boundary nodes hold zero by the layout contract
+        x[0, :] = x[-1, :] = x[:, 0] = x[:, -1] = 0.0
+    up = g.from_nodes(torch.as_tensor(a).float())
+    uc = g.from_nodes(torch.as_tensor(b).float())
     out = g.zeros()
     lib = _lib.load()
     import ctypes
@@ -159,3 +162,57 @@ def test_medium_check_reports_nonfinite():
     with pytest.raises(_lib.WaveError) as e:
         solver.Medium(g, velocity=v).validate()
     assert e.value.code == _lib.WFD_ENONFINITE
+
+
+SHAPES_2D = [(0, 1), (0, 2), (0, 4), (0, 8), (1, 1), (1, 2), (1, 4), (1, 8)]
+
+
+@pytest.mark.parametrize("variant,rows", SHAPES_2D)
+def test_every_2d_launch_shape_bitexact(variant, rows):
+    from neuralmech_amd import solver
+
+    print(f"\n[launch] 2D variant={variant} rows={rows}", flush=True)
+    for shape, order in (((61, 530), 8), ((70, 249), 4), ((33, 300), 2)):
+        case = make_case(shape, order, seed=7 + order, nsteps=25)
+        g, med, src, rec = gpu_objects(case, torch.float32)
+        g.set_launch(variant, rows)
+        assert g.launch() == (variant, rows)
+        try:
+            tr, (_, uc) = solver.run(g, med, 25, src=src, sig=case["sig"], rec=rec)
+            torch.cuda.synchronize()
+        finally:
+            g.set_launch(0, 0)
+        tr_o, _, uc_o = oracle_forward(case, np.float32)
+        assert np.array_equal(g.nodes(uc)[0].cpu().numpy(), uc_o), (shape, order)
+        assert np.array_equal(tr[0].cpu().numpy(), tr_o)
+
+
+@pytest.mark.parametrize("rows", [4, 8, 16, 64])
+def test_every_3d_launch_shape_bitexact(rows):
+    from neuralmech_amd import solver
+
+    case = make_case((30, 21, 70), 8, seed=31, nsteps=12)
+    g, med, src, rec = gpu_objects(case, torch.float32)
+    g.set_launch(0, rows)
+    try:
+        tr, (_, uc) = solver.run(g, med, 12, src=src, sig=case["sig"], rec=rec)
+        torch.cuda.synchronize()
+    finally:
+        g.set_launch(0, 0)
+    tr_o, _, uc_o = oracle_forward(case, np.float32)
+    assert np.array_equal(g.nodes(uc)[0].cpu().numpy(), uc_o)
+    assert np.array_equal(tr[0].cpu().numpy(), tr_o)
+
+
+def test_tuner_picks_a_valid_shape_and_results_do_not_change():
+    from neuralmech_amd import solver
+
+    case = make_case((257, 700), 8, seed=41, nsteps=20)
+    g, med, src, rec = gpu_objects(case, torch.float32)
+    info = g.tune(med, steps=6)
+    assert (info["variant"], info["rows"]) in SHAPES_2D and info["us_per_step"] > 0
+    tr, (_, uc) = solver.run(g, med, 20, src=src, sig=case["sig"], rec=rec)
+    torch.cuda.synchronize()
+    g.set_launch(0, 0)
+    tr_o, _, uc_o = oracle_forward(case, np.float32)
+    assert np.array_equal(g.nodes(uc)[0].cpu().numpy(), uc_o)
