@@ -115,6 +115,94 @@ def fwi_main(args):
     print(json.dumps(out))
 
 
+def copy_main(args):
+    """Device-to-device copy ceiling (the reference measures the same with a 1 GiB copy,
+    2D_wave_roofline.py:46-57), at 1 GiB (HBM) and 64 MiB (fits the Infinity Cache); also the
+    calibration run for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    out = {}
+    for label, nbytes in (("1GiB", 1 << 30), ("64MiB", 64 << 20)):
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        reps = 20 if nbytes >= (1 << 30) else 400
+        b.copy_(a)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        s = e0.elapsed_time(e1) / 1e3
+        out[label] = {"GB_per_s": round(2 * nbytes * reps / s / 1e9, 1), "bytes_read_per_copy": nbytes,
+                      "bytes_written_per_copy": nbytes, "copies": reps + 1}
+        del a, b
+    print(json.dumps({"metric": "device copy bandwidth", "unit": "GB/s", "results": out}))
+
+
+def volume_main(args):
+    """3D forward (and, with --grad, forward+adjoint) of 512^3 nodes, order 8, sponge 32, Ricker
+    8 Hz at (40, 256, 256), 128 x 128 receivers on plane 36."""
+    import numpy as np
+    import torch
+
+    from neuralmech_amd import media, solver
+
+    n, nsteps = args.n3d, args.steps3d
+    h = (10.0, 10.0, 10.0)
+    v = media.volume_3d_model(n, seed=5, device="cuda")
+    dt = 0.9 * solver.stable_dt(h, float(v.max()), 8)
+    g = solver.WaveGrid((n, n, n), h, dt, 8, torch.float32)
+    beta = [media.sponge_profile(n, 32, 0.05) for _ in range(3)]
+    med = solver.Medium(g, velocity=v, beta_profiles=beta)
+    del v
+    torch.cuda.empty_cache()
+    src = solver.PointSet(g, [(40, n // 2, n // 2)])
+    idx = 64 + 3 * np.arange(128)
+    idx = idx[idx < n - 1]
+    rec = solver.PointSet(g, [(36, int(i), int(j)) for i in idx for j in idx])
+    sig = media.ricker(np.arange(nsteps) * dt, 8.0) / (h[0] * h[1] * h[2])
+    tuned = g.tune(med, steps=6) if not args.no_tune else None
+    state = (g.zeros(), g.zeros())
+    solver.run(g, med, 4, src=src, sig=sig, rec=rec, state=state)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    traces, _ = solver.run(g, med, nsteps - 4, src=src, sig=sig, rec=rec, state=state, step0=4)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    steps = nsteps - 4
+    cells = g.interior_cells
+    ms_step = ms / steps
+    achieved = 16 * cells / (ms_step / 1e3) / 1e9
+    out = {
+        "metric": "Gcell-updates/s (3D forward, fp32)", "value": round(cells * steps / (ms / 1e3) / 1e9, 3),
+        "unit": "Gcell-updates/s", "n_gpus": 1, "steps": steps, "warmup": 4, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded smoothed random velocity)",
+        "config": {"workload": f"3D forward {n}^3 nodes, order 8 (25-point), 32-node sponge, Ricker 8 Hz, "
+                               f"{rec.count} receivers", "launch": {"rows": g.launch()[1], "tuned": tuned}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_cell": 16},
+        "finite": bool(torch.isfinite(traces).all()),
+    }
+    del state
+    if args.grad:
+        plan = solver.GradientPlan(g, nsteps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        tr = plan.forward(med, src, sig, rec)
+        gk = plan.backward(med, src, rec, tr)
+        torch.cuda.synchronize()
+        tg = time.perf_counter() - t0
+        out["gradient"] = {"seconds": round(tg, 3), "checkpoint_spacing": plan.seg, "checkpoints": plan.nckpt,
+                           "plan_GB": round(plan.bytes / 1e9, 2), "finite": bool(torch.isfinite(gk).all()),
+                           "bytes_per_cell_step": 60,
+                           "achieved_GB_per_s": round(60 * cells * nsteps / tg / 1e9, 1)}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,7 +211,11 @@ def main():
     ap.add_argument("--graph", type=int, default=0, help="replay a hipGraph of this many steps (even)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch shape")
-    ap.add_argument("--workload", choices=["forward", "fwi"], default="forward")
+    ap.add_argument("--workload", choices=["forward", "fwi", "copy", "3d"], default="forward")
+    ap.add_argument("--launch", default="", help="force VARIANT,ROWS instead of tuning")
+    ap.add_argument("--n3d", type=int, default=512)
+    ap.add_argument("--steps3d", type=int, default=2000)
+    ap.add_argument("--grad", action="store_true", help="3D: also time the checkpointed gradient")
     ap.add_argument("--fwi-steps", type=int, default=3000)
     ap.add_argument("--plain-shots", type=int, default=1)
     args = ap.parse_args()
@@ -134,6 +226,12 @@ def main():
     if args.workload == "fwi":
         fwi_main(args)
         return
+    if args.workload == "copy":
+        copy_main(args)
+        return
+    if args.workload == "3d":
+        volume_main(args)
+        return
 
     import torch
 
@@ -142,7 +240,11 @@ def main():
     nsteps = args.warmup + args.steps
     w = workload(nsteps=max(nsteps, 4000))
     g, med, src, rec = w["grid"], w["medium"], w["src"], w["rec"]
-    tuned = g.tune(med, steps=40) if not args.no_tune else None
+    tuned = None
+    if args.launch:
+        g.set_launch(*[int(x) for x in args.launch.split(",")])
+    elif not args.no_tune:
+        tuned = g.tune(med, steps=40)
     state = (g.zeros(), g.zeros())
     # warm-up steps advance the same shot; the timed steps continue it
     solver.run(g, med, args.warmup, src=src, sig=w["sig"], rec=rec, state=state, graph_chunk=args.graph)

@@ -49,6 +49,24 @@ struct StepArgs {
     const int64_t* step_dev;  // graph replay: the step index is *step_dev + step
 };
 
+// Block index remapped so that each of the 8 XCDs (whose L2 caches are separate) processes one
+// contiguous range of the grid's blocks.  The hardware hands block b (linear id, x fastest) to
+// XCD b % 8; giving that block the position start(b % 8) + b / 8 keeps neighbouring tiles -- and
+// the halo rows they share -- inside one L2.  A bijection for any grid size.
+struct BlockPos {
+    uint32_t x, y, z;
+};
+__device__ __forceinline__ BlockPos xcd_block()
+{
+    constexpr uint32_t XCDS = 8;
+    const uint32_t gx = gridDim.x, gy = gridDim.y;
+    const uint32_t nb = gx * gy * gridDim.z;
+    const uint32_t b = (blockIdx.z * gy + blockIdx.y) * gx + blockIdx.x;
+    const uint32_t per = nb / XCDS, rem = nb % XCDS, x = b % XCDS, i = b / XCDS;
+    const uint32_t n = x * per + (x < rem ? x : rem) + i;
+    return {n % gx, (n / gx) % gy, n / (gx * gy)};
+}
+
 struct LaunchCfg {
     int ri;    // rows (2D) or planes (3D) marched per wave / block
     int tile;  // 3D: tile variant

@@ -1,17 +1,26 @@
 // 1D / 2D leapfrog step kernels for gfx950 (wave64).
 //
-// Mapping: a lane owns a chunk of 4 consecutive nodes along the fastest axis (one 16-byte
-// vector load per field and row for fp32), a wave owns 256 consecutive columns of RI
-// consecutive rows and marches down them, keeping the 2K+RI rows of u_cur its stencil needs
-// in registers (each row of u_cur is loaded from memory once per wave).  The column
-// neighbours of the centre row come from the adjacent chunks (same cache lines the
-// neighbouring lanes load, so they are cache hits).  A block is 4 waves stacked along the
-// rows.  The halo layout makes every neighbour read an in-bounds element, so there are no
-// bounds checks on loads; stores mask the boundary ring and the row padding to zero.
+// Mapping: a lane owns a chunk of 4 consecutive nodes along the fastest axis (one 16-byte vector
+// load per field and row for fp32); a wave owns a run of chunks of one row and marches down RI
+// consecutive rows, keeping the 2K+RI rows of u_cur its stencil needs in registers (each row of
+// u_cur is loaded once per wave).  A block is 4 waves stacked along the rows, and blocks are
+// remapped so that each XCD sweeps one contiguous band of the grid.
+//
+// Two variants of the column halo:
+//   step2d_kernel     a wave owns 64 chunks (256 columns); the left / right neighbour chunks of
+//                     the centre row are loaded (cache hits on lines the neighbour lanes fetch);
+//   step2d_xl_kernel  waves overlap by one chunk each side (lane 0 and lane 63 only load), so the
+//                     neighbour chunks of lanes 1..62 are the centre chunks of lanes l-1 / l+1,
+//                     moved with cross-lane shuffles; 248 owned columns per wave.
+// Every lane stays active through the row loop (the point passes are wave-cooperative); lanes
+// that own no node compute discarded values and store nothing.  The halo layout makes every
+// neighbour read an in-bounds element; stores mask the boundary ring and row padding to zero.
+// Inside the sponge-free interior the damping terms are skipped (b = 0 makes them exact no-ops).
 //
 // The arithmetic is the update documented in wavefd.h, in exactly that operation order.
 #pragma once
 #include "wfd_internal.h"
+#include "wfd_points_dev.h"
 
 namespace wfd {
 
@@ -66,191 +75,6 @@ __device__ __forceinline__ T pick(const C4<T>& L, const C4<T>& C, const C4<T>& R
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-// First index in [lo, hi) of a sorted column list with col >= c.
-__device__ __forceinline__ int lower_bound_col(const int32_t* col, int lo, int hi, int64_t c)
-{
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (col[mid] < c)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
-// num[e] = fma(kc[e], sig, num[e]) for every source in this row that falls in the chunk,
-// in list order (the CSR keeps the caller's order within a node).
-template <typename T>
-__device__ __forceinline__ void inject4(const PtsDev<T>& p, int shot, int64_t row, int64_t c,
-                                        int64_t step, const T (&kc)[4], T (&num)[4])
-{
-    const int32_t* rp = p.row_ptr + (int64_t)shot * (p.nrows + 1);
-    const int lo = rp[row], hi = rp[row + 1];
-    if (lo >= hi) return;
-    const int32_t* col = p.col + (int64_t)shot * p.count;
-    const int32_t* id = p.id + (int64_t)shot * p.count;
-    const T* sig = p.data + (int64_t)shot * p.shot_stride + step * p.width;
-    for (int q = lower_bound_col(col, lo, hi, c); q < hi; q++) {
-        const int64_t cc = col[q];
-        if (cc >= c + 4) break;
-        const T s = sig[id[q]];
-#pragma unroll
-        for (int e = 0; e < 4; e++)
-            if (cc == c + e) num[e] = fmaT(kc[e], s, num[e]);
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void record4(const PtsDev<T>& p, int shot, int64_t row, int64_t c,
-                                        int64_t step, const T (&val)[4])
-{
-    const int32_t* rp = p.row_ptr + (int64_t)shot * (p.nrows + 1);
-    const int lo = rp[row], hi = rp[row + 1];
-    if (lo >= hi) return;
-    const int32_t* col = p.col + (int64_t)shot * p.count;
-    const int32_t* id = p.id + (int64_t)shot * p.count;
-    T* tr = p.data + (int64_t)shot * p.shot_stride + step * p.width;
-    for (int q = lower_bound_col(col, lo, hi, c); q < hi; q++) {
-        const int64_t cc = col[q];
-        if (cc >= c + 4) break;
-#pragma unroll
-        for (int e = 0; e < 4; e++)
-            if (cc == c + e) tr[id[q]] = val[e];
-    }
-}
-
-// Laplacian of chunk element e: centre row `q` of window `win`, column chunks L / R.
-template <typename T, int K, bool VROW>
-__device__ __forceinline__ T lap_elem(const T wc, const T* wr, const T* wcol, const C4<T>* win,
-                                      int q, const C4<T>& L, const C4<T>& R, int e)
-{
-    constexpr int KR = VROW ? K : 0;
-    const C4<T>& C = win[q + KR];
-    T lap = wc * C.e[e];
-    if constexpr (VROW) {
-#pragma unroll
-        for (int m = 1; m <= K; m++) lap = fmaT(wr[m - 1], win[q + KR - m].e[e] + win[q + KR + m].e[e], lap);
-    }
-#pragma unroll
-    for (int m = 1; m <= K; m++) lap = fmaT(wcol[m - 1], pick(L, C, R, e - m) + pick(L, C, R, e + m), lap);
-    return lap;
-}
-
-template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI>
-__global__ __launch_bounds__(256) void step2d_kernel(const StepArgs<T> a)
-{
-    constexpr int KR = VROW ? K : 0;
-    constexpr int NW = RI + 2 * KR;
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int shot = blockIdx.z;
-    const int64_t c = ((int64_t)blockIdx.x * 64 + lane) * 4;
-    const int64_t r0 = a.row_lo + ((int64_t)blockIdx.y * 4 + wave) * RI;
-    if (r0 >= a.row_hi || c >= a.nc) return;
-
-    const int64_t sb = (int64_t)shot * a.shot_stride;
-    const T* __restrict__ uc = a.uc + sb;
-    const T* up = a.up + sb;  // may alias un (in-place leapfrog): same element, same lane
-    T* un = a.un + sb;
-    const T* __restrict__ coef = VARC ? a.coef + (int64_t)shot * a.coef_shot_stride : nullptr;
-    const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
-
-    C4<T> win[NW];
-#pragma unroll
-    for (int t = 0; t < NW; t++) {
-        const int64_t r = r0 - KR + t;
-        win[t] = (r < a.row_hi + KR) ? ld4(uc + r * a.ld + c) : zero4<T>();
-    }
-    C4<T> wf[IMG ? NW : 1];
-    if constexpr (IMG) {
-        const T* __restrict__ uf = a.uf + sb;
-#pragma unroll
-        for (int t = 0; t < NW; t++) {
-            const int64_t r = r0 - KR + t;
-            wf[t] = (r < a.row_hi + KR) ? ld4(uf + r * a.ld + c) : zero4<T>();
-        }
-    }
-
-    bool valid[4];
-    T bc[4];
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-        valid[e] = (c + e >= 1) && (c + e <= a.nc - 2);
-        bc[e] = T(0);
-        if constexpr (SPONGE) bc[e] = (c + e < a.nc) ? a.beta_c[c + e] : T(0);
-    }
-
-#pragma unroll
-    for (int q = 0; q < RI; q++) {
-        const int64_t r = r0 + q;
-        if (r >= a.row_hi) break;
-        const int64_t o = r * a.ld + c;
-        const C4<T> L = ld4(uc + o - 4);
-        const C4<T> R = ld4(uc + o + 4);
-        const C4<T> P = ld4(up + o);
-        T kc[4];
-        if constexpr (VARC) {
-            const C4<T> kv = ld4(coef + o);
-#pragma unroll
-            for (int e = 0; e < 4; e++) kc[e] = kv.e[e];
-        } else {
-#pragma unroll
-            for (int e = 0; e < 4; e++) kc[e] = a.kconst;
-        }
-        T b[4];
-        if constexpr (SPONGE) {
-            const T br = VROW ? a.beta_r[r] : T(0);
-#pragma unroll
-            for (int e = 0; e < 4; e++) b[e] = VROW ? br + bc[e] : bc[e];
-        }
-        T val[4];
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const T ctr = win[q + KR].e[e];
-            const T lap = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, win, q, L, R, e);
-            T t;
-            if constexpr (SPONGE)
-                t = T(2) * ctr - (T(1) - b[e]) * P.e[e];
-            else
-                t = T(2) * ctr - P.e[e];
-            val[e] = fmaT(kc[e], lap, t);
-        }
-        const int64_t prow = VROW ? r : 0;  // CSR row of the point lists
-        if (a.src.count) inject4(a.src, shot, prow, c, step, kc, val);
-        if constexpr (SPONGE) {
-#pragma unroll
-            for (int e = 0; e < 4; e++) val[e] = val[e] / (T(1) + b[e]);
-        }
-#pragma unroll
-        for (int e = 0; e < 4; e++)
-            if (!valid[e]) val[e] = T(0);
-        st4(un + o, val);
-        if (a.rec.count) record4(a.rec, shot, prow, c, step, val);
-        if constexpr (IMG) {
-            const T* __restrict__ uf = a.uf + sb;
-            T* gp = a.g + sb;
-            const C4<T> Lf = ld4(uf + o - 4);
-            const C4<T> Rf = ld4(uf + o + 4);
-            const C4<T> G = ld4(gp + o);
-            T gv[4];
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const T lapu = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, wf, q, Lf, Rf, e);
-                gv[e] = valid[e] ? fmaT(val[e], lapu, G.e[e]) : T(0);
-            }
-            st4(gp + o, gv);
-        }
-    }
-}
-
-// Variant with the horizontal halo exchanged between lanes: the waves of a row band overlap by
-// one chunk on each side (lane 0 and lane 63 only load), so the left / right neighbour chunks of
-// an owning lane are the centre chunks of lanes l-1 / l+1, moved with cross-lane shuffles instead
-// of extra cache loads.  A wave owns 62 chunks = 248 columns.  Inside the sponge-free interior
-// the damping terms are skipped (b = 0 makes them exact no-ops, so results are identical).
-constexpr int XL_COLS = 248;
-
 template <typename T>
 __device__ __forceinline__ C4<T> from_lane_left(const C4<T>& v)
 {
@@ -268,18 +92,42 @@ __device__ __forceinline__ C4<T> from_lane_right(const C4<T>& v)
     return r;
 }
 
-template <typename T, int K, bool VARC, bool SPONGE, bool IMG, int RI>
-__global__ __launch_bounds__(256) void step2d_xl_kernel(const StepArgs<T> a)
+// Laplacian of chunk element e: centre row `q` of window `win`, column chunks L / R.
+template <typename T, int K, bool VROW>
+__device__ __forceinline__ T lap_elem(const T wc, const T* wr, const T* wcol, const C4<T>* win, int q,
+                                      const C4<T>& L, const C4<T>& R, int e)
 {
-    constexpr int NW = RI + 2 * K;
+    constexpr int KR = VROW ? K : 0;
+    const C4<T>& C = win[q + KR];
+    T lap = wc * C.e[e];
+    if constexpr (VROW) {
+#pragma unroll
+        for (int m = 1; m <= K; m++) lap = fmaT(wr[m - 1], win[q + KR - m].e[e] + win[q + KR + m].e[e], lap);
+    }
+#pragma unroll
+    for (int m = 1; m <= K; m++) lap = fmaT(wcol[m - 1], pick(L, C, R, e - m) + pick(L, C, R, e + m), lap);
+    return lap;
+}
+
+// Shared body of both variants.  XL selects the lane-exchange halo; VROW = false is the 1D case
+// (a single row, no vertical stencil).
+template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL>
+__device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
+{
+    constexpr int KR = VROW ? K : 0;
+    constexpr int NW = RI + 2 * KR;
+    constexpr int64_t COLS = XL ? 248 : 256;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int shot = blockIdx.z;
-    const int64_t c = (int64_t)blockIdx.x * XL_COLS + lane * 4 - 4;
-    const int64_t r0 = a.row_lo + ((int64_t)blockIdx.y * 4 + wave) * RI;
+    const BlockPos bp = xcd_block();
+    const int shot = bp.z;
+    const int64_t base = (int64_t)bp.x * COLS - (XL ? 4 : 0);  // column of lane 0's chunk
+    const int64_t c = base + lane * 4;
+    const int64_t r0 = a.row_lo + ((int64_t)bp.y * 4 + wave) * RI;
     if (r0 >= a.row_hi) return;  // whole wave
-    const bool load_ok = c < a.ld;  // lanes beyond the row load nothing (no wrap into other rows)
-    const bool own = lane >= 1 && lane <= 62 && c < a.nc;
+    const int lane_lo = XL ? 1 : 0, lane_hi = XL ? 62 : 63;
+    const bool load_ok = c < a.ld;  // no wrap of a lane's chunk into the next row
+    const bool own = lane >= lane_lo && lane <= lane_hi && c < a.nc;
 
     const int64_t sb = (int64_t)shot * a.shot_stride;
     const T* __restrict__ uc = a.uc + sb;
@@ -291,24 +139,32 @@ __global__ __launch_bounds__(256) void step2d_xl_kernel(const StepArgs<T> a)
     C4<T> win[NW];
 #pragma unroll
     for (int t = 0; t < NW; t++) {
-        const int64_t r = r0 - K + t;
-        win[t] = (r < a.row_hi + K && load_ok) ? ld4(uc + r * a.ld + c) : zero4<T>();
+        const int64_t r = r0 - KR + t;
+        win[t] = (r < a.row_hi + KR && load_ok) ? ld4(uc + r * a.ld + c) : zero4<T>();
     }
     C4<T> wf[IMG ? NW : 1];
     if constexpr (IMG) {
         const T* __restrict__ uf = a.uf + sb;
 #pragma unroll
         for (int t = 0; t < NW; t++) {
-            const int64_t r = r0 - K + t;
-            wf[t] = (r < a.row_hi + K && load_ok) ? ld4(uf + r * a.ld + c) : zero4<T>();
+            const int64_t r = r0 - KR + t;
+            wf[t] = (r < a.row_hi + KR && load_ok) ? ld4(uf + r * a.ld + c) : zero4<T>();
         }
     }
+
+    // does any point list have an entry in this wave's rows? (one range check per wave)
+    const int64_t rlast = VROW ? (r0 + RI < a.row_hi ? r0 + RI : a.row_hi) : 1;
+    const int64_t rfirst = VROW ? r0 : 0;
+    const bool src_here = a.src.count &&
+        a.src.row_ptr[(int64_t)shot * (a.src.nrows + 1) + rfirst] != a.src.row_ptr[(int64_t)shot * (a.src.nrows + 1) + rlast];
+    const bool rec_here = a.rec.count &&
+        a.rec.row_ptr[(int64_t)shot * (a.rec.nrows + 1) + rfirst] != a.rec.row_ptr[(int64_t)shot * (a.rec.nrows + 1) + rlast];
 
     bool valid[4];
     T bc[4];
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-        valid[e] = (c + e >= 1) && (c + e <= a.nc - 2);
+        valid[e] = own && (c + e >= 1) && (c + e <= a.nc - 2);
         bc[e] = T(0);
         if constexpr (SPONGE) bc[e] = (c + e >= 0 && c + e < a.nc) ? a.beta_c[c + e] : T(0);
     }
@@ -316,20 +172,29 @@ __global__ __launch_bounds__(256) void step2d_xl_kernel(const StepArgs<T> a)
 #pragma unroll
     for (int q = 0; q < RI; q++) {
         const int64_t r = r0 + q;
-        if (r >= a.row_hi) break;
+        if (r >= a.row_hi) break;  // whole wave
         const int64_t o = r * a.ld + c;
-        const C4<T> L = from_lane_left(win[q + K]);
-        const C4<T> R = from_lane_right(win[q + K]);
-        C4<T> Lf, Rf;
-        if constexpr (IMG) {
-            Lf = from_lane_left(wf[q + K]);
-            Rf = from_lane_right(wf[q + K]);
+        C4<T> L, R, Lf, Rf;
+        if constexpr (XL) {
+            L = from_lane_left(win[q + KR]);
+            R = from_lane_right(win[q + KR]);
+            if constexpr (IMG) {
+                Lf = from_lane_left(wf[q + KR]);
+                Rf = from_lane_right(wf[q + KR]);
+            }
+        } else {
+            L = load_ok ? ld4(uc + o - 4) : zero4<T>();
+            R = load_ok ? ld4(uc + o + 4) : zero4<T>();
+            if constexpr (IMG) {
+                const T* __restrict__ uf = a.uf + sb;
+                Lf = load_ok ? ld4(uf + o - 4) : zero4<T>();
+                Rf = load_ok ? ld4(uf + o + 4) : zero4<T>();
+            }
         }
-        if (!own) continue;
-        const C4<T> P = ld4(up + o);
+        const C4<T> P = own ? ld4(up + o) : zero4<T>();
         T kc[4];
         if constexpr (VARC) {
-            const C4<T> kv = ld4(coef + o);
+            const C4<T> kv = own ? ld4(coef + o) : zero4<T>();
 #pragma unroll
             for (int e = 0; e < 4; e++) kc[e] = kv.e[e];
         } else {
@@ -339,22 +204,23 @@ __global__ __launch_bounds__(256) void step2d_xl_kernel(const StepArgs<T> a)
         T b[4] = {T(0), T(0), T(0), T(0)};
         bool damp = false;
         if constexpr (SPONGE) {
-            const T br = a.beta_r[r];
+            const T br = VROW ? a.beta_r[r] : T(0);
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                b[e] = br + bc[e];
+                b[e] = VROW ? br + bc[e] : bc[e];
                 damp |= b[e] != T(0);
             }
         }
         T val[4];
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            const T ctr = win[q + K].e[e];
-            const T lap = lap_elem<T, K, true>(a.wc, a.wr, a.wcol, win, q, L, R, e);
+            const T ctr = win[q + KR].e[e];
+            const T lap = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, win, q, L, R, e);
             const T t = damp ? T(2) * ctr - (T(1) - b[e]) * P.e[e] : T(2) * ctr - P.e[e];
             val[e] = fmaT(kc[e], lap, t);
         }
-        if (a.src.count) inject4(a.src, shot, r, c, step, kc, val);
+        const int64_t prow = VROW ? r : 0;  // CSR row of the point lists
+        if (src_here) wave_inject(a.src, shot, prow, base, lane_lo, lane_hi, step, lane, kc, val);
         if (damp) {
 #pragma unroll
             for (int e = 0; e < 4; e++) val[e] = val[e] / (T(1) + b[e]);
@@ -362,20 +228,35 @@ __global__ __launch_bounds__(256) void step2d_xl_kernel(const StepArgs<T> a)
 #pragma unroll
         for (int e = 0; e < 4; e++)
             if (!valid[e]) val[e] = T(0);
-        st4(un + o, val);
-        if (a.rec.count) record4(a.rec, shot, r, c, step, val);
+        if (own) st4(un + o, val);
+        if (rec_here) wave_record(a.rec, shot, prow, base, lane_lo, lane_hi, step, lane, val);
         if constexpr (IMG) {
-            T* gp = a.g + sb;
-            const C4<T> G = ld4(gp + o);
             T gv[4];
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const T lapu = lap_elem<T, K, true>(a.wc, a.wr, a.wcol, wf, q, Lf, Rf, e);
-                gv[e] = valid[e] ? fmaT(val[e], lapu, G.e[e]) : T(0);
+            for (int e = 0; e < 4; e++) gv[e] = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, wf, q, Lf, Rf, e);
+            if (own) {
+                T* gp = a.g + sb;
+                const C4<T> G = ld4(gp + o);
+#pragma unroll
+                for (int e = 0; e < 4; e++) gv[e] = valid[e] ? fmaT(val[e], gv[e], G.e[e]) : T(0);
+                st4(gp + o, gv);
             }
-            st4(gp + o, gv);
         }
     }
+}
+
+template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI>
+__global__ __launch_bounds__(256) void step2d_kernel(const StepArgs<T> a)
+{
+    step2d_body<T, K, VROW, VARC, SPONGE, IMG, RI, false>(a);
+}
+
+constexpr int XL_COLS = 248;
+
+template <typename T, int K, bool VARC, bool SPONGE, bool IMG, int RI>
+__global__ __launch_bounds__(256) void step2d_xl_kernel(const StepArgs<T> a)
+{
+    step2d_body<T, K, true, VARC, SPONGE, IMG, RI, true>(a);
 }
 
 // ------------------------------------------------------------------------------ dispatch
@@ -419,8 +300,8 @@ static hipError_t launch2d_k(const StepArgs<T>& a, bool vrow, int variant, int r
 }
 
 template <typename T, int K>
-static hipError_t launch2d_flags(const StepArgs<T>& a, bool vrow, bool varc, bool sponge, bool img,
-                                 int variant, int ri, int nshots, hipStream_t s)
+static hipError_t launch2d_flags(const StepArgs<T>& a, bool vrow, bool varc, bool sponge, bool img, int variant,
+                                 int ri, int nshots, hipStream_t s)
 {
     const int f = (varc ? 4 : 0) | (sponge ? 2 : 0) | (img ? 1 : 0);
     switch (f) {
@@ -436,8 +317,8 @@ static hipError_t launch2d_flags(const StepArgs<T>& a, bool vrow, bool varc, boo
 }
 
 template <typename T>
-hipError_t launch_step2d(const StepArgs<T>& a, bool vrow, int order, bool varc, bool sponge, bool img,
-                         int variant, int ri, int nshots, hipStream_t s)
+hipError_t launch_step2d(const StepArgs<T>& a, bool vrow, int order, bool varc, bool sponge, bool img, int variant,
+                         int ri, int nshots, hipStream_t s)
 {
     switch (order) {
         case 2: return launch2d_flags<T, 1>(a, vrow, varc, sponge, img, variant, ri, nshots, s);

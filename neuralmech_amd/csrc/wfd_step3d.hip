@@ -8,6 +8,7 @@
 // 4-node halo on the four in-plane sides, is staged in LDS (double-buffered, one barrier per
 // plane) and supplies the in-plane neighbours.  Arithmetic follows wavefd.h exactly.
 #include "wfd_internal.h"
+#include "wfd_points_dev.h"
 
 namespace wfd {
 
@@ -47,57 +48,6 @@ __device__ __forceinline__ void gst4<double>(double* p, const double (&v)[4])
 
 __device__ __forceinline__ float fma3(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma3(double a, double b, double c) { return __builtin_fma(a, b, c); }
-
-__device__ __forceinline__ int lb_col(const int32_t* col, int lo, int hi, int64_t c)
-{
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (col[mid] < c)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
-template <typename T>
-__device__ __forceinline__ void inject_chunk(const PtsDev<T>& p, int shot, int64_t row, int64_t c,
-                                             int64_t step, const T (&kc)[4], T (&num)[4])
-{
-    const int32_t* rp = p.row_ptr + (int64_t)shot * (p.nrows + 1);
-    const int lo = rp[row], hi = rp[row + 1];
-    if (lo >= hi) return;
-    const int32_t* col = p.col + (int64_t)shot * p.count;
-    const int32_t* id = p.id + (int64_t)shot * p.count;
-    const T* sig = p.data + (int64_t)shot * p.shot_stride + step * p.width;
-    for (int q = lb_col(col, lo, hi, c); q < hi; q++) {
-        const int64_t cc = col[q];
-        if (cc >= c + 4) break;
-        const T s = sig[id[q]];
-#pragma unroll
-        for (int e = 0; e < 4; e++)
-            if (cc == c + e) num[e] = fma3(kc[e], s, num[e]);
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void record_chunk(const PtsDev<T>& p, int shot, int64_t row, int64_t c,
-                                             int64_t step, const T (&val)[4])
-{
-    const int32_t* rp = p.row_ptr + (int64_t)shot * (p.nrows + 1);
-    const int lo = rp[row], hi = rp[row + 1];
-    if (lo >= hi) return;
-    const int32_t* col = p.col + (int64_t)shot * p.count;
-    const int32_t* id = p.id + (int64_t)shot * p.count;
-    T* tr = p.data + (int64_t)shot * p.shot_stride + step * p.width;
-    for (int q = lb_col(col, lo, hi, c); q < hi; q++) {
-        const int64_t cc = col[q];
-        if (cc >= c + 4) break;
-#pragma unroll
-        for (int e = 0; e < 4; e++)
-            if (cc == c + e) tr[id[q]] = val[e];
-    }
-}
 
 constexpr int T3_TI = 16;  // tile rows
 constexpr int T3_TJ = 64;  // tile columns
@@ -153,10 +103,13 @@ __global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk
 {
     __shared__ T tile[IMG ? 2 : 1][2][T3_LH][T3_LW];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int shot = blockIdx.z / nkb, kb = blockIdx.z % nkb;
-    const int64_t j0 = (int64_t)blockIdx.x * T3_TJ;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const BlockPos bp = xcd_block();
+    const int shot = bp.z / nkb, kb = bp.z % nkb;
+    const int64_t j0 = (int64_t)bp.x * T3_TJ;
     const int64_t c = j0 + tx * 4;
-    const int64_t i0 = 1 + (int64_t)blockIdx.y * T3_TI;
+    const int64_t i0 = 1 + (int64_t)bp.y * T3_TI;
     const int64_t i = i0 + ty;
     const int64_t k0 = 1 + (int64_t)kb * rk;
     const int64_t k1 = min(k0 + rk, a.np - 1);
@@ -173,21 +126,22 @@ __global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk
     const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
     const int64_t ld = a.ld, ps = a.ps;
     const int64_t rowoff = i * ld + c;
+    const V4<T> zero = {{T(0), T(0), T(0), T(0)}};
 
     V4<T> win[2 * K + 1];
     V4<T> wf[IMG ? 2 * K + 1 : 1];
 #pragma unroll
     for (int t = 1; t <= 2 * K; t++) {
         const int64_t k = k0 - K - 1 + t;
-        win[t] = load_row ? gld4(uc + k * ps + rowoff) : V4<T>{{T(0), T(0), T(0), T(0)}};
-        if constexpr (IMG) wf[t] = load_row ? gld4(uf + k * ps + rowoff) : V4<T>{{T(0), T(0), T(0), T(0)}};
+        win[t] = load_row ? gld4(uc + k * ps + rowoff) : zero;
+        if constexpr (IMG) wf[t] = load_row ? gld4(uf + k * ps + rowoff) : zero;
     }
 
     bool valid[4];
     T bc[4];
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-        valid[e] = (c + e >= 1) && (c + e <= a.nc - 2);
+        valid[e] = out && (c + e >= 1) && (c + e <= a.nc - 2);
         bc[e] = T(0);
         if constexpr (SPONGE) bc[e] = (c + e < a.nc) ? a.beta_c[c + e] : T(0);
     }
@@ -198,65 +152,83 @@ __global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk
     for (int64_t k = k0; k < k1; k++, buf ^= 1) {
 #pragma unroll
         for (int t = 0; t < 2 * K; t++) win[t] = win[t + 1];
-        win[2 * K] = load_row ? gld4(uc + (k + K) * ps + rowoff) : V4<T>{{T(0), T(0), T(0), T(0)}};
+        win[2 * K] = load_row ? gld4(uc + (k + K) * ps + rowoff) : zero;
         if constexpr (IMG) {
 #pragma unroll
             for (int t = 0; t < 2 * K; t++) wf[t] = wf[t + 1];
-            wf[2 * K] = load_row ? gld4(uf + (k + K) * ps + rowoff) : V4<T>{{T(0), T(0), T(0), T(0)}};
+            wf[2 * K] = load_row ? gld4(uf + (k + K) * ps + rowoff) : zero;
         }
         stage_plane<T>(tile[0][buf], uc, win[K], k, i0, j0, ty, tx, a.nr, ld, ps);
         if constexpr (IMG) stage_plane<T>(tile[IMG ? 1 : 0][buf], uf, wf[K], k, i0, j0, ty, tx, a.nr, ld, ps);
         __syncthreads();
-        if (!out) continue;
         const int64_t o = k * ps + rowoff;
-        const V4<T> P = gld4(up + o);
+        const V4<T> P = out ? gld4(up + o) : zero;
         T kc[4];
         if constexpr (VARC) {
-            const V4<T> kv = gld4(coef + o);
+            const V4<T> kv = out ? gld4(coef + o) : zero;
 #pragma unroll
             for (int e = 0; e < 4; e++) kc[e] = kv.e[e];
         } else {
 #pragma unroll
             for (int e = 0; e < 4; e++) kc[e] = a.kconst;
         }
-        T b[4];
+        T b[4] = {T(0), T(0), T(0), T(0)};
+        bool damp = false;
         if constexpr (SPONGE) {
             const T bpr = a.beta_p[k] + br;
 #pragma unroll
-            for (int e = 0; e < 4; e++) b[e] = bpr + bc[e];
+            for (int e = 0; e < 4; e++) {
+                b[e] = bpr + bc[e];
+                damp |= b[e] != T(0);
+            }
         }
         T val[4];
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const T ctr = win[K].e[e];
             const T lap = lap3<T, K>(a.wc, a.wp, a.wr, a.wcol, win, tile[0][buf], ty, tx, e);
-            T t;
-            if constexpr (SPONGE)
-                t = T(2) * ctr - (T(1) - b[e]) * P.e[e];
-            else
-                t = T(2) * ctr - P.e[e];
+            const T t = damp ? T(2) * ctr - (T(1) - b[e]) * P.e[e] : T(2) * ctr - P.e[e];
             val[e] = fma3(kc[e], lap, t);
         }
-        const int64_t prow = k * a.nr + i;
-        if (a.src.count) inject_chunk(a.src, shot, prow, c, step, kc, val);
-        if constexpr (SPONGE) {
+        // point passes: the wave holds 4 rows of the tile, 16 lanes each; one range check per
+        // plane decides whether any of them has points
+        const int64_t wr0 = min(i0 + 4 * wave, a.nr - 1), wr1 = min(i0 + 4 * wave + 4, a.nr - 1);
+        const bool src_here = a.src.count && a.src.row_ptr[(int64_t)shot * (a.src.nrows + 1) + k * a.nr + wr0] !=
+                                                 a.src.row_ptr[(int64_t)shot * (a.src.nrows + 1) + k * a.nr + wr1];
+        const bool rec_here = a.rec.count && a.rec.row_ptr[(int64_t)shot * (a.rec.nrows + 1) + k * a.nr + wr0] !=
+                                                 a.rec.row_ptr[(int64_t)shot * (a.rec.nrows + 1) + k * a.nr + wr1];
+        if (src_here) {
+            for (int rr = 0; rr < 4; rr++) {
+                const int64_t ir = i0 + 4 * wave + rr;
+                if (ir <= a.nr - 2)
+                    wave_inject(a.src, shot, k * a.nr + ir, j0 - 64 * rr, 16 * rr, 16 * rr + 15, step, lane, kc, val);
+            }
+        }
+        if (damp) {
 #pragma unroll
             for (int e = 0; e < 4; e++) val[e] = val[e] / (T(1) + b[e]);
         }
 #pragma unroll
         for (int e = 0; e < 4; e++)
             if (!valid[e]) val[e] = T(0);
-        gst4(un + o, val);
-        if (a.rec.count) record_chunk(a.rec, shot, prow, c, step, val);
+        if (out) gst4(un + o, val);
+        if (rec_here) {
+            for (int rr = 0; rr < 4; rr++) {
+                const int64_t ir = i0 + 4 * wave + rr;
+                if (ir <= a.nr - 2)
+                    wave_record(a.rec, shot, k * a.nr + ir, j0 - 64 * rr, 16 * rr, 16 * rr + 15, step, lane, val);
+            }
+        }
         if constexpr (IMG) {
-            const V4<T> G = gld4(gp + o);
             T gv[4];
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const T lapu = lap3<T, K>(a.wc, a.wp, a.wr, a.wcol, wf, tile[IMG ? 1 : 0][buf], ty, tx, e);
-                gv[e] = valid[e] ? fma3(val[e], lapu, G.e[e]) : T(0);
+            for (int e = 0; e < 4; e++) gv[e] = lap3<T, K>(a.wc, a.wp, a.wr, a.wcol, wf, tile[IMG ? 1 : 0][buf], ty, tx, e);
+            if (out) {
+                const V4<T> G = gld4(gp + o);
+#pragma unroll
+                for (int e = 0; e < 4; e++) gv[e] = valid[e] ? fma3(val[e], gv[e], G.e[e]) : T(0);
+                gst4(gp + o, gv);
             }
-            gst4(gp + o, gv);
         }
     }
 }

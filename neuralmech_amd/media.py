@@ -43,9 +43,28 @@ def layered_fwi_models(n=1024, seed=3):
     return true, start
 
 
-def volume_3d_model(n=512, seed=5) -> np.ndarray:
-    """3D model: smoothed random field (sigma 20) on [1500, 4500] m/s."""
-    return smoothed_random((n, n, n), seed, 20.0, 1500.0, 4500.0)
+def volume_3d_model(n=512, seed=5, device=None):
+    """3D model: white noise from ``default_rng(seed)``, Gaussian-smoothed with sigma 20 cells
+    (periodic, by FFT on the GPU when one is given, so a 512^3 model takes seconds), min-max
+    mapped to [1500, 4500] m/s.  Returns a float64 torch tensor on ``device`` (CPU if None)."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    dev = torch.device(device) if device is not None else torch.device("cpu")
+    noise = torch.from_numpy(rng.standard_normal((n, n, n))).to(dev)
+    k = [torch.fft.fftfreq(n, device=dev, dtype=torch.float64) for _ in range(2)] + [
+        torch.fft.rfftfreq(n, device=dev, dtype=torch.float64)]
+    sigma = 20.0
+    spec = torch.fft.rfftn(noise)
+    del noise
+    for axis, kk in enumerate(k):
+        shape = [1, 1, 1]
+        shape[axis] = kk.numel()
+        spec *= torch.exp(-2.0 * (math.pi * sigma * kk) ** 2).reshape(shape)
+    p = torch.fft.irfftn(spec, s=(n, n, n))
+    del spec
+    lo, hi = p.min(), p.max()
+    return 1500.0 + 3000.0 * (p - lo) / (hi - lo)
 
 
 def ricker(t, f0: float, t0: float | None = None) -> np.ndarray:
