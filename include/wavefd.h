@@ -79,16 +79,21 @@ typedef struct wfd_medium {
 /* Point list (sources or receivers) in device memory, CSR over rows: a row is every axis but the
  * fastest (n[0] rows in 2D, n[0]*n[1] in 3D, 1 in 1D).  Build it with wfd_points_csr.
  * `data` is the signal (sources: read) or the traces (receivers: written), per shot
- * [nsteps][width], shots `shot_stride` elements apart, dtype of the grid. */
+ * [nsteps][width], shots `shot_stride` elements apart, dtype of the grid.  Receivers are written
+ * by gather blocks that read the field a launch receives as its current level, i.e. one launch
+ * after the step that produced it, plus a final gather at the end of each call: a call's traces
+ * are complete when the call's work has finished on the stream. */
 typedef struct wfd_points {
     int32_t count;          /* points per shot (0 = none) */
     int32_t nrows;          /* CSR rows per shot */
-    const int32_t* row_ptr; /* [nshots][nrows + 1], offsets into this shot's col/id */
+    const int32_t* row_ptr; /* [nshots][nrows + 1], offsets into this shot's col/id/row */
     const int32_t* col;     /* [nshots][count], fastest-axis index, ascending within a row */
     const int32_t* id;      /* [nshots][count], column of the point in `data` */
+    const int32_t* row;     /* [nshots][count], CSR row of the point (needed for receivers) */
     void* data;
-    int64_t width;          /* entries per time step in data */
-    int64_t shot_stride;    /* elements between shots in data */
+    int64_t width;          /* entries per time step in data (>= count) */
+    int64_t nsteps;         /* time rows of data per shot */
+    int64_t shot_stride;    /* elements between shots in data (>= nsteps * width) */
 } wfd_points;
 
 /* --------------------------------------------------------------------------- host-only */
@@ -111,11 +116,11 @@ double wfd_stable_dt(int ndim, const double* h, double vmax, int order);
 int wfd_stencil_weights(const wfd_grid* g, double* wc, double* w);
 
 /* Sort `npts` logical flat node indices (C order over n) into a CSR point list of this grid.
- * Host arrays: row_ptr[nrows+1], col[npts], id[npts] (id = the point's position in `nodes`).
- * Points must be interior nodes (WFD_EINVAL otherwise).  Replaces cuwave's point_source /
- * Sensors node lookup (projects/18_fwi/plate.py:51). */
+ * Host arrays: row_ptr[nrows+1], col[npts], id[npts] (id = the point's position in `nodes`),
+ * row[npts].  Points must be interior nodes (WFD_EINVAL otherwise); points sharing a node keep
+ * their order.  Replaces cuwave's point_source / Sensors node lookup (projects/18_fwi/plate.py:51). */
 int wfd_points_csr(const wfd_grid* g, int32_t npts, const int64_t* nodes, int32_t* row_ptr,
-                   int32_t* col, int32_t* id);
+                   int32_t* col, int32_t* id, int32_t* row);
 int32_t wfd_points_nrows(const wfd_grid* g);
 
 /* ------------------------------------------------------------------------------ device */

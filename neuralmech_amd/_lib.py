@@ -57,8 +57,10 @@ class Points(ctypes.Structure):
         ("row_ptr", ctypes.c_void_p),
         ("col", ctypes.c_void_p),
         ("id", ctypes.c_void_p),
+        ("row", ctypes.c_void_p),
         ("data", ctypes.c_void_p),
         ("width", ctypes.c_int64),
+        ("nsteps", ctypes.c_int64),
         ("shot_stride", ctypes.c_int64),
     ]
 
@@ -79,7 +81,7 @@ SIGNATURES = {
                                      ctypes.c_int]),
     "wfd_stable_dt": (ctypes.c_double, [ctypes.c_int, _DP, ctypes.c_double, ctypes.c_int]),
     "wfd_stencil_weights": (ctypes.c_int, [_GP, _DP, _DP]),
-    "wfd_points_csr": (ctypes.c_int, [_GP, ctypes.c_int32, _I64P, _I32P, _I32P, _I32P]),
+    "wfd_points_csr": (ctypes.c_int, [_GP, ctypes.c_int32, _I64P, _I32P, _I32P, _I32P, _I32P]),
     "wfd_points_nrows": (ctypes.c_int32, [_GP]),
     "wfd_step": (ctypes.c_int, [_GP, _P, _P, _P, _MP, _PP, _PP, _I64, _P]),
     "wfd_forward": (ctypes.c_int, [_GP, _P, _P, _MP, _PP, _PP, _I64, _I64, _P]),

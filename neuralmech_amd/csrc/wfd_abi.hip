@@ -126,20 +126,24 @@ void fill_weights(const wfd_grid* g, T& wc, T (*w)[4])
 }
 
 template <typename T>
-int fill_points(PtsDev<T>& p, const wfd_points* src, int32_t nrows, const char* what)
+int fill_points(PtsDev<T>& p, const wfd_points* src, int32_t nrows, const char* what, bool gathered)
 {
     std::memset(&p, 0, sizeof p);
     if (!src || src->count == 0) return WFD_OK;
     if (src->count < 0) return fail(WFD_EINVAL, "%s: negative point count", what);
     if (src->nrows != nrows) return fail(WFD_EINVAL, "%s: CSR has %d rows, grid needs %d", what, src->nrows, nrows);
-    if (!src->row_ptr || !src->col || !src->id || !src->data)
+    if (!src->row_ptr || !src->col || !src->id || !src->data || (gathered && !src->row))
         return fail(WFD_EINVAL, "%s: NULL CSR or data pointer", what);
+    if (src->width < src->count || src->nsteps < 0 || src->shot_stride < src->nsteps * src->width)
+        return fail(WFD_EINVAL, "%s: data narrower than the point count or shorter than nsteps rows", what);
     p.row_ptr = src->row_ptr;
     p.col = src->col;
     p.id = src->id;
+    p.row = src->row;
     p.data = static_cast<T*>(src->data);
     p.width = src->width;
     p.shot_stride = src->shot_stride;
+    p.nt = src->nsteps;
     p.count = src->count;
     p.nrows = src->nrows;
     return WFD_OK;
@@ -219,13 +223,14 @@ struct Launch {
     void* un;
     const wfd_medium* med;
     const wfd_points* src;
-    const wfd_points* rec;
+    const wfd_points* rec;  // gathered: u_cur at these nodes -> trace row step + gather_off
     int64_t step;
     const int64_t* step_dev;
     const void* uf;
     void* gpart;
     bool img;
     LaunchCfg cfg;
+    int64_t gather_off;
 };
 
 template <typename T>
@@ -235,9 +240,9 @@ int launch_t(const Launch& L, hipStream_t s)
     StepArgs<T> a;
     std::memset(&a, 0, sizeof a);
     const int64_t off = g->offset;
-    a.up = static_cast<const T*>(L.up) + off;
+    a.up = L.up ? static_cast<const T*>(L.up) + off : nullptr;
     a.uc = static_cast<const T*>(L.uc) + off;
-    a.un = static_cast<T*>(L.un) + off;
+    a.un = L.un ? static_cast<T*>(L.un) + off : nullptr;
     const wfd_medium* m = L.med;
     const bool varc = m && m->coef;
     const bool sponge = m && m->beta;
@@ -284,13 +289,21 @@ int launch_t(const Launch& L, hipStream_t s)
         a.wp[mm] = g->ndim == 3 ? w[0][mm] : T(0);
     }
     const int32_t nrows = nrows_of(g);
-    int rc = fill_points<T>(a.src, L.src, nrows, L.img ? "adjoint sources" : "sources");
+    int rc = fill_points<T>(a.src, L.src, nrows, L.img ? "adjoint sources" : "sources", false);
     if (rc) return rc;
-    rc = fill_points<T>(a.rec, L.rec, nrows, "receivers");
+    rc = fill_points<T>(a.rec, L.rec, nrows, "receivers", true);
     if (rc) return rc;
+    a.gather_off = L.gather_off;
+    a.nshots = g->nshots;
+    a.ngb = a.rec.count ? (uint32_t)(((int64_t)g->nshots * a.rec.count + 255) / 256) : 0u;
     a.step = L.step;
     a.step_dev = L.step_dev;
     hipError_t e;
+    if (!L.un) {  // gather only: L.uc is the field, row = L.step + L.gather_off
+        e = launch_gather<T>(a, s);
+        if (e != hipSuccess) return hip_fail(e, "gather kernel launch");
+        return WFD_OK;
+    }
     if (g->ndim == 3)
         e = launch_step3d<T>(a, g->order, varc, sponge, L.img, std::max(1, L.cfg.ri), g->nshots, s);
     else
@@ -341,6 +354,22 @@ int device_counter(int64_t** out)
 }  // namespace
 
 int wfd::set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
+int wfd::step_launch(const wfd_grid* g, const void* up, const void* uc, void* un, const wfd_medium* med,
+                     const wfd_points* src, const wfd_points* rec, int64_t step, int64_t gather_off, const void* uf,
+                     void* gpart, bool img, hipStream_t s)
+{
+    Launch L{g, up, uc, un, med, src, rec, step, nullptr, uf, gpart, img, default_cfg(g), gather_off};
+    return do_launch(L, s);
+}
+
+int wfd::gather_launch(const wfd_grid* g, const void* field, const wfd_points* rec, int64_t row, hipStream_t s)
+{
+    if (!rec || rec->count == 0) return WFD_OK;
+    if (!field) return fail(WFD_EINVAL, "gather: NULL field");
+    Launch L{g, nullptr, field, nullptr, nullptr, nullptr, rec, row, nullptr, nullptr, nullptr, false, default_cfg(g), 0};
+    return do_launch(L, s);
+}
 
 // ======================================================================== exported C ABI
 extern "C" {
@@ -428,11 +457,11 @@ int wfd_stencil_weights(const wfd_grid* g, double* wc, double* w)
 int32_t wfd_points_nrows(const wfd_grid* g) { return check_grid(g) ? -1 : nrows_of(g); }
 
 int wfd_points_csr(const wfd_grid* g, int32_t npts, const int64_t* nodes, int32_t* row_ptr, int32_t* col,
-                   int32_t* id)
+                   int32_t* id, int32_t* row)
 {
     int rc = check_grid(g);
     if (rc) return rc;
-    if (npts < 0 || (npts > 0 && (!nodes || !col || !id)) || !row_ptr) return fail(WFD_EINVAL, "bad point arrays");
+    if (npts < 0 || (npts > 0 && (!nodes || !col || !id || !row)) || !row_ptr) return fail(WFD_EINVAL, "bad point arrays");
     int64_t np, nr, nc;
     extents(g, np, nr, nc);
     const int32_t nrows = nrows_of(g);
@@ -452,10 +481,11 @@ int wfd_points_csr(const wfd_grid* g, int32_t npts, const int64_t* nodes, int32_
     std::fill(row_ptr, row_ptr + nrows + 1, 0);
     for (int32_t q = 0; q < npts; q++) {
         const int32_t p = perm[q];
-        const int64_t row = key[p] / nc;
+        const int64_t r = key[p] / nc;
         col[q] = (int32_t)(key[p] % nc);
         id[q] = p;
-        row_ptr[row + 1]++;
+        row[q] = (int32_t)r;
+        row_ptr[r + 1]++;
     }
     for (int32_t r = 0; r < nrows; r++) row_ptr[r + 1] += row_ptr[r];
     return WFD_OK;
@@ -466,8 +496,10 @@ int wfd_step(const wfd_grid* g, const void* u_prev, const void* u_cur, void* u_n
 {
     int rc = check_step_args(g, u_prev, u_cur, u_next, med);
     if (rc) return rc;
-    Launch L{g, u_prev, u_cur, u_next, med, src, rec, step, nullptr, nullptr, nullptr, false, default_cfg(g)};
-    return do_launch(L, (hipStream_t)stream);
+    rc = wfd::step_launch(g, u_prev, u_cur, u_next, med, src, nullptr, step, 0, nullptr, nullptr, false,
+                          (hipStream_t)stream);
+    if (rc) return rc;
+    return wfd::gather_launch(g, u_next, rec, step, (hipStream_t)stream);
 }
 
 int wfd_forward(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med, const wfd_points* src,
@@ -476,15 +508,15 @@ int wfd_forward(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med, 
     int rc = check_step_args(g, u_a, u_b, u_a, med);
     if (rc) return rc;
     if (nsteps < 0) return fail(WFD_EINVAL, "nsteps must be >= 0");
-    const LaunchCfg cfg = default_cfg(g);
     void* p = u_a;
     void* c = u_b;
     for (int64_t k = 0; k < nsteps; k++) {
-        Launch L{g, p, c, p, med, src, rec, step0 + k, nullptr, nullptr, nullptr, false, cfg};
-        rc = do_launch(L, (hipStream_t)stream);
+        // each launch also records u^n (its u_cur) into trace row n-1
+        rc = wfd::step_launch(g, p, c, p, med, src, rec, step0 + k, -1, nullptr, nullptr, false, (hipStream_t)stream);
         if (rc) return rc;
         std::swap(p, c);
     }
+    if (nsteps > 0) return wfd::gather_launch(g, c, rec, step0 + nsteps - 1, (hipStream_t)stream);
     return WFD_OK;
 }
 
@@ -511,7 +543,7 @@ int wfd_forward_graph(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium*
             void* p = u_a;
             void* c = u_b;
             for (int k = 0; k < chunk && rc == WFD_OK; k++) {
-                Launch L{g, p, c, p, med, src, rec, k, ctr, nullptr, nullptr, false, cfg};
+                Launch L{g, p, c, p, med, src, rec, k, ctr, nullptr, nullptr, false, cfg, -1};
                 rc = do_launch(L, cap);
                 std::swap(p, c);
             }
@@ -534,13 +566,15 @@ int wfd_forward_graph(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium*
             for (int64_t r = 0; e == hipSuccess && r < reps; r++) e = hipGraphLaunch(exec, st);
             if (e != hipSuccess) rc = hip_fail(e, "graph replay");
         }
-        if (exec) hipGraphExecDestroy(exec);
-        if (graph) hipGraphDestroy(graph);
-        hipStreamDestroy(cap);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipStreamDestroy(cap);
         if (rc) return rc;
     }
     const int64_t rem = nsteps - reps * chunk;
-    // after an even number of replayed steps the roles of u_a / u_b are unchanged
+    // after an even number of replayed steps the roles of u_a / u_b are unchanged; the
+    // remainder's first launch records the last replayed level, otherwise record it here
+    if (rem == 0 && reps > 0) return wfd::gather_launch(g, u_b, rec, step0 + nsteps - 1, st);
     return wfd_forward(g, u_a, u_b, med, src, rec, step0 + reps * chunk, rem, stream);
 }
 
@@ -553,8 +587,10 @@ int wfd_adjoint_step(const wfd_grid* g, const void* mu_prev, const void* mu_cur,
     if (!u_fwd || !g_partial) return fail(WFD_EINVAL, "NULL forward field or gradient partial");
     if (u_fwd == mu_next || g_partial == mu_next || g_partial == mu_cur || g_partial == u_fwd)
         return fail(WFD_EINVAL, "adjoint buffers must not alias");
-    Launch L{g, mu_prev, mu_cur, mu_next, med, adj, rec, step, nullptr, u_fwd, g_partial, true, default_cfg(g)};
-    return do_launch(L, (hipStream_t)stream);
+    rc = wfd::step_launch(g, mu_prev, mu_cur, mu_next, med, adj, nullptr, step, 0, u_fwd, g_partial, true,
+                          (hipStream_t)stream);
+    if (rc) return rc;
+    return wfd::gather_launch(g, mu_next, rec, step, (hipStream_t)stream);
 }
 
 int wfd_grad_flush(const wfd_grid* g, void* g_partial, double* G, wfd_stream_t stream)
@@ -612,7 +648,7 @@ int wfd_check_medium(const wfd_grid* g, const wfd_medium* med, wfd_stream_t stre
             if (e == hipSuccess) e = hipStreamSynchronize(st);
         }
     }
-    hipFree(flag);
+    (void)hipFree(flag);
     if (rc) return rc;
     if (e != hipSuccess) return hip_fail(e, "medium check");
     if (host) return fail(WFD_ENONFINITE, "medium holds a NaN/Inf, a non-positive coefficient or negative damping");
@@ -679,14 +715,14 @@ int wfd_tune(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med, int
     LaunchCfg best = cands[0];
     for (const LaunchCfg& c : cands) {
         for (int k = 0; k < steps + 2 && rc == WFD_OK; k++) {
-            if (k == 2) hipEventRecord(e0, st);
+            if (k == 2) (void)hipEventRecord(e0, st);
             void* p = (k % 2) ? u_b : u_a;
             void* q = (k % 2) ? u_a : u_b;
-            Launch L{g, p, q, p, med, nullptr, nullptr, k, nullptr, nullptr, nullptr, false, c};
+            Launch L{g, p, q, p, med, nullptr, nullptr, k, nullptr, nullptr, nullptr, false, c, 0};
             rc = do_launch(L, st);
         }
         if (rc) break;
-        hipEventRecord(e1, st);
+        (void)hipEventRecord(e1, st);
         hipError_t e = hipEventSynchronize(e1);
         float ms = 0.f;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
@@ -699,8 +735,8 @@ int wfd_tune(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med, int
             best = c;
         }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc) return rc;
     remember_cfg(g, best);
     if (variant) *variant = best.tile;

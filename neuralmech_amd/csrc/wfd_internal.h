@@ -17,9 +17,11 @@ struct PtsDev {
     const int32_t* row_ptr;
     const int32_t* col;
     const int32_t* id;
+    const int32_t* row;  // CSR row of each point (receivers: gathered by dedicated blocks)
     T* data;
     int64_t width;
     int64_t shot_stride;
+    int64_t nt;  // time rows of `data` per shot
     int32_t count;
     int32_t nrows;
 };
@@ -47,24 +49,60 @@ struct StepArgs {
     PtsDev<T> src, rec;
     int64_t step;
     const int64_t* step_dev;  // graph replay: the step index is *step_dev + step
+    // Receivers are not written by the stencil blocks: `rec` lists nodes whose value in u_cur is
+    // copied to trace row (step + gather_off) by the first `ngb` blocks of the launch (the
+    // forward records u^n, written by the previous launch, into row n-1: gather_off = -1).
+    int64_t gather_off;
+    uint32_t ngb;        // gather blocks (256 points each)
+    uint32_t gx, gy;     // stencil block grid (x fastest); the launch is 1D: ngb + gx*gy*gz blocks
+    int32_t nshots;
 };
 
-// Block index remapped so that each of the 8 XCDs (whose L2 caches are separate) processes one
-// contiguous range of the grid's blocks.  The hardware hands block b (linear id, x fastest) to
-// XCD b % 8; giving that block the position start(b % 8) + b / 8 keeps neighbouring tiles -- and
-// the halo rows they share -- inside one L2.  A bijection for any grid size.
-struct BlockPos {
-    uint32_t x, y, z;
+// Position of this block in a step launch: a gather block or a stencil tile.  The hardware hands
+// block b to XCD b % 8 (each XCD has its own L2) and starts an XCD's blocks in order b / 8.  The
+// first `gper` blocks of every XCD are gather blocks (so they start first); the remaining blocks
+// of XCD x take one contiguous range of stencil tiles, start(x) + i, the same ranges whether or
+// not gather blocks are present.  Neighbouring tiles -- and the halo rows / planes they share --
+// then stay in one L2.  The launch has 8 * gper + tiles blocks (see step_grid).
+struct StepBlock {
+    bool gather;
+    uint32_t g;        // gather block index
+    uint32_t x, y, z;  // stencil tile
 };
-__device__ __forceinline__ BlockPos xcd_block()
+constexpr uint32_t XCDS = 8;
+__device__ __forceinline__ StepBlock step_block(uint32_t ngb, uint32_t gx, uint32_t gy)
 {
-    constexpr uint32_t XCDS = 8;
-    const uint32_t gx = gridDim.x, gy = gridDim.y;
-    const uint32_t nb = gx * gy * gridDim.z;
-    const uint32_t b = (blockIdx.z * gy + blockIdx.y) * gx + blockIdx.x;
-    const uint32_t per = nb / XCDS, rem = nb % XCDS, x = b % XCDS, i = b / XCDS;
-    const uint32_t n = x * per + (x < rem ? x : rem) + i;
-    return {n % gx, (n / gx) % gy, n / (gx * gy)};
+    const uint32_t gper = (ngb + XCDS - 1) / XCDS;
+    const uint32_t b = blockIdx.x, x = b % XCDS, i = b / XCDS;
+    if (i < gper) return {true, i * XCDS + x, 0, 0, 0};  // index >= ngb: an idle gather block
+    const uint32_t nt = gridDim.x - gper * XCDS;         // stencil tiles
+    const uint32_t per = nt / XCDS, rem = nt % XCDS;
+    const uint32_t j = x * per + (x < rem ? x : rem) + (i - gper);
+    return {false, 0, j % gx, (j / gx) % gy, j / (gx * gy)};
+}
+// blocks of a step launch with `tiles` stencil tiles and ngb gather blocks
+__host__ __device__ inline uint32_t step_grid(uint32_t tiles, uint32_t ngb)
+{
+    return tiles + (ngb + XCDS - 1) / XCDS * XCDS;
+}
+
+// Copy u_cur at gather block `blk`'s points (256 of the flattened (shot, point) list) to trace
+// row step + gather_off, when that row exists.
+template <typename T>
+__device__ __forceinline__ void gather_points(const StepArgs<T>& a, uint32_t blk, int64_t step)
+{
+    const PtsDev<T>& p = a.rec;
+    const int64_t t = step + a.gather_off;
+    if (t < 0 || t >= p.nt) return;
+    const int64_t idx = (int64_t)blk * 256 + threadIdx.x;
+    const int64_t shot = idx / p.count, q = idx % p.count;
+    if (shot >= a.nshots) return;
+    const int32_t* rp = p.row_ptr + shot * (p.nrows + 1);
+    if (q >= rp[p.nrows]) return;  // this shot has fewer points
+    const int64_t pq = shot * p.count + q;
+    const int64_t row = p.row[pq], col = p.col[pq];
+    const int64_t off = (row / a.nr) * a.ps + (row % a.nr) * a.ld + col;  // 1D/2D: row / nr == 0 or nr == 1
+    p.data[shot * p.shot_stride + t * p.width + p.id[pq]] = a.uc[shot * a.shot_stride + off];
 }
 
 struct LaunchCfg {
@@ -86,11 +124,22 @@ template <typename T>
 hipError_t launch_step3d(const StepArgs<T>& a, int order, bool varc, bool sponge, bool img, int rk,
                          int nshots, hipStream_t s);
 
+template <typename T>
+hipError_t launch_gather(const StepArgs<T>& a, hipStream_t s);
+
 hipError_t launch_add_i64(int64_t* p, int64_t v, hipStream_t s);
 hipError_t launch_set_i64(int64_t* p, int64_t v, hipStream_t s);
 hipError_t launch_zero(void* p, int64_t bytes, hipStream_t s);
 
 // record an error message for wfd_last_error and return `code`
 int set_error(int code, const char* msg);
+
+// One step launch (stencil tiles + gather blocks copying u_cur at `rec` into trace row
+// step + gather_off), and a gather-only launch of `field` into trace row `row`.  Used by the
+// public entry points and the gradient plan.
+int step_launch(const wfd_grid* g, const void* up, const void* uc, void* un, const wfd_medium* med,
+                const wfd_points* src, const wfd_points* rec, int64_t step, int64_t gather_off, const void* uf,
+                void* gpart, bool img, hipStream_t s);
+int gather_launch(const wfd_grid* g, const void* field, const wfd_points* rec, int64_t row, hipStream_t s);
 
 }  // namespace wfd

@@ -180,11 +180,14 @@ int wfd_plan_forward(wfd_plan* p, const wfd_medium* med, const wfd_points* src, 
             D = P;  // in place over the oldest level
         else
             D = Cu == p->work[0] ? p->work[1] : p->work[0];
-        int rc = wfd_step(&p->g, P, Cu, D, med, src, rec, n, stream);
+        // the launch also records u^n (its current level) into trace row n-1
+        int rc = wfd::step_launch(&p->g, P, Cu, D, med, src, rec, n, -1, nullptr, nullptr, false, st);
         if (rc) return rc;
         P = Cu;
         Cu = D;
     }
+    int rc = wfd::gather_launch(&p->g, Cu, rec, N - 1, st);
+    if (rc) return rc;
     if (u_last) {
         e = hipMemcpyAsync(u_last, P, p->batch_bytes, hipMemcpyDeviceToDevice, st);
         if (e == hipSuccess)
@@ -214,13 +217,15 @@ int wfd_plan_backward(wfd_plan* p, const wfd_medium* med, const wfd_points* src,
         const int64_t s0 = k * C, s1 = std::min(s0 + C, N);
         for (int64_t n = s0; n + 1 < s1; n++) {
             const void* prev = n == s0 ? p->ck_prev[k] : field_at(p, n - 1, s0, k);
-            int rc = wfd_step(&p->g, prev, field_at(p, n, s0, k), field_at(p, n + 1, s0, k), med, src, nullptr, n,
-                              stream);
+            int rc = wfd::step_launch(&p->g, prev, field_at(p, n, s0, k), field_at(p, n + 1, s0, k), med, src,
+                                      nullptr, n, 0, nullptr, nullptr, false, st);
             if (rc) return rc;
         }
         for (int64_t m = s1; m > s0; m--) {
-            int rc = wfd_adjoint_step(&p->g, muP, muC, muP, med, adj, src_rec, m - 1, field_at(p, m - 1, s0, k),
-                                      p->gpart, stream);
+            // adjoint step m (signal row m-1) makes mu^m; its launch records mu^{m+1} (its current
+            // level) into row m of src_rec
+            int rc = wfd::step_launch(&p->g, muP, muC, muP, med, adj, src_rec, m - 1, +1, field_at(p, m - 1, s0, k),
+                                      p->gpart, true, st);
             if (rc) return rc;
             std::swap(muP, muC);  // muC = mu^m, muP = mu^{m+1}
             if ((m - 1) % p->flush == 0) {
@@ -229,7 +234,7 @@ int wfd_plan_backward(wfd_plan* p, const wfd_medium* med, const wfd_points* src,
             }
         }
     }
-    return WFD_OK;
+    return wfd::gather_launch(&p->g, muC, src_rec, 0, st);  // mu^1 into row 0
 }
 
 }  // extern "C"

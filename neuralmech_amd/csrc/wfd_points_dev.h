@@ -1,14 +1,12 @@
-// Wave-cooperative source injection and receiver recording for the step kernels.
+// Wave-cooperative source injection for the step kernels (receivers are recorded by dedicated
+// gather blocks, see gather_points).
 //
 // A wave updates one row of 4-node chunks at a time: lane l owns nodes [base + 4l, base + 4l + 4)
 // of a CSR row, for lanes lane_lo..lane_hi.  The row's points are read 64 at a time with
-// coalesced loads (one point per lane, all loads independent), then
-//   injection: a ballot picks the points this wave owns, and for each (in list order) the
-//              owning lane applies num = fma(kc, sig, num) with the value broadcast by readlane;
-//   recording: each point's lane fetches the owner's value with a cross-lane permute and
-//              writes the trace entry.
-// No dependent chains of memory loads (a per-lane binary search would serialise one load
-// latency per probe).  All 64 lanes of the wave must be active when these are called.
+// coalesced loads (one point per lane, all loads independent); a ballot picks the points this
+// wave owns, and for each (in list order) the owning lane applies num = fma(kc, sig, num) with
+// the value broadcast by readlane.  No dependent chains of memory loads (a per-lane binary
+// search would serialise one load latency per probe).  All 64 lanes of the wave must be active.
 #pragma once
 
 #include "wfd_internal.h"
@@ -28,20 +26,13 @@ __device__ __forceinline__ double lane_read(double v, int l)
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-__device__ __forceinline__ float lane_fetch(float v, int src)
-{
-    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v)));
-}
-__device__ __forceinline__ double lane_fetch(double v, int src)
-{
-    const long long x = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_ds_bpermute(src << 2, (int)(x & 0xffffffff));
-    const int hi = __builtin_amdgcn_ds_bpermute(src << 2, (int)(x >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
 __device__ __forceinline__ float pt_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double pt_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Points of a row are handled PB batches of 64 at a time: all loads of a pass are issued before
+// any is used, so a pass costs one load latency (plus one for the signal gather) however many
+// batches it holds.
+constexpr int PB = 4;
 
 template <typename T>
 __device__ __forceinline__ void wave_inject(const PtsDev<T>& p, int shot, int64_t row, int64_t base, int lane_lo,
@@ -53,52 +44,36 @@ __device__ __forceinline__ void wave_inject(const PtsDev<T>& p, int shot, int64_
     const int32_t* col = p.col + (int64_t)shot * p.count;
     const int32_t* id = p.id + (int64_t)shot * p.count;
     const T* sig = p.data + (int64_t)shot * p.shot_stride + step * p.width;
-    for (int b = lo; b < hi; b += 64) {
-        const int q = b + lane;
-        const bool have = q < hi;
-        const int64_t off = have ? (int64_t)col[q] - base : -1;
-        const T s = have ? sig[id[q]] : T(0);
-        const bool mine = have && off >= 4 * lane_lo && off < 4 * (lane_hi + 1);
-        uint64_t m = __ballot(mine);
-        const int offi = mine ? (int)off : 0;
-        while (m) {
-            const int pl = __builtin_ctzll(m);
-            m &= m - 1;
-            const int o = lane_read(offi, pl);
-            const T ts = lane_read(s, pl);
-            if (lane == (o >> 2)) {
-                const int e = o & 3;
+    for (int b = lo; b < hi; b += 64 * PB) {
+        int cl[PB], pid[PB];
 #pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if (e == k) num[k] = pt_fma(kc[k], ts, num[k]);
+        for (int j = 0; j < PB; j++) {
+            const int q = b + 64 * j + lane;
+            cl[j] = q < hi ? col[q] : -1;
+            pid[j] = q < hi ? id[q] : -1;
+        }
+        T s[PB];
+#pragma unroll
+        for (int j = 0; j < PB; j++) s[j] = pid[j] >= 0 ? sig[pid[j]] : T(0);
+#pragma unroll
+        for (int j = 0; j < PB; j++) {
+            const int64_t off = cl[j] >= 0 ? (int64_t)cl[j] - base : -1;
+            const bool mine = off >= 4 * lane_lo && off < 4 * (lane_hi + 1);
+            uint64_t m = __ballot(mine);
+            const int offi = mine ? (int)off : 0;
+            while (m) {
+                const int pl = __builtin_ctzll(m);
+                m &= m - 1;
+                const int o = lane_read(offi, pl);
+                const T ts = lane_read(s[j], pl);
+                if (lane == (o >> 2)) {
+                    const int e = o & 3;
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        if (e == k) num[k] = pt_fma(kc[k], ts, num[k]);
+                }
             }
         }
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void wave_record(const PtsDev<T>& p, int shot, int64_t row, int64_t base, int lane_lo,
-                                            int lane_hi, int64_t step, int lane, const T (&val)[4])
-{
-    const int32_t* rp = p.row_ptr + (int64_t)shot * (p.nrows + 1);
-    const int lo = rp[row], hi = rp[row + 1];
-    if (lo >= hi) return;
-    const int32_t* col = p.col + (int64_t)shot * p.count;
-    const int32_t* id = p.id + (int64_t)shot * p.count;
-    T* tr = p.data + (int64_t)shot * p.shot_stride + step * p.width;
-    for (int b = lo; b < hi; b += 64) {
-        const int q = b + lane;
-        const bool have = q < hi;
-        const int64_t off = have ? (int64_t)col[q] - base : -1;
-        const bool mine = have && off >= 4 * lane_lo && off < 4 * (lane_hi + 1);
-        const int src = mine ? (int)(off >> 2) : lane;
-        const int e = mine ? (int)(off & 3) : 0;
-        const T v0 = lane_fetch(val[0], src);
-        const T v1 = lane_fetch(val[1], src);
-        const T v2 = lane_fetch(val[2], src);
-        const T v3 = lane_fetch(val[3], src);
-        const T v = e == 0 ? v0 : (e == 1 ? v1 : (e == 2 ? v2 : v3));
-        if (mine) tr[id[q]] = v;
     }
 }
 

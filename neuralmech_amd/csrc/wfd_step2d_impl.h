@@ -119,7 +119,12 @@ __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
     constexpr int64_t COLS = XL ? 248 : 256;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const BlockPos bp = xcd_block();
+    const StepBlock bp = step_block(a.ngb, a.gx, a.gy);
+    const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
+    if (bp.gather) {
+        gather_points(a, bp.g, step);
+        return;
+    }
     const int shot = bp.z;
     const int64_t base = (int64_t)bp.x * COLS - (XL ? 4 : 0);  // column of lane 0's chunk
     const int64_t c = base + lane * 4;
@@ -134,7 +139,6 @@ __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
     const T* up = a.up + sb;  // may alias un (in-place leapfrog): same element, same lane
     T* un = a.un + sb;
     const T* __restrict__ coef = VARC ? a.coef + (int64_t)shot * a.coef_shot_stride : nullptr;
-    const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
 
     C4<T> win[NW];
 #pragma unroll
@@ -152,13 +156,11 @@ __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
         }
     }
 
-    // does any point list have an entry in this wave's rows? (one range check per wave)
+    // does the source list have an entry in this wave's rows? (one range check per wave)
     const int64_t rlast = VROW ? (r0 + RI < a.row_hi ? r0 + RI : a.row_hi) : 1;
     const int64_t rfirst = VROW ? r0 : 0;
     const bool src_here = a.src.count &&
         a.src.row_ptr[(int64_t)shot * (a.src.nrows + 1) + rfirst] != a.src.row_ptr[(int64_t)shot * (a.src.nrows + 1) + rlast];
-    const bool rec_here = a.rec.count &&
-        a.rec.row_ptr[(int64_t)shot * (a.rec.nrows + 1) + rfirst] != a.rec.row_ptr[(int64_t)shot * (a.rec.nrows + 1) + rlast];
 
     bool valid[4];
     T bc[4];
@@ -229,7 +231,6 @@ __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
         for (int e = 0; e < 4; e++)
             if (!valid[e]) val[e] = T(0);
         if (own) st4(un + o, val);
-        if (rec_here) wave_record(a.rec, shot, prow, base, lane_lo, lane_hi, step, lane, val);
         if constexpr (IMG) {
             T gv[4];
 #pragma unroll
@@ -261,40 +262,46 @@ __global__ __launch_bounds__(256) void step2d_xl_kernel(const StepArgs<T> a)
 
 // ------------------------------------------------------------------------------ dispatch
 template <typename T, int K, bool VARC, bool SPONGE, bool IMG>
-static hipError_t launch2d_k(const StepArgs<T>& a, bool vrow, int variant, int ri, int nshots, hipStream_t s)
+static hipError_t launch2d_k(const StepArgs<T>& a0, bool vrow, int variant, int ri, int nshots, hipStream_t s)
 {
     const dim3 block(256);
-    const int64_t rows = a.row_hi - a.row_lo;
+    const int64_t rows = a0.row_hi - a0.row_lo;
+    StepArgs<T> a = a0;
+    // 1D launch: a.ngb gather blocks, then gx * gy * nshots stencil tiles
+    auto grid = [&](int64_t cols, int r) {
+        a.gx = (uint32_t)((a.nc + cols - 1) / cols);
+        a.gy = vrow ? (uint32_t)((rows + 4 * r - 1) / (4 * r)) : 1u;
+        return dim3(step_grid((uint32_t)((int64_t)a.gx * a.gy * nshots), a.ngb));
+    };
     if (!vrow) {
-        hipLaunchKernelGGL((step2d_kernel<T, K, false, VARC, SPONGE, IMG, 1>),
-                           dim3((unsigned)((a.nc + 255) / 256), 1, nshots), block, 0, s, a);
+        const dim3 g = grid(256, 1);
+        hipLaunchKernelGGL((step2d_kernel<T, K, false, VARC, SPONGE, IMG, 1>), g, block, 0, s, a);
         return hipGetLastError();
     }
-    auto grid = [&](int64_t cols, int r) {
-        return dim3((unsigned)((a.nc + cols - 1) / cols), (unsigned)((rows + 4 * r - 1) / (4 * r)), nshots);
-    };
     if (variant == 1) {
         if constexpr (sizeof(T) == 4) {
             switch (ri) {
-                case 1: hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 1>), grid(XL_COLS, 1), block, 0, s, a); break;
-                case 2: hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 2>), grid(XL_COLS, 2), block, 0, s, a); break;
-                case 8: hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 8>), grid(XL_COLS, 8), block, 0, s, a); break;
-                default: hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4>), grid(XL_COLS, 4), block, 0, s, a); break;
+                case 1: { const dim3 g = grid(XL_COLS, 1); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 1>), g, block, 0, s, a); break; }
+                case 2: { const dim3 g = grid(XL_COLS, 2); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 2>), g, block, 0, s, a); break; }
+                case 8: { const dim3 g = grid(XL_COLS, 8); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 8>), g, block, 0, s, a); break; }
+                default: { const dim3 g = grid(XL_COLS, 4); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4>), g, block, 0, s, a); break; }
             }
         } else {
-            hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4>), grid(XL_COLS, 4), block, 0, s, a);
+            const dim3 g = grid(XL_COLS, 4);
+            hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4>), g, block, 0, s, a);
         }
         return hipGetLastError();
     }
     if constexpr (sizeof(T) == 4) {
         switch (ri) {
-            case 1: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 1>), grid(256, 1), block, 0, s, a); break;
-            case 2: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 2>), grid(256, 2), block, 0, s, a); break;
-            case 8: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 8>), grid(256, 8), block, 0, s, a); break;
-            default: hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4>), grid(256, 4), block, 0, s, a); break;
+            case 1: { const dim3 g = grid(256, 1); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 1>), g, block, 0, s, a); break; }
+            case 2: { const dim3 g = grid(256, 2); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 2>), g, block, 0, s, a); break; }
+            case 8: { const dim3 g = grid(256, 8); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 8>), g, block, 0, s, a); break; }
+            default: { const dim3 g = grid(256, 4); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4>), g, block, 0, s, a); break; }
         }
     } else {
-        hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4>), grid(256, 4), block, 0, s, a);
+        const dim3 g = grid(256, 4);
+        hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4>), g, block, 0, s, a);
     }
     return hipGetLastError();
 }

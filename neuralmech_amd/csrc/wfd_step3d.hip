@@ -105,7 +105,12 @@ __global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const BlockPos bp = xcd_block();
+    const StepBlock bp = step_block(a.ngb, a.gx, a.gy);
+    const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
+    if (bp.gather) {
+        gather_points(a, bp.g, step);
+        return;
+    }
     const int shot = bp.z / nkb, kb = bp.z % nkb;
     const int64_t j0 = (int64_t)bp.x * T3_TJ;
     const int64_t c = j0 + tx * 4;
@@ -123,7 +128,6 @@ __global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk
     const T* __restrict__ coef = VARC ? a.coef + (int64_t)shot * a.coef_shot_stride : nullptr;
     const T* __restrict__ uf = IMG ? a.uf + sb : nullptr;
     T* gp = IMG ? a.g + sb : nullptr;
-    const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
     const int64_t ld = a.ld, ps = a.ps;
     const int64_t rowoff = i * ld + c;
     const V4<T> zero = {{T(0), T(0), T(0), T(0)}};
@@ -148,6 +152,9 @@ __global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk
     T br = T(0);
     if constexpr (SPONGE) br = (i < a.nr) ? a.beta_r[i] : T(0);
 
+    const int64_t wr0 = min(i0 + 4 * wave, a.nr - 1), wr1 = min(i0 + 4 * wave + 4, a.nr - 1);
+    uint64_t src_mask = 0;
+    int64_t mask_k = k0 - 64;
     int buf = 0;
     for (int64_t k = k0; k < k1; k++, buf ^= 1) {
 #pragma unroll
@@ -190,13 +197,20 @@ __global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk
             const T t = damp ? T(2) * ctr - (T(1) - b[e]) * P.e[e] : T(2) * ctr - P.e[e];
             val[e] = fma3(kc[e], lap, t);
         }
-        // point passes: the wave holds 4 rows of the tile, 16 lanes each; one range check per
-        // plane decides whether any of them has points
-        const int64_t wr0 = min(i0 + 4 * wave, a.nr - 1), wr1 = min(i0 + 4 * wave + 4, a.nr - 1);
-        const bool src_here = a.src.count && a.src.row_ptr[(int64_t)shot * (a.src.nrows + 1) + k * a.nr + wr0] !=
-                                                 a.src.row_ptr[(int64_t)shot * (a.src.nrows + 1) + k * a.nr + wr1];
-        const bool rec_here = a.rec.count && a.rec.row_ptr[(int64_t)shot * (a.rec.nrows + 1) + k * a.nr + wr0] !=
-                                                 a.rec.row_ptr[(int64_t)shot * (a.rec.nrows + 1) + k * a.nr + wr1];
+        // source pass: the wave holds 4 rows of the tile, 16 lanes each.  Which of the next 64
+        // planes have sources in these rows is found once per 64 planes (lane l checks plane
+        // k + l, one ballot), so no per-plane load latency sits in the loop.
+        if (a.src.count && k >= mask_k + 64) {
+            mask_k = k;
+            const int64_t kk = k + lane;
+            bool has = false;
+            if (kk < k1) {
+                const int32_t* rp = a.src.row_ptr + (int64_t)shot * (a.src.nrows + 1) + kk * a.nr;
+                has = rp[wr0] != rp[wr1];
+            }
+            src_mask = __ballot(has);
+        }
+        const bool src_here = a.src.count && ((src_mask >> (k - mask_k)) & 1);
         if (src_here) {
             for (int rr = 0; rr < 4; rr++) {
                 const int64_t ir = i0 + 4 * wave + rr;
@@ -212,13 +226,6 @@ __global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk
         for (int e = 0; e < 4; e++)
             if (!valid[e]) val[e] = T(0);
         if (out) gst4(un + o, val);
-        if (rec_here) {
-            for (int rr = 0; rr < 4; rr++) {
-                const int64_t ir = i0 + 4 * wave + rr;
-                if (ir <= a.nr - 2)
-                    wave_record(a.rec, shot, k * a.nr + ir, j0 - 64 * rr, 16 * rr, 16 * rr + 15, step, lane, val);
-            }
-        }
         if constexpr (IMG) {
             T gv[4];
 #pragma unroll
@@ -236,11 +243,13 @@ __global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk
 template <typename T, int K, bool VARC, bool SPONGE, bool IMG>
 static hipError_t launch3d_k(const StepArgs<T>& a, int rk, int nshots, hipStream_t s)
 {
-    const int64_t gx = (a.nc + T3_TJ - 1) / T3_TJ;
-    const int64_t gy = (a.nr - 2 + T3_TI - 1) / T3_TI;
+    StepArgs<T> b = a;
+    b.gx = (uint32_t)((a.nc + T3_TJ - 1) / T3_TJ);
+    b.gy = (uint32_t)((a.nr - 2 + T3_TI - 1) / T3_TI);
     const int64_t nkb = (a.np - 2 + rk - 1) / rk;
-    hipLaunchKernelGGL((step3d_kernel<T, K, VARC, SPONGE, IMG>),
-                       dim3((unsigned)gx, (unsigned)gy, (unsigned)(nkb * nshots)), dim3(256), 0, s, a, rk,
+    // 1D launch: b.ngb gather blocks, then gx * gy * (nkb * nshots) tiles
+    const int64_t nblocks = step_grid((uint32_t)((int64_t)b.gx * b.gy * nkb * nshots), b.ngb);
+    hipLaunchKernelGGL((step3d_kernel<T, K, VARC, SPONGE, IMG>), dim3((unsigned)nblocks), dim3(256), 0, s, b, rk,
                        (int)nkb);
     return hipGetLastError();
 }
@@ -278,6 +287,23 @@ template hipError_t launch_step3d<float>(const StepArgs<float>&, int, bool, bool
 template hipError_t launch_step3d<double>(const StepArgs<double>&, int, bool, bool, bool, int, int, hipStream_t);
 
 // ------------------------------------------------------------------------- small kernels
+// Gather-only launch: copy a field at the receiver nodes into one trace row.
+template <typename T>
+__global__ __launch_bounds__(256) void gather_kernel(const StepArgs<T> a)
+{
+    gather_points(a, blockIdx.x, a.step);
+}
+
+template <typename T>
+hipError_t launch_gather(const StepArgs<T>& a, hipStream_t s)
+{
+    if (a.ngb == 0) return hipSuccess;
+    hipLaunchKernelGGL((gather_kernel<T>), dim3(a.ngb), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template hipError_t launch_gather<float>(const StepArgs<float>&, hipStream_t);
+template hipError_t launch_gather<double>(const StepArgs<double>&, hipStream_t);
+
 template <typename T>
 __global__ void flush_kernel(T* g, double* G, int64_t elems, int nshots)
 {

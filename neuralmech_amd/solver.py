@@ -161,24 +161,27 @@ class PointSet:
         rp = np.zeros((grid.nshots, nr + 1), np.int32)
         col = np.zeros((grid.nshots, max(self.count, 1)), np.int32)
         ids = np.zeros((grid.nshots, max(self.count, 1)), np.int32)
+        rows = np.zeros((grid.nshots, max(self.count, 1)), np.int32)
+        i32 = ctypes.POINTER(ctypes.c_int32)
         for s, f in enumerate(flat):
             f = np.ascontiguousarray(f, np.int64)
             r = np.zeros(nr + 1, np.int32)
             c = np.zeros(max(len(f), 1), np.int32)
             i = np.zeros(max(len(f), 1), np.int32)
-            check(lib.wfd_points_csr(ctypes.byref(grid.c), len(f),
-                                     f.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
-                                     r.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                     c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                     i.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+            w = np.zeros(max(len(f), 1), np.int32)
+            check(lib.wfd_points_csr(ctypes.byref(grid.c), len(f), f.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                     r.ctypes.data_as(i32), c.ctypes.data_as(i32), i.ctypes.data_as(i32),
+                                     w.ctypes.data_as(i32)))
             rp[s] = r
             col[s, : len(f)] = c[: len(f)]
             ids[s, : len(f)] = i[: len(f)]
+            rows[s, : len(f)] = w[: len(f)]
         dev = device or grid.device
         self.nodes_flat = flat
         self.row_ptr = torch.from_numpy(rp).to(dev)
         self.col = torch.from_numpy(col).to(dev)
         self.id = torch.from_numpy(ids).to(dev)
+        self.row = torch.from_numpy(rows).to(dev)
 
     def struct(self, data: Optional[torch.Tensor]) -> Optional[_lib.Points]:
         """C descriptor with ``data`` ``(nshots, steps, width)`` as signal or trace buffer."""
@@ -191,8 +194,10 @@ class PointSet:
         p = _lib.Points()
         p.count, p.nrows = self.count, self.grid.nrows
         p.row_ptr, p.col, p.id = self.row_ptr.data_ptr(), self.col.data_ptr(), self.id.data_ptr()
+        p.row = self.row.data_ptr()
         p.data = data.data_ptr()
         p.width = data.shape[2]
+        p.nsteps = data.shape[1]
         p.shot_stride = data.shape[1] * data.shape[2]
         return p
 

@@ -91,10 +91,14 @@ def _csr(g, nodes):
     rp = np.zeros(nrows + 1, np.int32)
     col = np.zeros(max(len(nodes), 1), np.int32)
     ids = np.zeros(max(len(nodes), 1), np.int32)
+    rows = np.zeros(max(len(nodes), 1), np.int32)
+    i32 = ctypes.POINTER(ctypes.c_int32)
     rc = _lib.load().wfd_points_csr(ctypes.byref(g), len(nodes), nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
-                                    rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                    col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                                    ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+                                    rp.ctypes.data_as(i32), col.ctypes.data_as(i32), ids.ctypes.data_as(i32),
+                                    rows.ctypes.data_as(i32))
+    if rc == 0:  # the row array repeats the CSR row of each point
+        for r in range(len(rp) - 1):
+            assert all(rows[rp[r]:rp[r + 1]] == r)
     return rc, rp, col[: len(nodes)], ids[: len(nodes)]
 
 
