@@ -1,0 +1,84 @@
+"""``stable_dt``, ``grid_coords`` and ``simulate``: the ``cuwave.wave`` calls of the drivers
+(``projects/18_fwi/helper.py:52``, ``plate.py:47,56``, ``8_physics_drivers/waveND.py:94-95``,
+``18_fwi/animate_bhai_waves.py:29``) over the wavefd solver."""
+
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import solver
+
+
+def stable_dt(dx, wavespeed, space_order: int = 2) -> float:
+    """Largest stable step for spacing ``dx`` (tuple), the largest of ``wavespeed`` and the order."""
+    h = tuple(float(x) for x in (dx if np.ndim(dx) else [dx]))
+    vmax = float(torch.as_tensor(wavespeed, dtype=torch.float64).max())
+    return solver.stable_dt(h, vmax, int(space_order))
+
+
+def grid_coords(Nx, dx, dtype=np.float64):
+    """Node coordinates, one ``ij``-indexed array per axis: node ``i`` at ``(i - 1) dx``."""
+    axes = [(np.arange(n, dtype=np.float64) - 1.0) * h for n, h in zip(Nx, dx if np.ndim(dx) else [dx] * len(Nx))]
+    return [a.astype(dtype) for a in np.meshgrid(*axes, indexing="ij")]
+
+
+def _check_indicator(sim, indicator):
+    if indicator is None:
+        return
+    g = torch.as_tensor(indicator)
+    if not bool(torch.all(g == 1)):
+        raise NotImplementedError(
+            "a density indicator (the variable-density parametrisation of cuwave's misfit_gradient) "
+            "is not implemented by this solver; pass the model as ScalarWave(wavespeed=...)")
+
+
+def _setup(sim, sources, sensors, nshots):
+    g = solver.WaveGrid(sim.Nx, sim.dx, sim.dt, sim.space_order, sim.dtype, nshots=nshots, device=sim.device)
+    med = solver.Medium(g, velocity=sim.velocity(), beta_profiles=sim.damping)
+    if sources:
+        nodes = [[tuple(int(c) for c in p) for p in s.idx.T] for s in sources]
+        src = solver.PointSet(g, nodes, per_shot=nshots > 1) if nshots > 1 else solver.PointSet(g, nodes[0])
+        width = max(s.signal.shape[1] for s in sources)
+        sig = np.zeros((nshots, sim.N, width))
+        for k, s in enumerate(sources):
+            sig[k, :, : s.signal.shape[1]] = s.signal[: sim.N]
+        sig = torch.as_tensor(sig)
+    else:
+        src, sig = None, None
+    rec = solver.PointSet(g, [tuple(int(c) for c in p) for p in sensors.nodes]) if sensors is not None else None
+    return g, med, src, sig, rec
+
+
+def simulate(sim, source=None, indicator=None, sensors=None, record_every: Optional[int] = None):
+    """Run ``sim.N`` steps from rest.
+
+    Returns ``u`` (final field, node array on the device), ``(u, traces)`` with ``sensors``
+    (``traces`` of shape ``(N, R)``), ``(u, snaps)`` with ``record_every`` (host arrays every that many
+    steps), or ``(u, traces, frames)`` with both.
+    """
+    _check_indicator(sim, indicator)
+    sources = [source] if source is not None else []
+    g, med, src, sig, rec = _setup(sim, sources, sensors, 1)
+    state = (g.zeros(), g.zeros())
+    frames = []
+    traces = []
+    step = 0
+    chunk = int(record_every) if record_every else sim.N
+    while step < sim.N:
+        n = min(chunk, sim.N - step)
+        tr, state = solver.run(g, med, n, src=src, sig=sig, rec=rec, state=state, step0=step)
+        if tr is not None:
+            traces.append(tr[0])
+        step += n
+        if record_every:
+            frames.append(g.nodes(state[1])[0].detach().cpu().numpy().copy())
+    u = g.nodes(state[1])[0].clone()
+    out = [u]
+    if sensors is not None:
+        out.append(torch.cat(traces, 0) if traces else torch.zeros(sim.N, 0, dtype=sim.dtype, device=sim.device))
+    if record_every:
+        out.append(frames)
+    return out[0] if len(out) == 1 else tuple(out)

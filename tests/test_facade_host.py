@@ -1,0 +1,98 @@
+"""CPU tests of the cuwave-style facade: descriptor validation (CFL guard), coordinates, source
+snapping and scaling, sponge padding, resampling and wavelets.  No kernel is launched."""
+
+import math
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from neuralmech_amd import boundary, media, signals, utils, wave
+from neuralmech_amd.scalar import ScalarWave
+from oracle import slice_ref
+
+
+def _sim(**kw):
+    base = dict(Nx=(64, 80), dx=(0.01, 0.01), N=100, dt=0.004, precision="float32", space_order=4, wavespeed=1.0,
+                device="cpu")
+    base.update(kw)
+    return ScalarWave(**base)
+
+
+def test_cfl_guard_rejects_unstable_dt():
+    dt_max = wave.stable_dt((0.01, 0.01), 1.0, 4)
+    assert dt_max == pytest.approx(slice_ref.stable_dt((0.01, 0.01), 1.0, 4))
+    _sim(dt=0.999 * dt_max)
+    with pytest.raises(ValueError):
+        _sim(dt=1.01 * dt_max)
+    v = np.ones((64, 80))
+    v[10, 10] = 3.0  # the fastest node sets the limit
+    with pytest.raises(ValueError):
+        _sim(dt=0.9 * dt_max, wavespeed=v)
+
+
+def test_descriptor_basics_and_replace():
+    s = _sim()
+    assert s.Nx_padded == (64, 80) and s.ndim == 2 and s.cell_volume == pytest.approx(1e-4)
+    prof = boundary.sponge(s, None, 10, 0.2)
+    s2 = replace(s, damping=prof)
+    assert s2.damping is prof and s2.Nx == s.Nx
+    with pytest.raises(ValueError):
+        _sim(space_order=5)
+
+
+def test_grid_coords_convention():
+    x, y = wave.grid_coords((6, 7), (0.5, 0.25))
+    assert x[1, 0] == 0.0 and y[0, 1] == 0.0 and x[0, 0] == -0.5 and y[0, 6] == 1.25
+    assert x.shape == (6, 7)
+
+
+def test_point_source_snaps_and_scales():
+    s = _sim()
+    src = utils.point_source(s, (0.203, 0.5), np.ones(100))
+    assert src.idx.shape == (2, 1) and tuple(src.idx[:, 0]) == (21, 51)
+    assert np.allclose(src.signal, 1.0 / 1e-4)
+    far = utils.point_source(s, (-5.0, 99.0), np.ones(100))
+    assert tuple(far.idx[:, 0]) == (1, 78)  # clipped to the interior
+    many = utils.shots(s, utils.line((0.1, 0.0), (0.3, 0.0), 5), np.ones(100))
+    assert len(many) == 5 and [int(m.idx[0, 0]) for m in many] == [11, 16, 21, 26, 31]
+    sens = utils.Sensors(s, utils.line((0.0, 0.1), (0.0, 0.5), 3))
+    assert sens.nodes.shape == (3, 2) and sens.count == 3
+
+
+def test_sponge_profiles_and_padding():
+    s = _sim()
+    prof = boundary.sponge(s, None, 12, 0.3, faces=[(0, "lo"), (1, "hi")])
+    assert np.array_equal(prof[0], slice_ref.sponge_profile(64, 12, 0.3, lo=True, hi=False))
+    assert np.array_equal(prof[1], slice_ref.sponge_profile(80, 12, 0.3, lo=False, hi=True))
+    assert np.array_equal(media.sponge_profile(50, 7, 0.1), slice_ref.sponge_profile(50, 7, 0.1))
+    Nx, width, origin, region = boundary.pad_for_sponge((100, 60), (0.01, 0.01), 0.1, faces=[(0, 0), (0, 1), (1, 1)])
+    assert width == 10 and Nx == (120, 70) and origin == (-0.1, 0.0)
+    assert region == (slice(10, 110), slice(0, 60))
+
+
+def test_resample_is_linear_interpolation():
+    t = np.arange(50) * 0.1
+    rec = np.stack([2 * t + 1, -t], axis=1)
+    out = utils.resample(rec, 0.1, 0.025, 150)
+    tt = np.arange(150) * 0.025
+    inside = tt <= t[-1]
+    assert np.allclose(out[inside, 0], 2 * tt[inside] + 1) and np.allclose(out[inside, 1], -tt[inside])
+    assert np.array_equal(utils.resample(rec, 0.1, 0.1, 50), rec)
+
+
+def test_sineburst_window():
+    t = np.linspace(0, 1, 1001)
+    s = signals.sineburst(t, 2.0, 10.0, 3)
+    assert np.all(s[t > 0.3 + 1e-9] == 0) and s[0] == 0
+    assert np.max(np.abs(s)) <= 2.0 and np.max(np.abs(s)) > 1.5
+    r = signals.ricker(np.array([0.1]), 10.0)
+    assert r[0] == 1.0
+
+
+def test_indicator_other_than_one_is_refused():
+    s = _sim()
+    wave._check_indicator(s, None)
+    wave._check_indicator(s, np.ones(s.Nx))
+    with pytest.raises(NotImplementedError):
+        wave._check_indicator(s, np.full(s.Nx, 0.5))
