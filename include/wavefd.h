@@ -198,6 +198,76 @@ int wfd_plan_backward(wfd_plan* plan, const wfd_medium* med, const wfd_points* s
                       const wfd_points* adj, const wfd_points* src_rec, double* G,
                       wfd_stream_t stream);
 
+/* ---------------------------------------- uneven shot batch (one persistent launch) */
+/* Many independent 2D shots -- each with its own grid size, order, step count, checkpoint
+ * spacing, medium, source and receivers -- run their whole misfit gradient in one launch: for
+ * every shot the checkpointed forward run of wfd_plan_forward, the residual
+ * r = traces - observed, and the adjoint run of wfd_plan_backward injecting -r.  Per shot the
+ * results are bit-identical to those two calls on that shot alone with adj = -(traces - observed):
+ *   rec.data  traces [nsteps][rec.width]
+ *   adj       -(traces - observed), same shape (written; the adjoint source)
+ *   mu_src    mu^{n+1} at the sources in row n [nsteps][src.width]
+ *   G         sum_n mu^{n+1} (L u^n) on the interior nodes, float64, one field (entries outside
+ *             the interior are not written and must be zero).
+ * The launch is a persistent kernel: a grid of resident blocks walks a schedule of phases (one
+ * time step of every shot in flight per phase) separated by grid barriers whose waits are bounded
+ * by timeout_ms (a timed-out run ends early and wfd_batch_status reports it).  Shots are started
+ * longest first, as soon as the interior cells in flight stay within max_active_cells.
+ * Replaces the per-shot loop over sources inside cuwave.utils.misfit_gradient
+ * (projects/18_fwi/2D_fwi.py:40; projects/9_structural_optimization/fwi_penalization.py:143) for
+ * shots whose grids differ.  The caller keeps every buffer a shot names alive while the batch is
+ * used; runs on one device must not overlap. */
+typedef struct wfd_batch_shot {
+    wfd_grid grid;        /* ndim 2, nshots 1; one dtype for the whole batch */
+    int64_t nsteps;
+    int64_t seg;          /* checkpoint spacing, 0 = memory-optimal as in wfd_plan_create */
+    wfd_medium med;       /* coef (kc field) required, kconst unused; beta optional */
+    wfd_points src;       /* data = source signal, >= nsteps rows */
+    wfd_points rec;       /* data = traces (written), nsteps rows */
+    const void* observed; /* [nsteps][rec.width] */
+    void* adj;            /* [nsteps][rec.width] */
+    void* mu_src;         /* [nsteps][src.width] */
+    double* G;            /* elems */
+} wfd_batch_shot;
+
+typedef struct wfd_batch_opts {
+    int64_t max_active_cells; /* 0 = 12 Mi cells (about 12 shots of 1024^2) */
+    int32_t blocks_per_cu;    /* persistent grid: resident blocks per CU, 0 = occupancy limit */
+    int32_t mode;             /* 0 = one persistent launch; 1 = one launch per phase */
+    int32_t timeout_ms;       /* bound of each grid-barrier wait, 0 = 2000 */
+    int32_t reserved;
+} wfd_batch_opts;
+
+typedef struct wfd_batch wfd_batch;
+/* Validate the shots, allocate and clear the field store (host-synchronising), build the
+ * schedule.  opts may be NULL. */
+int wfd_batch_create(wfd_batch** batch, int32_t nshots, const wfd_batch_shot* shots,
+                     const wfd_batch_opts* opts);
+int wfd_batch_destroy(wfd_batch* batch);
+/* Run every shot's gradient (use_graph != 0: replay a hipGraph captured at the first call). */
+int wfd_batch_run(wfd_batch* batch, int32_t use_graph, wfd_stream_t stream);
+/* Synchronise `stream` and report whether the last run completed (WFD_EHIP if a barrier wait
+ * timed out; the next run then clears the partial sums first). */
+int wfd_batch_status(wfd_batch* batch, wfd_stream_t stream);
+/* Tile shapes per size class (largest grid side <= 768, 1024, 1280, 1536, larger): time `steps`
+ * forward steps of each class's shots under every candidate (variant 0/1, 2/4 rows per wave)
+ * inside the persistent kernel and keep the fastest; host-synchronising.  Replaces the fixed
+ * launch shapes of the reference drivers (2D_wave_cuda_padding.py:22,62-63). */
+int wfd_batch_tune(wfd_batch* batch, int32_t steps, wfd_stream_t stream);
+/* Force a tile shape for one size class (-1 = all). */
+int wfd_batch_set_config(wfd_batch* batch, int32_t size_class, int32_t variant, int32_t rows);
+int wfd_batch_info(const wfd_batch* batch, int64_t* phases, int64_t* blocks, size_t* bytes,
+                   int64_t* cell_steps);
+int wfd_batch_shot_info(const wfd_batch* batch, int32_t shot, int32_t* size_class,
+                        int32_t* variant, int32_t* rows, int64_t* seg, int64_t* start_phase,
+                        int64_t* nphases);
+/* Host-only: the op list of one shot (10 int32 per op: kind 0 gather / 1 step / 2 adjoint step,
+ * flags 1 inject / 2 flush / 4 first flush / 8 gather receivers / 16 gather sources, field
+ * indices up, uc, un, uf, gp, gf (-1 = none), signal row, trace row).  ops may be NULL to query
+ * the count. */
+int wfd_batch_shot_ops(int64_t nsteps, int64_t seg, int32_t has_src, int32_t has_rec,
+                       int32_t* ops, int64_t cap, int64_t* count, int32_t* nfields);
+
 
 /* ------------------------------------- reference-layout steps (projects/18_simple_wave_fd) */
 /* Fields are N rows of M_padded elements with the boundary ring inside the array (node (i, j)

@@ -65,6 +65,31 @@ class Points(ctypes.Structure):
     ]
 
 
+class BatchShot(ctypes.Structure):
+    _fields_ = [
+        ("grid", Grid),
+        ("nsteps", ctypes.c_int64),
+        ("seg", ctypes.c_int64),
+        ("med", Medium),
+        ("src", Points),
+        ("rec", Points),
+        ("observed", ctypes.c_void_p),
+        ("adj", ctypes.c_void_p),
+        ("mu_src", ctypes.c_void_p),
+        ("G", ctypes.c_void_p),
+    ]
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [
+        ("max_active_cells", ctypes.c_int64),
+        ("blocks_per_cu", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("timeout_ms", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
 # name -> (restype, argtypes); every symbol declared in include/wavefd.h
 _P = ctypes.c_void_p
 _GP = ctypes.POINTER(Grid)
@@ -98,6 +123,16 @@ SIGNATURES = {
     "wfd_plan_info": (ctypes.c_int, [_P, _I64P, _I64P, ctypes.POINTER(ctypes.c_size_t)]),
     "wfd_plan_forward": (ctypes.c_int, [_P, _MP, _PP, _PP, _P, _P]),
     "wfd_plan_backward": (ctypes.c_int, [_P, _MP, _PP, _PP, _PP, _P, _P]),
+    "wfd_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int32, ctypes.POINTER(BatchShot),
+                                        ctypes.POINTER(BatchOpts)]),
+    "wfd_batch_destroy": (ctypes.c_int, [_P]),
+    "wfd_batch_run": (ctypes.c_int, [_P, ctypes.c_int32, _P]),
+    "wfd_batch_status": (ctypes.c_int, [_P, _P]),
+    "wfd_batch_tune": (ctypes.c_int, [_P, ctypes.c_int32, _P]),
+    "wfd_batch_set_config": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "wfd_batch_info": (ctypes.c_int, [_P, _I64P, _I64P, ctypes.POINTER(ctypes.c_size_t), _I64P]),
+    "wfd_batch_shot_info": (ctypes.c_int, [_P, ctypes.c_int32, _I32P, _I32P, _I32P, _I64P, _I64P, _I64P]),
+    "wfd_batch_shot_ops": (ctypes.c_int, [_I64, _I64, ctypes.c_int32, ctypes.c_int32, _I32P, _I64, _I64P, _I32P]),
     "wfd_ref_step2d": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P]),

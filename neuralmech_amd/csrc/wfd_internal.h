@@ -39,6 +39,11 @@ struct StepArgs {
     const T* beta_c;
     const T* uf;  // forward field for the imaging condition (adjoint only)
     T* g;         // imaging partial (adjoint only)
+    // Fused flush of the imaging partial (2D adjoint of a single-shot grid only): when set, the
+    // step adds its new partial to gsum (float64, field layout) and stores a zero partial;
+    // gsum_first makes the sum start from 0.0 instead of reading gsum.
+    double* gsum;
+    int32_t gsum_first;
     int64_t shot_stride;
     int64_t np, nr, nc;      // nodes along plane / row / column axis (1 if absent)
     int64_t row_lo, row_hi;  // output rows [lo, hi) of the 1D/2D kernels

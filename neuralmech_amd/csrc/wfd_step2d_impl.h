@@ -109,26 +109,19 @@ __device__ __forceinline__ T lap_elem(const T wc, const T* wr, const T* wcol, co
     return lap;
 }
 
-// Shared body of both variants.  XL selects the lane-exchange halo; VROW = false is the 1D case
-// (a single row, no vertical stencil).
+// One stencil tile of both variants: column block tx, row block ty of shot `shot`.  XL selects
+// the lane-exchange halo; VROW = false is the 1D case (a single row, no vertical stencil).
 template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL>
-__device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
+__device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, uint32_t ty, int shot, int64_t step)
 {
     constexpr int KR = VROW ? K : 0;
     constexpr int NW = RI + 2 * KR;
     constexpr int64_t COLS = XL ? 248 : 256;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const StepBlock bp = step_block(a.ngb, a.gx, a.gy);
-    const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
-    if (bp.gather) {
-        gather_points(a, bp.g, step);
-        return;
-    }
-    const int shot = bp.z;
-    const int64_t base = (int64_t)bp.x * COLS - (XL ? 4 : 0);  // column of lane 0's chunk
+    const int64_t base = (int64_t)tx * COLS - (XL ? 4 : 0);  // column of lane 0's chunk
     const int64_t c = base + lane * 4;
-    const int64_t r0 = a.row_lo + ((int64_t)bp.y * 4 + wave) * RI;
+    const int64_t r0 = a.row_lo + ((int64_t)ty * 4 + wave) * RI;
     if (r0 >= a.row_hi) return;  // whole wave
     const int lane_lo = XL ? 1 : 0, lane_hi = XL ? 62 : 63;
     const bool load_ok = c < a.ld;  // no wrap of a lane's chunk into the next row
@@ -240,10 +233,31 @@ __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
                 const C4<T> G = ld4(gp + o);
 #pragma unroll
                 for (int e = 0; e < 4; e++) gv[e] = valid[e] ? fmaT(val[e], gv[e], G.e[e]) : T(0);
+                if (a.gsum) {  // fused flush: gsum += partial, partial = 0 (same bits as the flush kernel)
+                    double* gs = a.gsum + sb + o;
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        if (valid[e]) gs[e] = (a.gsum_first ? 0.0 : gs[e]) + (double)gv[e];
+                        gv[e] = T(0);
+                    }
+                }
                 st4(gp + o, gv);
             }
         }
     }
+}
+
+// A step launch: gather blocks copy receiver values, the other blocks update one tile each.
+template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL>
+__device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
+{
+    const StepBlock bp = step_block(a.ngb, a.gx, a.gy);
+    const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
+    if (bp.gather) {
+        gather_points(a, bp.g, step);
+        return;
+    }
+    step2d_tile<T, K, VROW, VARC, SPONGE, IMG, RI, XL>(a, bp.x, bp.y, (int)bp.z, step);
 }
 
 template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI>

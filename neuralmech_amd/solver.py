@@ -326,6 +326,22 @@ def run(grid: WaveGrid, medium: Medium, nsteps: int, src: Optional[PointSet] = N
     return out, (ua, ub)
 
 
+def kc_gradient(grid: WaveGrid, Gn: torch.Tensor, mu_src, sig, src: Optional[PointSet], kc) -> torch.Tensor:
+    """``dJ/dkc`` on the nodes from the imaging sum ``Gn = sum_n mu^{n+1} L u^n`` (node array,
+    modified in place): the source term ``sum_n mu^{n+1}(x_s) sig^n_s`` is added at the source
+    nodes (``mu_src`` ``(nshots, nsteps, S)``, row n = mu^{n+1}), then ``-G / kc`` on the interior."""
+    if mu_src is not None:
+        contrib = (mu_src.double() * sig.double()[:, :, : src.count]).sum(1)  # (nshots, count)
+        flat = Gn.reshape(-1)
+        for s, f in enumerate(src.nodes_flat):
+            if len(f):
+                flat.index_add_(0, torch.as_tensor(f, device=Gn.device), contrib[s, : len(f)])
+    grad = torch.zeros_like(Gn)
+    inner = tuple(slice(1, n - 1) for n in grid.shape)
+    grad[inner] = -Gn[inner] / kc.double()[inner]
+    return grad
+
+
 class GradientPlan:
     """Checkpointed forward + adjoint for ``nsteps`` steps from rest (``wfd_plan_*``).
 
@@ -392,17 +408,7 @@ class GradientPlan:
         Gn = g.nodes(G[None])[0].clone()
         self.last_G = Gn.clone()  # sum_n mu^{n+1} L u^n before the source term
         self.last_mu_src = mu_src
-        if mu_src is not None:
-            # source term of the imaging condition: sum_n mu^{n+1}(x_s) sig^n_s at the source nodes
-            contrib = (mu_src.double() * self._sig.double()[:, :, : src.count]).sum(1)  # (nshots, count)
-            flat = Gn.reshape(-1)
-            for s, f in enumerate(src.nodes_flat):
-                if len(f):
-                    flat.index_add_(0, torch.as_tensor(f, device=g.device), contrib[s, : len(f)])
-        kc = medium.kc_nodes().double()
+        kc = medium.kc_nodes()
         if kc.shape[0] != 1:
             raise ValueError("the summed gradient needs one medium shared by all shots")
-        grad = torch.zeros_like(Gn)
-        inner = tuple(slice(1, n - 1) for n in g.shape)
-        grad[inner] = -Gn[inner] / kc[0][inner]
-        return grad
+        return kc_gradient(g, Gn, mu_src, self._sig, src, kc[0])
