@@ -115,6 +115,97 @@ def fwi_main(args):
     print(json.dumps(out))
 
 
+def uneven_main(args):
+    """Uneven batch: 64 shots of different grid sizes (512^2-1536^2), step counts (1500-5000) and
+    orders (4/8), every shot's forward + adjoint gradient in one persistent launch replayed from a
+    hipGraph; against the same shots run one by one through the single-shot gradient plan."""
+    import torch
+
+    from neuralmech_amd import uneven
+
+    t0 = time.perf_counter()
+    shots = uneven.setup(nshots=args.uneven_shots, step_scale=args.uneven_scale)
+    t_setup = time.perf_counter() - t0
+    cs = uneven.cell_steps(shots)
+    t0 = time.perf_counter()
+    b = uneven.batched(shots, tune_steps=0 if args.no_tune else 10, max_active_cells=args.max_active)
+    t_tune = time.perf_counter() - t0
+    info = b.info()
+    classes = {}
+    for k in range(len(shots)):
+        c, v, r, seg, _, _ = b.shot_info(k)
+        classes[str(c)] = [v, r]
+    stream = torch.cuda.current_stream()
+    for _ in range(max(1, args.warmup if args.warmup < 10 else 1)):  # graph capture + replay
+        b.run(graph=True)
+    b.status()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    runs = max(1, args.uneven_runs)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(runs):
+        b.run(graph=True)
+    e1.record(stream)
+    b.status()
+    t_run = e0.elapsed_time(e1) / 1e3 / runs
+    t0 = time.perf_counter()
+    grads = b.gradients()
+    torch.cuda.synchronize()
+    t_fin = time.perf_counter() - t0
+    t_batch = t_run + t_fin
+    # one launch per phase, same tiles (the persistent kernel's barrier against kernel boundaries)
+    t_phases = None
+    if not args.no_phases:
+        bp = uneven.batched(shots, tune_steps=0, max_active_cells=args.max_active, mode="phases")
+        for c, (v, r) in classes.items():
+            bp.set_config(v, r, int(c))
+        bp.run(graph=True)
+        bp.status()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        bp.run(graph=True)
+        e1.record(stream)
+        bp.status()
+        t_phases = e0.elapsed_time(e1) / 1e3
+        del bp
+    del b
+    seq, t_seq = uneven.sequential(shots)
+    diff = max(float((a[1] - s_).norm() / s_.norm()) for a, s_ in zip(grads, seq))
+    nshots = len(shots)
+    gbs = 64.0 * cs / t_run / 1e9
+    out = {
+        "metric": "FWI gradient s/shot (uneven batch, one persistent launch)",
+        "value": round(t_batch / nshots, 5),
+        "unit": "s/shot",
+        "n_gpus": 1,
+        "steps": runs,
+        "warmup": 1,
+        "ms_per_step": round(t_run * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered models +-10 %, smoothed start models, Ricker shots)",
+        "config": {"workload": f"uneven batch: {nshots} shots, n ~ U{{512..1536}}^2, steps ~ U{{1500..5000}}"
+                               f" x {args.uneven_scale}, order 4/8, 201 receivers, sqrt(n) checkpointing, rng(4)",
+                   "shots": nshots, "cell_steps": cs, "phases": info["phases"], "blocks": info["blocks"],
+                   "store_gb": round(info["bytes"] / 1e9, 2), "tiles_per_class": classes},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "note": "64 algorithmic B per cell-step (forward 16 + recompute 20 + adjoint 28)"},
+        "gcell_updates_per_s": round(cs / t_run / 1e9, 2),
+        "batch_run_s": round(t_run, 4),
+        "finalize_s": round(t_fin, 4),
+        "per_phase_launches_s": None if t_phases is None else round(t_phases, 4),
+        "sequential_s_per_shot": round(t_seq / nshots, 5),
+        "speedup_vs_sequential": round(t_seq / t_batch, 2),
+        "grad_rel_l2_vs_sequential": diff,
+        "setup_s": round(t_setup, 1),
+        "tune_s": round(t_tune, 1),
+    }
+    print(json.dumps(out))
+
+
 def copy_main(args):
     """Device-to-device copy ceiling (the reference measures the same with a 1 GiB copy,
     2D_wave_roofline.py:46-57), at 1 GiB (HBM) and 64 MiB (fits the Infinity Cache); also the
@@ -211,13 +302,18 @@ def main():
     ap.add_argument("--graph", type=int, default=0, help="replay a hipGraph of this many steps (even)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch shape")
-    ap.add_argument("--workload", choices=["forward", "fwi", "copy", "3d"], default="forward")
+    ap.add_argument("--workload", choices=["forward", "fwi", "copy", "3d", "uneven"], default="forward")
     ap.add_argument("--launch", default="", help="force VARIANT,ROWS instead of tuning")
     ap.add_argument("--n3d", type=int, default=512)
     ap.add_argument("--steps3d", type=int, default=2000)
     ap.add_argument("--grad", action="store_true", help="3D: also time the checkpointed gradient")
     ap.add_argument("--fwi-steps", type=int, default=3000)
     ap.add_argument("--plain-shots", type=int, default=1)
+    ap.add_argument("--uneven-shots", type=int, default=64)
+    ap.add_argument("--uneven-scale", type=float, default=1.0, help="scale the drawn step counts")
+    ap.add_argument("--uneven-runs", type=int, default=2)
+    ap.add_argument("--max-active", type=int, default=0, help="uneven: cells in flight (0 = default)")
+    ap.add_argument("--no-phases", action="store_true", help="uneven: skip the per-phase launch timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the shot runs on a single GPU and this build has no "
@@ -231,6 +327,9 @@ def main():
         return
     if args.workload == "3d":
         volume_main(args)
+        return
+    if args.workload == "uneven":
+        uneven_main(args)
         return
 
     import torch

@@ -10,12 +10,16 @@
 // GPU); phase p then holds one item per shot active at p.  Items of a phase are laid end to end
 // as tiles (stencil tiles, then the 256-point gather tiles of the item).
 //
-// Device side.  A grid of resident blocks (8 * blocks-per-XCD) walks the phases.  In every phase
-// the tiles are split into 8 contiguous ranges, one per XCD (block b runs on XCD b % 8), and the
-// blocks of an XCD stride through its range; a grid barrier ends the phase.  The barrier is two
-// level (per-XCD arrival counter, one global counter of XCD leaders, a per-XCD release word),
-// every wait is bounded by a wall-clock limit that sets an error word and ends the kernel, so the
-// launch always terminates even if the grid were not co-resident.
+// Device side.  A grid of resident blocks (one per CU) is split into 8 groups by the XCD each
+// block runs on (the XCC_ID hardware register; blocks register at kernel start to learn their rank
+// and their group's size).  Shots are assigned to groups longest-processing-time first and each
+// group walks its own phases: its blocks stride through the tiles of the phase, then meet at a
+// group barrier.  All of a shot's fields are touched by one XCD only, so they stay coherent in
+// that XCD's L2: the barrier needs no L2 write-back, only the store-completion wait and one cache
+// invalidation per block, and no group ever waits for another.  Every wait is bounded by a
+// wall-clock limit that sets an error word and ends the kernel, so the launch always terminates
+// even if the grid were not co-resident.  The per-phase mode (one launch per phase of a single
+// global schedule) shares the tile code.
 //
 // Fusions (the same bits as the separate launches of the gradient plan): the imaging partial is
 // flushed to the float64 sum inside the adjoint step that would be followed by a flush; the
@@ -72,10 +76,11 @@ struct BItem {
 static_assert(sizeof(BItem) == 32, "BItem layout");
 
 struct BatchSync {
-    uint32_t arrive[XCDS * 32];  // per-XCD arrival counters, 128 bytes apart
-    uint32_t release[XCDS * 32];
-    uint32_t global[32];
-    uint32_t error[32];
+    uint32_t arrive[XCDS * 32];   // per-group arrival counters, 128 bytes apart
+    uint32_t release[XCDS * 32];  // per-group released generation
+    uint32_t reg[XCDS * 32];      // blocks registered per XCD
+    uint32_t reg_all[32];         // blocks registered in total
+    uint32_t error[32];           // bit 0: a wait timed out; bit 1: a group with work has no block
 };
 
 template <typename T>
@@ -89,51 +94,47 @@ struct BatchArgs {
     int32_t persistent;
 };
 
-__device__ __forceinline__ bool batch_barrier(BatchSync* s, uint32_t gen, uint32_t bpx, uint64_t timeout)
+// Phase barrier of one XCD group.  Every block of the group runs on the same XCD (checked against
+// the XCC_ID hardware register when the kernel starts), so the group's fields live in one L2: a
+// wave's stores are visible to the group once they have completed (s_waitcnt vmcnt(0)) and no L2
+// write-back is needed.  What a CU may still hold are stale L1 lines of tiles other CUs rewrote;
+// on gfx950 only the device-scope invalidate clears them (a workgroup-scope one leaves the L1 of a
+// non-split dispatch alone), so one thread per block issues it and waits before the block goes on.
+// The last block to arrive publishes the generation in a release word the others poll (polling
+// the arrival counter itself slows the arrivals); every wait is bounded.
+__device__ __forceinline__ bool group_barrier(BatchSync* s, uint32_t g, uint32_t gen, uint32_t bpx, uint64_t timeout)
 {
-    // every wave: its stores complete and written back from this XCD's L2
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's stores are in the L2
     __syncthreads();
     __shared__ uint32_t s_ok;
     if (threadIdx.x == 0) {
-        const uint32_t x = blockIdx.x % XCDS;
         uint32_t ok = 1;
         const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-        auto stop = [&](uint32_t it) {
-            if ((it & 31) != 31) return false;
-            if (__hip_atomic_load(&s->error[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return true;
-            return __builtin_amdgcn_s_memrealtime() - t0 > timeout;
-        };
-        const uint32_t old = __hip_atomic_fetch_add(&s->arrive[x * 32], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (old + 1 == gen * bpx) {  // last block of this XCD: report to the global counter
-            __hip_atomic_fetch_add(&s->global[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            for (uint32_t it = 0;
-                 __hip_atomic_load(&s->global[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen * XCDS; it++) {
-                if (stop(it)) {
-                    ok = 0;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
-            __hip_atomic_store(&s->release[x * 32], gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t old = __hip_atomic_fetch_add(&s->arrive[g * 32], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old + 1 == gen * bpx) {
+            __hip_atomic_store(&s->release[g * 32], gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             for (uint32_t it = 0;
-                 __hip_atomic_load(&s->release[x * 32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen; it++) {
-                if (stop(it)) {
+                 __hip_atomic_load(&s->release[g * 32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gen; it++) {
+                if ((it & 31) == 31 && (__hip_atomic_load(&s->error[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                                        __builtin_amdgcn_s_memrealtime() - t0 > timeout)) {
                     ok = 0;
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
         }
-        if (!ok) __hip_atomic_store(&s->error[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // fields written by other XCDs in the phase: drop stale L1 / L2 lines
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (!ok) __hip_atomic_fetch_or(&s->error[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        asm volatile("buffer_inv sc1" ::: "memory");  // this CU's L1 (and the XCD's L2)
+        __builtin_amdgcn_s_waitcnt(0x0F70);
         s_ok = ok;
     }
     __syncthreads();
     return s_ok != 0;
 }
+
+// XCC (XCD) that runs the calling wave, from the XCC_ID hardware register.
+__device__ __forceinline__ uint32_t hw_xcc_id() { return __builtin_amdgcn_s_getreg((15 << 11) | 20) & 15u; }
 
 template <typename T, int K, bool IMG>
 __device__ __forceinline__ void batch_tile_k(const StepArgs<T>& a, int cfg, uint32_t tx, uint32_t ty, int64_t step)
@@ -194,7 +195,7 @@ __device__ __forceinline__ void batch_item(const BShot<T>& sh, const BItem& it, 
     const uint32_t gt = it.kind == K_GATHER ? local : local - sh.stiles;
     const bool rec = (it.flags & F_GREC) != 0;
     const PtsDev<T>& p = rec ? sh.rec : sh.msrc;
-    const int64_t q = (int64_t)gt * 256 + threadIdx.x;
+    const int64_t q = (int64_t)gt * 256 + (threadIdx.x & 255);
     if (q >= p.count || it.grow < 0 || it.grow >= p.nt) return;
     const T* f = F + it.gf * fs;
     const T u = f[(int64_t)p.row[q] * sh.a.ld + p.col[q]];
@@ -203,24 +204,81 @@ __device__ __forceinline__ void batch_item(const BShot<T>& sh, const BItem& it, 
     if (rec) sh.adj.data[d] = -(u - sh.obs[d]);
 }
 
+struct GroupPhases {
+    int64_t b[XCDS + 1];  // group g runs phases [b[g], b[g+1])
+};
+
+// Threads per block of the persistent launch: one block per CU, made of 256-thread tile slots
+// (4 waves each).  float32: 3 slots (12 waves, 3 per SIMD at <= 168 VGPRs); float64: 1 slot.
+template <typename T>
+constexpr int batch_block() { return sizeof(T) == 4 ? 768 : 256; }
+
 // Read-only tables as separate restrict parameters: their loads are provably unclobbered by
 // the field stores, so uniform ones become scalar loads.
+//
+// persistent != 0: group g = the XCD a block runs on (XCC_ID register) runs its own phases; the
+// tile slots of the group's blocks stride through each phase's tiles and meet at group_barrier.
+// persistent == 0: one launch per phase (p0 = p1 - 1), tiles split into 8 contiguous ranges, one
+// per XCD under the usual round-robin placement of blocks.
 template <typename T>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) void batch_kernel(const BShot<T>* __restrict__ shots, const BItem* __restrict__ items,
-                                                    const uint32_t* __restrict__ phase_off, int64_t p0, int64_t p1,
-                                                    BatchSync* sync, uint64_t timeout, int32_t persistent)
+__global__ __launch_bounds__(batch_block<T>(), sizeof(T) == 4 ? 3 : 1) void batch_kernel(
+    const BShot<T>* __restrict__ shots, const BItem* __restrict__ items, const uint32_t* __restrict__ phase_off,
+    int64_t p0, int64_t p1, GroupPhases gp, BatchSync* sync, uint64_t timeout, int32_t persistent)
 {
     __shared__ uint32_t s_t0[BATCH_MAX_ITEMS + 1];
-    const uint32_t x = blockIdx.x % XCDS, bi = blockIdx.x / XCDS, bpx = gridDim.x / XCDS;
+    const uint32_t x = blockIdx.x % XCDS;
+    const uint32_t nsub = blockDim.x >> 8, sub = threadIdx.x >> 8;  // tile slots of this block
+    uint32_t g = x, bi = blockIdx.x / XCDS, bpx = gridDim.x / XCDS;
+    if (persistent) {
+        // group = the XCD this block runs on; rank and count of the group's blocks by registration
+        // (the XCD the hardware starts its round-robin placement on varies between launches)
+        __shared__ uint32_t s_reg[3];
+        if (threadIdx.x == 0) {
+            const uint32_t xcc = hw_xcc_id();
+            uint32_t ok = xcc < XCDS ? 1u : 0u;
+            const uint32_t gg = ok ? xcc : 0u;
+            const uint32_t rank = __hip_atomic_fetch_add(&sync->reg[gg * 32], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&sync->reg_all[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            for (uint32_t it = 0;
+                 __hip_atomic_load(&sync->reg_all[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gridDim.x; it++) {
+                if ((it & 31) == 31 && (__hip_atomic_load(&sync->error[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                                        __builtin_amdgcn_s_memrealtime() - t0 > timeout)) {
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            for (uint32_t k = 0; k < XCDS && ok; k++)  // every group with phases needs a block
+                if (gp.b[k + 1] > gp.b[k] &&
+                    __hip_atomic_load(&sync->reg[k * 32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+                    ok = 0;
+            if (!ok) __hip_atomic_fetch_or(&sync->error[0], 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_reg[0] = ok;
+            s_reg[1] = gg;
+            s_reg[2] = rank;
+        }
+        __syncthreads();
+        if (!s_reg[0]) return;
+        g = s_reg[1];
+        bi = s_reg[2];
+        bpx = __hip_atomic_load(&sync->reg[g * 32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        p0 = gp.b[g];
+        p1 = gp.b[g + 1];
+    }
     for (int64_t p = p0; p < p1; p++) {
         const uint32_t i0 = phase_off[p], n = phase_off[p + 1] - i0;
         for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) s_t0[t] = items[i0 + t].tile0;
         if (threadIdx.x == 0) s_t0[n] = n ? items[i0 + n - 1].tile0 + items[i0 + n - 1].ntiles : 0u;
         __syncthreads();
         const uint32_t total = s_t0[n];
-        const uint32_t per = total / XCDS, rem = total % XCDS;
-        const uint32_t lo = x * per + (x < rem ? x : rem), hi = lo + per + (x < rem ? 1u : 0u);
-        for (uint32_t j = lo + bi; j < hi; j += bpx) {
+        uint32_t lo = 0, hi = total;
+        if (!persistent) {
+            const uint32_t per = total / XCDS, rem = total % XCDS;
+            lo = x * per + (x < rem ? x : rem);
+            hi = lo + per + (x < rem ? 1u : 0u);
+        }
+        for (uint32_t j = lo + bi * nsub + sub; j < hi; j += bpx * nsub) {
             uint32_t a = 0, b = n;  // largest e with s_t0[e] <= j
             while (b - a > 1) {
                 const uint32_t m = (a + b) >> 1;
@@ -234,7 +292,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) void batch_kernel(cons
         }
         if (p + 1 < p1) {
             if (persistent) {
-                if (!batch_barrier(sync, (uint32_t)(p - p0 + 1), bpx, timeout)) return;
+                if (!group_barrier(sync, g, (uint32_t)(p - p0 + 1), bpx, timeout)) return;
             } else {
                 __syncthreads();  // s_t0 reuse
             }
@@ -243,17 +301,19 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 3 : 1) void batch_kernel(cons
 }
 
 template <typename T>
-hipError_t launch_batch(const BatchArgs<T>& A, uint32_t blocks, hipStream_t s)
+hipError_t launch_batch(const BatchArgs<T>& A, const GroupPhases& gp, uint32_t blocks, int threads, hipStream_t s)
 {
-    hipLaunchKernelGGL((batch_kernel<T>), dim3(blocks), dim3(256), 0, s, A.shots, A.items, A.phase_off, A.p0, A.p1,
-                       A.sync, A.timeout, A.persistent);
+    hipLaunchKernelGGL((batch_kernel<T>), dim3(blocks), dim3(threads), 0, s, A.shots, A.items, A.phase_off, A.p0,
+                       A.p1, gp, A.sync, A.timeout, A.persistent);
     return hipGetLastError();
 }
 
 template <typename T>
 int batch_occupancy(int* blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batch_kernel<T>, 256, 0) == hipSuccess ? 0 : 1;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batch_kernel<T>, batch_block<T>(), 0) == hipSuccess
+               ? 0
+               : 1;
 }
 
 }  // namespace wfd
@@ -387,23 +447,37 @@ constexpr int NCLASS = 5;
 
 }  // namespace
 
+// A schedule: the phase tables on the device and, per group, its range of phases.
+struct Sched {
+    BItem* items = nullptr;
+    uint32_t* off = nullptr;
+    int64_t nphases = 0;
+    GroupPhases gp{};
+    std::vector<uint32_t> tiles;          // tiles per phase
+    std::vector<int64_t> start, group;    // per shot: global first phase, group
+    void release()
+    {
+        (void)hipFree(items);
+        (void)hipFree(off);
+        items = nullptr;
+        off = nullptr;
+    }
+};
+
 struct wfd_batch {
     int dtype = WFD_F32;
     int nshots = 0;
     wfd_batch_opts opts{};
     std::vector<wfd_batch_shot> in;
     std::vector<std::vector<HostOp>> ops;
-    std::vector<int64_t> seg, nfields, start, fbytes;
+    std::vector<int64_t> seg, nfields, fbytes;
     std::vector<int> cfg, cls;
-    int64_t nphases = 0;
     size_t bytes = 0;
     char* store = nullptr;  // all shots' fields
     void* zprof = nullptr;  // zero damping profile for shots without a sponge
     void* shots_dev = nullptr;
-    BItem* items_dev = nullptr;
-    uint32_t* off_dev = nullptr;
     BatchSync* sync = nullptr;
-    std::vector<uint32_t> phase_tiles;
+    Sched sched;
     uint32_t blocks = 0;
     uint64_t timeout_ticks = 0;
     hipStream_t cap_stream = nullptr;
@@ -512,16 +586,65 @@ uint32_t op_tiles(const wfd_batch* b, int s, const HostOp& o)
     return t;
 }
 
-// Phase tables of a schedule: shot s runs ops[s] from phase start[s].
-int upload_schedule(wfd_batch* b, const std::vector<std::vector<HostOp>>& ops, const std::vector<int64_t>& start,
-                    BItem** items_out, uint32_t** off_out, int64_t* nphases_out, std::vector<uint32_t>* tiles_out)
+int64_t shot_cells(const wfd_batch* b, int s) { return (b->in[s].grid.n[0] - 2) * (b->in[s].grid.n[1] - 2); }
+
+// Shots -> groups (persistent: the 8 XCDs, longest processing time first on cells x ops), then
+// per group a list schedule: longest shots first, each started at the first phase where the
+// group's cells in flight stay within its share of max_active_cells.
+int make_schedule(wfd_batch* b, const std::vector<std::vector<HostOp>>& ops, Sched& out)
 {
-    int64_t P = 0;
-    for (int s = 0; s < b->nshots; s++)
-        if (!ops[s].empty()) P = std::max<int64_t>(P, start[s] + (int64_t)ops[s].size());
+    const int S = b->nshots;
+    const bool grouped = b->opts.mode == 0;
+    const int G = grouped ? (int)XCDS : 1;
+    std::vector<int> order;
+    for (int s = 0; s < S; s++)
+        if (!ops[s].empty()) order.push_back(s);
+    auto work = [&](int s) { return (double)shot_cells(b, s) * (double)ops[s].size(); };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const size_t a = ops[x].size(), c = ops[y].size();
+        return a != c ? a > c : shot_cells(b, x) > shot_cells(b, y);
+    });
+    out.group.assign(S, 0);
+    if (grouped) {
+        std::vector<int> by_work(order);
+        std::stable_sort(by_work.begin(), by_work.end(), [&](int x, int y) { return work(x) > work(y); });
+        std::vector<double> load(G, 0.0);
+        for (int s : by_work) {
+            const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            out.group[s] = g;
+            load[g] += work(s);
+        }
+    }
+    const int64_t total_target = b->opts.max_active_cells > 0 ? b->opts.max_active_cells : (int64_t)12 * 1024 * 1024;
+    const int64_t target = std::max<int64_t>(1, total_target / G);
+    std::vector<int64_t> local(S, 0), gphases(G, 0);
+    for (int g = 0; g < G; g++) {
+        using Ev = std::pair<int64_t, int64_t>;  // (end phase, cells)
+        std::priority_queue<Ev, std::vector<Ev>, std::greater<Ev>> act;
+        int64_t t = 0, cur = 0;
+        for (int s : order) {
+            if (out.group[s] != g) continue;
+            while (!act.empty() && (cur + shot_cells(b, s) > target || (int)act.size() >= BATCH_MAX_ITEMS)) {
+                t = std::max(t, act.top().first);
+                while (!act.empty() && act.top().first <= t) {
+                    cur -= act.top().second;
+                    act.pop();
+                }
+            }
+            local[s] = t;
+            act.push({t + (int64_t)ops[s].size(), shot_cells(b, s)});
+            cur += shot_cells(b, s);
+            gphases[g] = std::max<int64_t>(gphases[g], t + (int64_t)ops[s].size());
+        }
+    }
+    std::memset(&out.gp, 0, sizeof out.gp);
+    for (int g = 0; g < (int)XCDS; g++) out.gp.b[g + 1] = out.gp.b[g] + (g < G ? gphases[g] : 0);
+    const int64_t P = out.gp.b[XCDS];
+    out.start.assign(S, 0);
+    for (int s = 0; s < S; s++) out.start[s] = out.gp.b[out.group[s]] + local[s];
     std::vector<uint32_t> count(P + 1, 0);
-    for (int s = 0; s < b->nshots; s++)
-        for (size_t l = 0; l < ops[s].size(); l++) count[start[s] + l]++;
+    for (int s = 0; s < S; s++)
+        for (size_t l = 0; l < ops[s].size(); l++) count[out.start[s] + l]++;
     std::vector<uint32_t> off(P + 1, 0);
     for (int64_t p = 0; p < P; p++) {
         if (count[p] > (uint32_t)BATCH_MAX_ITEMS) return bfail(WFD_EINVAL, "wfd_batch: more than %d shots in one phase", BATCH_MAX_ITEMS);
@@ -529,15 +652,15 @@ int upload_schedule(wfd_batch* b, const std::vector<std::vector<HostOp>>& ops, c
     }
     std::vector<BItem> items(off[P]);
     std::vector<uint32_t> fill(off.begin(), off.end() - 1);
-    std::vector<uint32_t> tiles(P, 0);
-    for (int s = 0; s < b->nshots; s++) {  // shot order inside a phase
+    out.tiles.assign(P, 0);
+    for (int s = 0; s < S; s++) {  // shot order inside a phase
         for (size_t l = 0; l < ops[s].size(); l++) {
-            const int64_t p = start[s] + (int64_t)l;
+            const int64_t p = out.start[s] + (int64_t)l;
             const HostOp& o = ops[s][l];
             BItem& it = items[fill[p]++];
             it.ntiles = op_tiles(b, s, o);
-            it.tile0 = tiles[p];
-            tiles[p] += it.ntiles;
+            it.tile0 = out.tiles[p];
+            out.tiles[p] += it.ntiles;
             it.shot = (uint16_t)s;
             it.kind = (uint8_t)o.kind;
             it.flags = (uint8_t)o.flags;
@@ -551,113 +674,63 @@ int upload_schedule(wfd_batch* b, const std::vector<std::vector<HostOp>>& ops, c
             it.grow = o.grow;
         }
     }
-    BItem* di = nullptr;
-    uint32_t* doff = nullptr;
-    if (hipMalloc(&di, std::max<size_t>(1, items.size()) * sizeof(BItem)) != hipSuccess ||
-        hipMalloc(&doff, off.size() * sizeof(uint32_t)) != hipSuccess) {
-        (void)hipFree(di);
+    if (hipMalloc(&out.items, std::max<size_t>(1, items.size()) * sizeof(BItem)) != hipSuccess ||
+        hipMalloc(&out.off, off.size() * sizeof(uint32_t)) != hipSuccess) {
+        out.release();
         return bfail(WFD_ENOMEM, "wfd_batch: hipMalloc of the phase tables failed");
     }
-    if ((!items.empty() && hipMemcpy(di, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice) != hipSuccess) ||
-        hipMemcpy(doff, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipFree(di);
-        (void)hipFree(doff);
+    if ((!items.empty() && hipMemcpy(out.items, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(out.off, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        out.release();
         return bfail(WFD_EHIP, "wfd_batch: upload of the phase tables failed");
     }
-    *items_out = di;
-    *off_out = doff;
-    *nphases_out = P;
-    if (tiles_out) *tiles_out = tiles;
+    out.nphases = P;
     return WFD_OK;
-}
-
-// Start phases: longest shots first, each started as soon as the cells in flight allow.
-void list_schedule(wfd_batch* b)
-{
-    const int S = b->nshots;
-    std::vector<int> order(S);
-    std::iota(order.begin(), order.end(), 0);
-    auto cells = [&](int s) { return (b->in[s].grid.n[0] - 2) * (b->in[s].grid.n[1] - 2); };
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        const size_t a = b->ops[x].size(), c = b->ops[y].size();
-        return a != c ? a > c : cells(x) > cells(y);
-    });
-    const int64_t target = b->opts.max_active_cells > 0 ? b->opts.max_active_cells : (int64_t)12 * 1024 * 1024;
-    using Ev = std::pair<int64_t, int64_t>;  // (end phase, cells)
-    std::priority_queue<Ev, std::vector<Ev>, std::greater<Ev>> act;
-    int64_t t = 0, cur = 0;
-    b->start.assign(S, 0);
-    for (int s : order) {
-        while (!act.empty() && (cur + cells(s) > target || (int)act.size() >= BATCH_MAX_ITEMS)) {
-            t = std::max(t, act.top().first);
-            while (!act.empty() && act.top().first <= t) {
-                cur -= act.top().second;
-                act.pop();
-            }
-        }
-        b->start[s] = t;
-        act.push({t + (int64_t)b->ops[s].size(), cells(s)});
-        cur += cells(s);
-    }
-}
-
-void free_tables(wfd_batch* b)
-{
-    (void)hipFree(b->items_dev);
-    (void)hipFree(b->off_dev);
-    b->items_dev = nullptr;
-    b->off_dev = nullptr;
 }
 
 int rebuild(wfd_batch* b)
 {
-    free_tables(b);
+    b->sched.release();
     if (b->exec) {
         (void)hipGraphExecDestroy(b->exec);
         b->exec = nullptr;
     }
     int rc = upload_shot_table(b);
     if (rc) return rc;
-    list_schedule(b);
-    return upload_schedule(b, b->ops, b->start, &b->items_dev, &b->off_dev, &b->nphases, &b->phase_tiles);
+    return make_schedule(b, b->ops, b->sched);
 }
 
 template <typename T>
-int launch_t(wfd_batch* b, const BItem* items, const uint32_t* off, int64_t nphases,
-             const std::vector<uint32_t>& tiles, hipStream_t s)
+int launch_t(wfd_batch* b, const Sched& sc, hipStream_t s)
 {
     hipError_t e = launch_zero(b->sync, sizeof(BatchSync), s);
     if (e != hipSuccess) return bfail(WFD_EHIP, "wfd_batch: reset of the barrier state failed");
     BatchArgs<T> A;
     std::memset(&A, 0, sizeof A);
     A.shots = static_cast<const BShot<T>*>(b->shots_dev);
-    A.items = items;
-    A.phase_off = off;
+    A.items = sc.items;
+    A.phase_off = sc.off;
     A.sync = b->sync;
     A.timeout = b->timeout_ticks;
     if (b->opts.mode == 0) {
-        A.p0 = 0;
-        A.p1 = nphases;
         A.persistent = 1;
-        e = launch_batch<T>(A, b->blocks, s);
+        e = launch_batch<T>(A, sc.gp, b->blocks, batch_block<T>(), s);
         if (e != hipSuccess) return bfail(WFD_EHIP, "wfd_batch: persistent launch failed (%s)", hipGetErrorString(e));
         return WFD_OK;
     }
-    for (int64_t p = 0; p < nphases; p++) {
+    for (int64_t p = 0; p < sc.nphases; p++) {
         A.p0 = p;
         A.p1 = p + 1;
-        const uint32_t blocks = std::max<uint32_t>(XCDS, (tiles[p] + XCDS - 1) / XCDS * XCDS);
-        e = launch_batch<T>(A, blocks, s);
+        const uint32_t blocks = std::max<uint32_t>(XCDS, (sc.tiles[p] + XCDS - 1) / XCDS * XCDS);
+        e = launch_batch<T>(A, sc.gp, blocks, 256, s);
         if (e != hipSuccess) return bfail(WFD_EHIP, "wfd_batch: phase launch failed (%s)", hipGetErrorString(e));
     }
     return WFD_OK;
 }
 
-int launch_sched(wfd_batch* b, const BItem* items, const uint32_t* off, int64_t nphases,
-                 const std::vector<uint32_t>& tiles, hipStream_t s)
+int launch_sched(wfd_batch* b, const Sched& sc, hipStream_t s)
 {
-    return b->dtype == WFD_F64 ? launch_t<double>(b, items, off, nphases, tiles, s)
-                               : launch_t<float>(b, items, off, nphases, tiles, s);
+    return b->dtype == WFD_F64 ? launch_t<double>(b, sc, s) : launch_t<float>(b, sc, s);
 }
 
 int check_shot(const wfd_batch_shot& sh, int i, int dtype)
@@ -777,7 +850,7 @@ int wfd_batch_destroy(wfd_batch* b)
     if (!b) return WFD_OK;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
-    free_tables(b);
+    b->sched.release();
     (void)hipFree(b->shots_dev);
     (void)hipFree(b->sync);
     (void)hipFree(b->zprof);
@@ -789,7 +862,7 @@ int wfd_batch_destroy(wfd_batch* b)
 int wfd_batch_info(const wfd_batch* b, int64_t* phases, int64_t* blocks, size_t* bytes, int64_t* cell_steps)
 {
     if (!b) return bfail(WFD_EINVAL, "wfd_batch_info: NULL batch");
-    if (phases) *phases = b->nphases;
+    if (phases) *phases = b->sched.nphases;
     if (blocks) *blocks = b->opts.mode == 0 ? (int64_t)b->blocks : 0;
     if (bytes) *bytes = b->bytes;
     if (cell_steps) {
@@ -808,7 +881,7 @@ int wfd_batch_shot_info(const wfd_batch* b, int32_t shot, int32_t* size_cls, int
     if (variant) *variant = cfg_variant(b->cfg[shot]);
     if (rows) *rows = b->dtype == WFD_F64 ? 2 : cfg_rows(b->cfg[shot]);
     if (seg) *seg = b->seg[shot];
-    if (start_phase) *start_phase = b->start[shot];
+    if (start_phase) *start_phase = b->sched.start[shot];
     if (nphases) *nphases = (int64_t)b->ops[shot].size();
     return WFD_OK;
 }
@@ -827,13 +900,13 @@ int wfd_batch_run(wfd_batch* b, int32_t use_graph, wfd_stream_t stream)
         }
         b->dirty = false;
     }
-    if (!use_graph) return launch_sched(b, b->items_dev, b->off_dev, b->nphases, b->phase_tiles, s);
+    if (!use_graph) return launch_sched(b, b->sched, s);
     if (!b->exec) {
         if (!b->cap_stream && hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess)
             return bfail(WFD_EHIP, "wfd_batch_run: stream creation failed");
         if (hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
             return bfail(WFD_EHIP, "wfd_batch_run: hipStreamBeginCapture failed");
-        int rc = launch_sched(b, b->items_dev, b->off_dev, b->nphases, b->phase_tiles, b->cap_stream);
+        int rc = launch_sched(b, b->sched, b->cap_stream);
         hipGraph_t graph = nullptr;
         const hipError_t e = hipStreamEndCapture(b->cap_stream, &graph);
         if (rc) {
@@ -861,7 +934,10 @@ int wfd_batch_status(wfd_batch* b, wfd_stream_t stream)
         return bfail(WFD_EHIP, "wfd_batch_status: reading the barrier state failed");
     if (err) {
         b->dirty = true;
-        return bfail(WFD_EHIP, "wfd_batch: a grid barrier wait timed out (blocks not co-resident?)");
+        if (err & 2u)
+            return bfail(WFD_EHIP, "wfd_batch: the persistent grid was not resident on every XCD with work "
+                                   "(use mode 1)");
+        return bfail(WFD_EHIP, "wfd_batch: a group barrier wait timed out (blocks not co-resident?)");
     }
     return WFD_OK;
 }
@@ -890,7 +966,6 @@ int wfd_batch_tune(wfd_batch* b, int32_t steps, wfd_stream_t stream)
             }
         }
         if (!any) continue;
-        const std::vector<int64_t> zero(b->nshots, 0);
         double best_t = 1e30;
         int best = cands[0];
         for (int cand : cands) {
@@ -898,16 +973,13 @@ int wfd_batch_tune(wfd_batch* b, int32_t steps, wfd_stream_t stream)
                 if (b->cls[sh] == c) b->cfg[sh] = cand;
             rc = upload_shot_table(b);
             if (rc) break;
-            BItem* items = nullptr;
-            uint32_t* off = nullptr;
-            int64_t P = 0;
-            std::vector<uint32_t> tiles;
-            rc = upload_schedule(b, probe, zero, &items, &off, &P, &tiles);
+            Sched probe_sched;
+            rc = make_schedule(b, probe, probe_sched);
             if (rc) break;
             double t = 1e30;
             for (int r = 0; r < 3 && !rc; r++) {
                 (void)hipEventRecord(e0, s);
-                rc = launch_sched(b, items, off, P, tiles, s);
+                rc = launch_sched(b, probe_sched, s);
                 (void)hipEventRecord(e1, s);
                 if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = bfail(WFD_EHIP, "wfd_batch_tune: launch failed");
                 float ms = 0.f;
@@ -915,8 +987,7 @@ int wfd_batch_tune(wfd_batch* b, int32_t steps, wfd_stream_t stream)
                 if (r) t = std::min(t, (double)ms);  // first run warms up
             }
             if (!rc) rc = wfd_batch_status(b, stream);
-            (void)hipFree(items);
-            (void)hipFree(off);
+            probe_sched.release();
             if (rc) break;
             if (t < best_t) {
                 best_t = t;

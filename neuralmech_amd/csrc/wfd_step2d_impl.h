@@ -118,7 +118,7 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
     constexpr int NW = RI + 2 * KR;
     constexpr int64_t COLS = XL ? 248 : 256;
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wave = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) & 3);  // wave of the 4-wave tile
     const int64_t base = (int64_t)tx * COLS - (XL ? 4 : 0);  // column of lane 0's chunk
     const int64_t c = base + lane * 4;
     const int64_t r0 = a.row_lo + ((int64_t)ty * 4 + wave) * RI;
