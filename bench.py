@@ -169,8 +169,10 @@ def uneven_main(args):
         t_phases = e0.elapsed_time(e1) / 1e3
         del bp
     del b
-    seq, t_seq = uneven.sequential(shots)
-    diff = max(float((a[1] - s_).norm() / s_.norm()) for a, s_ in zip(grads, seq))
+    t_seq, diff = None, None
+    if not args.no_sequential:
+        seq, t_seq = uneven.sequential(shots)
+        diff = max(float((a[1] - s_).norm() / s_.norm()) for a, s_ in zip(grads, seq))
     nshots = len(shots)
     gbs = 64.0 * cs / t_run / 1e9
     out = {
@@ -197,8 +199,8 @@ def uneven_main(args):
         "batch_run_s": round(t_run, 4),
         "finalize_s": round(t_fin, 4),
         "per_phase_launches_s": None if t_phases is None else round(t_phases, 4),
-        "sequential_s_per_shot": round(t_seq / nshots, 5),
-        "speedup_vs_sequential": round(t_seq / t_batch, 2),
+        "sequential_s_per_shot": None if t_seq is None else round(t_seq / nshots, 5),
+        "speedup_vs_sequential": None if t_seq is None else round(t_seq / t_batch, 2),
         "grad_rel_l2_vs_sequential": diff,
         "setup_s": round(t_setup, 1),
         "tune_s": round(t_tune, 1),
@@ -314,6 +316,7 @@ def main():
     ap.add_argument("--uneven-runs", type=int, default=2)
     ap.add_argument("--max-active", type=int, default=0, help="uneven: cells in flight (0 = default)")
     ap.add_argument("--no-phases", action="store_true", help="uneven: skip the per-phase launch timing")
+    ap.add_argument("--no-sequential", action="store_true", help="uneven: skip the shot-by-shot baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the shot runs on a single GPU and this build has no "

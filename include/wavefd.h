@@ -2,7 +2,7 @@
  *
  * Plain pointers and sizes only.  Every device pointer is a HIP device allocation owned by the
  * caller (typically a torch tensor's data_ptr()); the library allocates device memory only
- * inside wfd_plan objects.  Every launch is stream-ordered on the caller's hipStream_t (passed as
+ * inside wfd_plan and wfd_batch objects.  Every launch is stream-ordered on the caller's hipStream_t (passed as
  * void*, NULL = the default stream) and the library never synchronises the host except where a
  * function says so.  Functions return WFD_OK (0) or a positive status; wfd_last_error() gives a
  * message (thread-local).
@@ -261,10 +261,10 @@ int wfd_batch_info(const wfd_batch* batch, int64_t* phases, int64_t* blocks, siz
 int wfd_batch_shot_info(const wfd_batch* batch, int32_t shot, int32_t* size_class,
                         int32_t* variant, int32_t* rows, int64_t* seg, int64_t* start_phase,
                         int64_t* nphases);
-/* Host-only: the op list of one shot (10 int32 per op: kind 0 gather / 1 step / 2 adjoint step,
+/* Host-only: the op list of one shot (11 int32 per op: kind 0 gather / 1 step / 2 adjoint step,
  * flags 1 inject / 2 flush / 4 first flush / 8 gather receivers / 16 gather sources, field
- * indices up, uc, un, uf, gp, gf (-1 = none), signal row, trace row).  ops may be NULL to query
- * the count. */
+ * indices up, uc, un, uf, gp, gf (-1 = none), signal row, trace row, phase; ops of one phase run
+ * concurrently).  ops may be NULL to query the count. */
 int wfd_batch_shot_ops(int64_t nsteps, int64_t seg, int32_t has_src, int32_t has_rec,
                        int32_t* ops, int64_t cap, int64_t* count, int32_t* nfields);
 

@@ -174,13 +174,13 @@ class ShotBatch:
 
 
 def shot_ops(nsteps: int, seg: int = 0, has_src: bool = True, has_rec: bool = True):
-    """The op list the batch runs for one shot, ``(ops (n, 10) int32, nfields)`` (host only):
-    columns kind, flags, up, uc, un, uf, gp, gf, step, row (see ``wfd_batch_shot_ops``)."""
+    """The op list the batch runs for one shot, ``(ops (n, 11) int32, nfields)`` (host only):
+    columns kind, flags, up, uc, un, uf, gp, gf, step, row, phase (see ``wfd_batch_shot_ops``)."""
     lib = _lib.load()
     cnt, nf = ctypes.c_int64(), ctypes.c_int32()
     check(lib.wfd_batch_shot_ops(int(nsteps), int(seg), int(has_src), int(has_rec), None, 0, ctypes.byref(cnt),
                                  ctypes.byref(nf)))
-    ops = np.zeros((cnt.value, 10), np.int32)
+    ops = np.zeros((cnt.value, 11), np.int32)
     check(lib.wfd_batch_shot_ops(int(nsteps), int(seg), int(has_src), int(has_rec),
                                  ops.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), cnt.value, ctypes.byref(cnt),
                                  ctypes.byref(nf)))

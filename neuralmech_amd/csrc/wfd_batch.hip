@@ -42,7 +42,7 @@
 namespace wfd {
 
 // ------------------------------------------------------------------------------ device side
-constexpr int BATCH_MAX_ITEMS = 256;  // active shots per phase
+constexpr int BATCH_MAX_ITEMS = 256;  // items per phase (a shot has one or two per phase)
 constexpr uint32_t F_INJECT = 1, F_FLUSH = 2, F_FIRST = 4, F_GREC = 8, F_GSRC = 16;
 constexpr int K_GATHER = 0, K_STEP = 1, K_ADJ = 2;
 
@@ -59,7 +59,8 @@ struct BShot {
                                     // recording traces, sources recording mu
     const T* obs;                   // observed traces [nsteps][rec.width]
     double* G;                      // node (0, 0) of the imaging sum
-    uint32_t gx, gy, stiles;
+    uint32_t gx, gy, stiles;        // tile grid of forward / recompute steps
+    uint32_t gy_adj, stiles_adj;    // adjoint steps (2 rows per wave)
     int32_t K, cfg;
 };
 
@@ -136,20 +137,23 @@ __device__ __forceinline__ bool group_barrier(BatchSync* s, uint32_t g, uint32_t
 // XCC (XCD) that runs the calling wave, from the XCC_ID hardware register.
 __device__ __forceinline__ uint32_t hw_xcc_id() { return __builtin_amdgcn_s_getreg((15 << 11) | 20) & 15u; }
 
+// Forward / recompute tiles march cfg_rows(cfg) rows per wave; adjoint tiles (two register
+// windows) and every float64 tile march 2, which keeps the kernel at 3 waves per SIMD.  At that
+// occupancy the tiles load all their rows' operands up front (PF = rows).
 template <typename T, int K, bool IMG>
 __device__ __forceinline__ void batch_tile_k(const StepArgs<T>& a, int cfg, uint32_t tx, uint32_t ty, int64_t step)
 {
-    if constexpr (sizeof(T) == 8) {  // fp64: two rows per wave keep the register windows small
+    if constexpr (sizeof(T) == 8 || IMG) {
         if (cfg_variant(cfg) == 1)
-            step2d_tile<T, K, true, true, true, IMG, 2, true>(a, tx, ty, 0, step);
+            step2d_tile<T, K, true, true, true, IMG, 2, true, 2>(a, tx, ty, 0, step);
         else
-            step2d_tile<T, K, true, true, true, IMG, 2, false>(a, tx, ty, 0, step);
+            step2d_tile<T, K, true, true, true, IMG, 2, false, 2>(a, tx, ty, 0, step);
     } else {
         switch (cfg) {
-            case 0: step2d_tile<T, K, true, true, true, IMG, 4, true>(a, tx, ty, 0, step); break;
-            case 1: step2d_tile<T, K, true, true, true, IMG, 2, true>(a, tx, ty, 0, step); break;
-            case 2: step2d_tile<T, K, true, true, true, IMG, 4, false>(a, tx, ty, 0, step); break;
-            default: step2d_tile<T, K, true, true, true, IMG, 2, false>(a, tx, ty, 0, step); break;
+            case 0: step2d_tile<T, K, true, true, true, IMG, 4, true, 4>(a, tx, ty, 0, step); break;
+            case 1: step2d_tile<T, K, true, true, true, IMG, 2, true, 2>(a, tx, ty, 0, step); break;
+            case 2: step2d_tile<T, K, true, true, true, IMG, 4, false, 4>(a, tx, ty, 0, step); break;
+            default: step2d_tile<T, K, true, true, true, IMG, 2, false, 2>(a, tx, ty, 0, step); break;
         }
     }
 }
@@ -171,7 +175,8 @@ __device__ __forceinline__ void batch_item(const BShot<T>& sh, const BItem& it, 
 {
     T* const F = sh.fields;
     const int64_t fs = sh.fstride;
-    if (it.kind != K_GATHER && local < sh.stiles) {
+    const uint32_t stiles = it.kind == K_ADJ ? sh.stiles_adj : (it.kind == K_STEP ? sh.stiles : 0u);
+    if (local < stiles) {
         StepArgs<T> a = sh.a;
         a.up = F + it.up * fs;
         a.uc = F + it.uc * fs;
@@ -192,7 +197,7 @@ __device__ __forceinline__ void batch_item(const BShot<T>& sh, const BItem& it, 
         return;
     }
     // gather tile: 256 points of the item's receivers (+ adjoint source) or sources (mu)
-    const uint32_t gt = it.kind == K_GATHER ? local : local - sh.stiles;
+    const uint32_t gt = local - stiles;
     const bool rec = (it.flags & F_GREC) != 0;
     const PtsDev<T>& p = rec ? sh.rec : sh.msrc;
     const int64_t q = (int64_t)gt * 256 + (threadIdx.x & 255);
@@ -335,21 +340,25 @@ int bfail(int code, const char* fmt, ...)
 
 struct HostOp {
     int32_t kind, flags, up, uc, un, uf, gp, gf, step, grow;
+    int32_t phase;  // phase of the shot's schedule the op runs in (ops of one phase run concurrently)
 };
 
-size_t plan_fields(int64_t N, int64_t C)
+// Fields of a shot's store: checkpoints (as in the gradient plan), two sets of segment slots
+// (the backward recomputes segment k-1 into one set while the adjoint of segment k reads the
+// other), two work fields, two adjoint fields and the imaging partial.
+size_t batch_fields(int64_t N, int64_t C)
 {
     const int64_t nck = (N + C - 1) / C;
     const int64_t ck_fields = C == 1 ? nck + 1 : 2 * nck;
-    return (size_t)(ck_fields + (C - 1) + 5);
+    return (size_t)(ck_fields + (nck >= 2 ? 2 : 1) * (C - 1) + 5);
 }
 
 int64_t optimal_seg(int64_t N)
 {
     int64_t best = 1;
-    size_t best_f = plan_fields(N, 1);
+    size_t best_f = batch_fields(N, 1);
     for (int64_t C = 2; C <= N; C++) {
-        const size_t f = plan_fields(N, C);
+        const size_t f = batch_fields(N, C);
         if (f < best_f) {
             best_f = f;
             best = C;
@@ -358,14 +367,17 @@ int64_t optimal_seg(int64_t N)
     return best;
 }
 
-// The ops of one shot: the schedule of wfd_plan_forward / wfd_plan_backward over field indices,
-// field 0 (checkpoint 0's newer level, the rest state, never written) doubling as the zero field
-// the adjoint starts from.  Returns the number of fields.
+// The ops of one shot over field indices: the forward run of wfd_plan_forward (steps writing
+// straight into checkpoint storage), then the backward of wfd_plan_backward with the recompute of
+// segment k-1 running in the same phases as the adjoint of segment k (they touch disjoint fields:
+// segments alternate between the two slot sets).  Field 0 (checkpoint 0's newer level, the rest
+// state, never written) doubles as the zero field the adjoint starts from.  Every value, and the
+// order of every floating-point operation, is the plan's.  Returns the number of fields.
 int shot_ops(int64_t N, int64_t C, bool has_src, bool has_rec, std::vector<HostOp>& ops)
 {
     const int64_t nck = (N + C - 1) / C;
     int nf = 0;
-    std::vector<int> ck_cur(nck), ck_prev(nck), slot(C > 1 ? C - 1 : 0);
+    std::vector<int> ck_cur(nck), ck_prev(nck), slot_a(C > 1 ? C - 1 : 0), slot_b(C > 1 && nck >= 2 ? C - 1 : 0);
     for (int64_t k = 0; k < nck; k++) ck_cur[k] = nf++;
     if (C == 1) {
         ck_prev[0] = nf++;
@@ -373,12 +385,18 @@ int shot_ops(int64_t N, int64_t C, bool has_src, bool has_rec, std::vector<HostO
     } else {
         for (int64_t k = 0; k < nck; k++) ck_prev[k] = nf++;
     }
-    for (auto& s : slot) s = nf++;
+    for (auto& f : slot_a) f = nf++;
+    for (auto& f : slot_b) f = nf++;
     const int w0 = nf++, w1 = nf++, mu0 = nf++, mu1 = nf++, gpart = nf++;
     const int Z = ck_cur[0];
     ops.clear();
-    auto op = [&](int kind, int flags, int up, int uc, int un, int uf, int gp, int gf, int64_t step, int64_t grow) {
-        ops.push_back({kind, flags, up, uc, un, uf, gp, gf, (int32_t)step, (int32_t)grow});
+    int32_t ph = 0;
+    auto mk = [&](int kind, int flags, int up, int uc, int un, int uf, int gp, int gf, int64_t step, int64_t grow) {
+        return HostOp{kind, flags, up, uc, un, uf, gp, gf, (int32_t)step, (int32_t)grow, 0};
+    };
+    auto emit = [&](HostOp o) {
+        o.phase = ph++;
+        ops.push_back(o);
     };
     // forward
     int P = ck_prev[0], Cu = ck_cur[0];
@@ -394,21 +412,29 @@ int shot_ops(int64_t N, int64_t C, bool has_src, bool has_rec, std::vector<HostO
         else
             D = Cu == w0 ? w1 : w0;
         const int fl = (has_src ? (int)F_INJECT : 0) | (has_rec && n >= 1 ? (int)F_GREC : 0);
-        op(K_STEP, fl, P, Cu, D, -1, -1, Cu, n, n - 1);
+        emit(mk(K_STEP, fl, P, Cu, D, -1, -1, Cu, n, n - 1));
         P = Cu;
         Cu = D;
     }
-    if (has_rec) op(K_GATHER, F_GREC, -1, -1, -1, -1, -1, Cu, -1, N - 1);
+    const int last = Cu;  // u^N: receiver row N-1 is gathered from it
     // backward
-    auto field_at = [&](int64_t n, int64_t s0, int64_t k) { return n == s0 ? ck_cur[k] : slot[n - s0 - 1]; };
-    int muP = Z, muC = Z;
-    bool first = true;
-    for (int64_t k = nck - 1; k >= 0; k--) {
+    auto slots = [&](int64_t k) -> const std::vector<int>& { return ((nck - 1 - k) & 1) ? slot_b : slot_a; };
+    auto field_at = [&](int64_t n, int64_t s0, int64_t k) { return n == s0 ? ck_cur[k] : slots(k)[n - s0 - 1]; };
+    auto recompute = [&](int64_t k) {
+        std::vector<HostOp> r;
         const int64_t s0 = k * C, s1 = std::min(s0 + C, N);
         for (int64_t n = s0; n + 1 < s1; n++) {
             const int prev = n == s0 ? ck_prev[k] : field_at(n - 1, s0, k);
-            op(K_STEP, has_src ? F_INJECT : 0, prev, field_at(n, s0, k), field_at(n + 1, s0, k), -1, -1, -1, n, -1);
+            r.push_back(mk(K_STEP, has_src ? F_INJECT : 0, prev, field_at(n, s0, k), field_at(n + 1, s0, k), -1, -1, -1,
+                           n, -1));
         }
+        return r;
+    };
+    int muP = Z, muC = Z;
+    bool first = true;
+    auto adjoint = [&](int64_t k) {
+        std::vector<HostOp> a;
+        const int64_t s0 = k * C, s1 = std::min(s0 + C, N);
         for (int64_t m = s1; m > s0; m--) {
             const int dest = muP != Z ? muP : (muC == mu0 ? mu1 : mu0);
             int fl = (has_rec ? (int)F_INJECT : 0) | (has_src && m <= N - 1 ? (int)F_GSRC : 0);
@@ -416,21 +442,41 @@ int shot_ops(int64_t N, int64_t C, bool has_src, bool has_rec, std::vector<HostO
                 fl |= F_FLUSH | (first ? (int)F_FIRST : 0);
                 first = false;
             }
-            op(K_ADJ, fl, muP, muC, dest, field_at(m - 1, s0, k), gpart, muC, m - 1, m);
+            a.push_back(mk(K_ADJ, fl, muP, muC, dest, field_at(m - 1, s0, k), gpart, muC, m - 1, m));
             muP = muC;
             muC = dest;
         }
-    }
-    if (has_src) op(K_GATHER, F_GSRC, -1, -1, -1, -1, -1, muC, -1, 0);
-    // a lone receiver gather followed by a recompute step shares its phase
-    for (size_t i = 0; i + 1 < ops.size(); i++)
-        if (ops[i].kind == K_GATHER && (ops[i].flags & F_GREC) && ops[i + 1].kind == K_STEP && !(ops[i + 1].flags & (F_GREC | F_GSRC))) {
-            ops[i + 1].flags |= F_GREC;
-            ops[i + 1].gf = ops[i].gf;
-            ops[i + 1].grow = ops[i].grow;
-            ops.erase(ops.begin() + (long)i);
-            break;
+        return a;
+    };
+    // the last segment's recompute runs alone; the final receiver gather shares its first phase
+    std::vector<HostOp> r = recompute(nck - 1);
+    bool gathered = !has_rec;
+    for (HostOp& o : r) {
+        if (!gathered) {
+            o.flags |= F_GREC;
+            o.gf = last;
+            o.grow = (int32_t)(N - 1);
+            gathered = true;
         }
+        emit(o);
+    }
+    if (!gathered) emit(mk(K_GATHER, F_GREC, -1, -1, -1, -1, -1, last, -1, N - 1));
+    for (int64_t k = nck - 1; k >= 0; k--) {
+        const std::vector<HostOp> a = adjoint(k);
+        const std::vector<HostOp> rr = k >= 1 ? recompute(k - 1) : std::vector<HostOp>();
+        for (size_t i = 0; i < std::max(a.size(), rr.size()); i++) {
+            if (i < a.size()) {
+                ops.push_back(a[i]);
+                ops.back().phase = ph;
+            }
+            if (i < rr.size()) {
+                ops.push_back(rr[i]);
+                ops.back().phase = ph;
+            }
+            ph++;
+        }
+    }
+    if (has_src) emit(mk(K_GATHER, F_GSRC, -1, -1, -1, -1, -1, muC, -1, 0));
     return nf;
 }
 
@@ -504,10 +550,10 @@ void fill_pts(PtsDev<T>& p, const wfd_points& w, void* data, int64_t width, int6
     p.nrows = w.nrows;
 }
 
-void stiles_of(const wfd_grid& g, int cfg, uint32_t& gx, uint32_t& gy)
+void stiles_of(const wfd_grid& g, int cfg, bool adj, uint32_t& gx, uint32_t& gy)
 {
     const int64_t cols = cfg_variant(cfg) == 1 ? 248 : 256;
-    const int64_t ri = g.dtype == WFD_F64 ? 2 : cfg_rows(cfg);  // the float64 tiles march 2 rows
+    const int64_t ri = (g.dtype == WFD_F64 || adj) ? 2 : cfg_rows(cfg);  // as batch_tile_k
     gx = (uint32_t)((g.n[1] + cols - 1) / cols);
     gy = (uint32_t)((g.n[0] - 2 + 4 * ri - 1) / (4 * ri));
 }
@@ -559,8 +605,11 @@ int upload_shots(wfd_batch* b)
         h.G = in.G + off;
         h.K = K;
         h.cfg = b->cfg[s];
-        stiles_of(g, h.cfg, h.gx, h.gy);
+        stiles_of(g, h.cfg, false, h.gx, h.gy);
         h.stiles = h.gx * h.gy;
+        uint32_t gxa = 0;
+        stiles_of(g, h.cfg, true, gxa, h.gy_adj);
+        h.stiles_adj = gxa * h.gy_adj;
     }
     if (!b->shots_dev) {
         if (hipMalloc(&b->shots_dev, sizeof(BShot<T>) * b->nshots) != hipSuccess)
@@ -578,7 +627,7 @@ uint32_t op_tiles(const wfd_batch* b, int s, const HostOp& o)
     uint32_t t = 0;
     if (o.kind != K_GATHER) {
         uint32_t gx, gy;
-        stiles_of(b->in[s].grid, b->cfg[s], gx, gy);
+        stiles_of(b->in[s].grid, b->cfg[s], o.kind == K_ADJ, gx, gy);
         t = gx * gy;
     }
     if (o.flags & F_GREC) t += (uint32_t)((b->in[s].rec.count + 255) / 256);
@@ -599,9 +648,10 @@ int make_schedule(wfd_batch* b, const std::vector<std::vector<HostOp>>& ops, Sch
     std::vector<int> order;
     for (int s = 0; s < S; s++)
         if (!ops[s].empty()) order.push_back(s);
+    auto nph = [&](int s) { return ops[s].empty() ? (int64_t)0 : (int64_t)ops[s].back().phase + 1; };
     auto work = [&](int s) { return (double)shot_cells(b, s) * (double)ops[s].size(); };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        const size_t a = ops[x].size(), c = ops[y].size();
+        const int64_t a = nph(x), c = nph(y);
         return a != c ? a > c : shot_cells(b, x) > shot_cells(b, y);
     });
     out.group.assign(S, 0);
@@ -624,7 +674,7 @@ int make_schedule(wfd_batch* b, const std::vector<std::vector<HostOp>>& ops, Sch
         int64_t t = 0, cur = 0;
         for (int s : order) {
             if (out.group[s] != g) continue;
-            while (!act.empty() && (cur + shot_cells(b, s) > target || (int)act.size() >= BATCH_MAX_ITEMS)) {
+            while (!act.empty() && (cur + shot_cells(b, s) > target || 2 * (int)act.size() >= BATCH_MAX_ITEMS)) {
                 t = std::max(t, act.top().first);
                 while (!act.empty() && act.top().first <= t) {
                     cur -= act.top().second;
@@ -632,9 +682,9 @@ int make_schedule(wfd_batch* b, const std::vector<std::vector<HostOp>>& ops, Sch
                 }
             }
             local[s] = t;
-            act.push({t + (int64_t)ops[s].size(), shot_cells(b, s)});
+            act.push({t + nph(s), shot_cells(b, s)});
             cur += shot_cells(b, s);
-            gphases[g] = std::max<int64_t>(gphases[g], t + (int64_t)ops[s].size());
+            gphases[g] = std::max<int64_t>(gphases[g], t + nph(s));
         }
     }
     std::memset(&out.gp, 0, sizeof out.gp);
@@ -644,19 +694,18 @@ int make_schedule(wfd_batch* b, const std::vector<std::vector<HostOp>>& ops, Sch
     for (int s = 0; s < S; s++) out.start[s] = out.gp.b[out.group[s]] + local[s];
     std::vector<uint32_t> count(P + 1, 0);
     for (int s = 0; s < S; s++)
-        for (size_t l = 0; l < ops[s].size(); l++) count[out.start[s] + l]++;
+        for (const HostOp& o : ops[s]) count[out.start[s] + o.phase]++;
     std::vector<uint32_t> off(P + 1, 0);
     for (int64_t p = 0; p < P; p++) {
-        if (count[p] > (uint32_t)BATCH_MAX_ITEMS) return bfail(WFD_EINVAL, "wfd_batch: more than %d shots in one phase", BATCH_MAX_ITEMS);
+        if (count[p] > (uint32_t)BATCH_MAX_ITEMS) return bfail(WFD_EINVAL, "wfd_batch: more than %d items in one phase", BATCH_MAX_ITEMS);
         off[p + 1] = off[p] + count[p];
     }
     std::vector<BItem> items(off[P]);
     std::vector<uint32_t> fill(off.begin(), off.end() - 1);
     out.tiles.assign(P, 0);
     for (int s = 0; s < S; s++) {  // shot order inside a phase
-        for (size_t l = 0; l < ops[s].size(); l++) {
-            const int64_t p = out.start[s] + (int64_t)l;
-            const HostOp& o = ops[s][l];
+        for (const HostOp& o : ops[s]) {
+            const int64_t p = out.start[s] + o.phase;
             BItem& it = items[fill[p]++];
             it.ntiles = op_tiles(b, s, o);
             it.tile0 = out.tiles[p];
@@ -882,7 +931,7 @@ int wfd_batch_shot_info(const wfd_batch* b, int32_t shot, int32_t* size_cls, int
     if (rows) *rows = b->dtype == WFD_F64 ? 2 : cfg_rows(b->cfg[shot]);
     if (seg) *seg = b->seg[shot];
     if (start_phase) *start_phase = b->sched.start[shot];
-    if (nphases) *nphases = (int64_t)b->ops[shot].size();
+    if (nphases) *nphases = b->ops[shot].empty() ? 0 : (int64_t)b->ops[shot].back().phase + 1;
     return WFD_OK;
 }
 
@@ -962,7 +1011,7 @@ int wfd_batch_tune(wfd_batch* b, int32_t steps, wfd_stream_t stream)
             const int w0 = (int)b->nfields[sh] - 5, w1 = w0 + 1;
             for (int i = 0; i < steps; i++) {
                 const int A = i % 2 ? w1 : w0, B = i % 2 ? w0 : w1;
-                probe[sh].push_back({K_STEP, 0, A, B, A, -1, -1, -1, i, -1});
+                probe[sh].push_back({K_STEP, 0, A, B, A, -1, -1, -1, i, -1, i});
             }
         }
         if (!any) continue;
@@ -1025,7 +1074,7 @@ int wfd_batch_shot_ops(int64_t nsteps, int64_t seg, int32_t has_src, int32_t has
     if (nfields) *nfields = nf;
     if (ops) {
         if (cap < (int64_t)v.size()) return bfail(WFD_EINVAL, "wfd_batch_shot_ops: output holds %lld ops, %zu needed", (long long)cap, v.size());
-        for (size_t i = 0; i < v.size(); i++) std::memcpy(ops + 10 * i, &v[i], sizeof(HostOp));
+        for (size_t i = 0; i < v.size(); i++) std::memcpy(ops + 11 * i, &v[i], sizeof(HostOp));
     }
     return WFD_OK;
 }

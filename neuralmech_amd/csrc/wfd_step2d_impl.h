@@ -110,8 +110,9 @@ __device__ __forceinline__ T lap_elem(const T wc, const T* wr, const T* wcol, co
 }
 
 // One stencil tile of both variants: column block tx, row block ty of shot `shot`.  XL selects
-// the lane-exchange halo; VROW = false is the 1D case (a single row, no vertical stencil).
-template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL>
+// the lane-exchange halo; VROW = false is the 1D case (a single row, no vertical stencil).  PF is
+// the number of rows whose per-row operands are loaded together (see the row loop).
+template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, int PF = 1>
 __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, uint32_t ty, int shot, int64_t step)
 {
     constexpr int KR = VROW ? K : 0;
@@ -164,84 +165,99 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
         if constexpr (SPONGE) bc[e] = (c + e >= 0 && c + e < a.nc) ? a.beta_c[c + e] : T(0);
     }
 
+    // Rows are updated in groups of PD.  The per-row operands of a group (u_prev, kc and the
+    // imaging partial) are all loaded before the group stores anything, so a group waits for one
+    // memory latency instead of one per row, at the cost of registers (worth it where occupancy
+    // cannot hide the latency).  u_next may alias u_prev only element for element, and a group's
+    // loads precede its own stores: the values read are the same as row by row.
+    constexpr int PD = RI < PF ? RI : PF;
 #pragma unroll
-    for (int q = 0; q < RI; q++) {
-        const int64_t r = r0 + q;
-        if (r >= a.row_hi) break;  // whole wave
-        const int64_t o = r * a.ld + c;
-        C4<T> L, R, Lf, Rf;
-        if constexpr (XL) {
-            L = from_lane_left(win[q + KR]);
-            R = from_lane_right(win[q + KR]);
-            if constexpr (IMG) {
-                Lf = from_lane_left(wf[q + KR]);
-                Rf = from_lane_right(wf[q + KR]);
-            }
-        } else {
-            L = load_ok ? ld4(uc + o - 4) : zero4<T>();
-            R = load_ok ? ld4(uc + o + 4) : zero4<T>();
-            if constexpr (IMG) {
-                const T* __restrict__ uf = a.uf + sb;
-                Lf = load_ok ? ld4(uf + o - 4) : zero4<T>();
-                Rf = load_ok ? ld4(uf + o + 4) : zero4<T>();
-            }
+    for (int q0 = 0; q0 < RI; q0 += PD) {
+        if (r0 + q0 >= a.row_hi) break;  // whole wave
+        C4<T> Pv[PD], Kv[VARC ? PD : 1], Gv[IMG ? PD : 1];
+#pragma unroll
+        for (int i = 0; i < PD; i++) {
+            const int64_t r = r0 + q0 + i;
+            const bool ld = own && r < a.row_hi;
+            const int64_t o = r * a.ld + c;
+            Pv[i] = ld ? ld4(up + o) : zero4<T>();
+            if constexpr (VARC) Kv[i] = ld ? ld4(coef + o) : zero4<T>();
+            if constexpr (IMG) Gv[i] = ld ? ld4(a.g + sb + o) : zero4<T>();
         }
-        const C4<T> P = own ? ld4(up + o) : zero4<T>();
-        T kc[4];
-        if constexpr (VARC) {
-            const C4<T> kv = own ? ld4(coef + o) : zero4<T>();
 #pragma unroll
-            for (int e = 0; e < 4; e++) kc[e] = kv.e[e];
-        } else {
+        for (int i = 0; i < PD; i++) {
+            const int q = q0 + i;
+            const int64_t r = r0 + q;
+            if (r >= a.row_hi) break;  // whole wave
+            const int64_t o = r * a.ld + c;
+            C4<T> L, R, Lf, Rf;
+            if constexpr (XL) {
+                L = from_lane_left(win[q + KR]);
+                R = from_lane_right(win[q + KR]);
+                if constexpr (IMG) {
+                    Lf = from_lane_left(wf[q + KR]);
+                    Rf = from_lane_right(wf[q + KR]);
+                }
+            } else {
+                L = load_ok ? ld4(uc + o - 4) : zero4<T>();
+                R = load_ok ? ld4(uc + o + 4) : zero4<T>();
+                if constexpr (IMG) {
+                    const T* __restrict__ uf = a.uf + sb;
+                    Lf = load_ok ? ld4(uf + o - 4) : zero4<T>();
+                    Rf = load_ok ? ld4(uf + o + 4) : zero4<T>();
+                }
+            }
+            const C4<T>& P = Pv[i];
+            T kc[4];
 #pragma unroll
-            for (int e = 0; e < 4; e++) kc[e] = a.kconst;
-        }
-        T b[4] = {T(0), T(0), T(0), T(0)};
-        bool damp = false;
-        if constexpr (SPONGE) {
-            const T br = VROW ? a.beta_r[r] : T(0);
+            for (int e = 0; e < 4; e++) kc[e] = VARC ? Kv[VARC ? i : 0].e[e] : a.kconst;
+            T b[4] = {T(0), T(0), T(0), T(0)};
+            bool damp = false;
+            if constexpr (SPONGE) {
+                const T br = VROW ? a.beta_r[r] : T(0);
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    b[e] = VROW ? br + bc[e] : bc[e];
+                    damp |= b[e] != T(0);
+                }
+            }
+            T val[4];
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                b[e] = VROW ? br + bc[e] : bc[e];
-                damp |= b[e] != T(0);
+                const T ctr = win[q + KR].e[e];
+                const T lap = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, win, q, L, R, e);
+                const T t = damp ? T(2) * ctr - (T(1) - b[e]) * P.e[e] : T(2) * ctr - P.e[e];
+                val[e] = fmaT(kc[e], lap, t);
             }
-        }
-        T val[4];
+            const int64_t prow = VROW ? r : 0;  // CSR row of the point lists
+            if (src_here) wave_inject(a.src, shot, prow, base, lane_lo, lane_hi, step, lane, kc, val);
+            if (damp) {
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const T ctr = win[q + KR].e[e];
-            const T lap = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, win, q, L, R, e);
-            const T t = damp ? T(2) * ctr - (T(1) - b[e]) * P.e[e] : T(2) * ctr - P.e[e];
-            val[e] = fmaT(kc[e], lap, t);
-        }
-        const int64_t prow = VROW ? r : 0;  // CSR row of the point lists
-        if (src_here) wave_inject(a.src, shot, prow, base, lane_lo, lane_hi, step, lane, kc, val);
-        if (damp) {
+                for (int e = 0; e < 4; e++) val[e] = val[e] / (T(1) + b[e]);
+            }
 #pragma unroll
-            for (int e = 0; e < 4; e++) val[e] = val[e] / (T(1) + b[e]);
-        }
+            for (int e = 0; e < 4; e++)
+                if (!valid[e]) val[e] = T(0);
+            if (own) st4(un + o, val);
+            if constexpr (IMG) {
+                T gv[4];
 #pragma unroll
-        for (int e = 0; e < 4; e++)
-            if (!valid[e]) val[e] = T(0);
-        if (own) st4(un + o, val);
-        if constexpr (IMG) {
-            T gv[4];
+                for (int e = 0; e < 4; e++) gv[e] = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, wf, q, Lf, Rf, e);
+                if (own) {
+                    T* gp = a.g + sb;
+                    const C4<T>& G = Gv[IMG ? i : 0];
 #pragma unroll
-            for (int e = 0; e < 4; e++) gv[e] = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, wf, q, Lf, Rf, e);
-            if (own) {
-                T* gp = a.g + sb;
-                const C4<T> G = ld4(gp + o);
+                    for (int e = 0; e < 4; e++) gv[e] = valid[e] ? fmaT(val[e], gv[e], G.e[e]) : T(0);
+                    if (a.gsum) {  // fused flush: gsum += partial, partial = 0 (same bits as the flush kernel)
+                        double* gs = a.gsum + sb + o;
 #pragma unroll
-                for (int e = 0; e < 4; e++) gv[e] = valid[e] ? fmaT(val[e], gv[e], G.e[e]) : T(0);
-                if (a.gsum) {  // fused flush: gsum += partial, partial = 0 (same bits as the flush kernel)
-                    double* gs = a.gsum + sb + o;
-#pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        if (valid[e]) gs[e] = (a.gsum_first ? 0.0 : gs[e]) + (double)gv[e];
-                        gv[e] = T(0);
+                        for (int e = 0; e < 4; e++) {
+                            if (valid[e]) gs[e] = (a.gsum_first ? 0.0 : gs[e]) + (double)gv[e];
+                            gv[e] = T(0);
+                        }
                     }
+                    st4(gp + o, gv);
                 }
-                st4(gp + o, gv);
             }
         }
     }

@@ -25,46 +25,59 @@ def _holds(tag, kind, k, N):
 
 def replay(N, seg, has_src, has_rec):
     ops, nf = shot_ops(N, seg, has_src, has_rec)
+    assert np.all(np.diff(ops[:, 10]) >= 0) and ops[0, 10] == 0  # ops in phase order
     tag = [ZERO] * nf
     rec_rows, src_rows, flushes, firsts = [], [], [], []
     adj_seen = []
     fwd_done = False
     gpart = None
-    for kind, flags, up, uc, un, uf, gp, gf, step, row in ops.tolist():
-        # gather and stencil of one op run concurrently: the gathered field is not written
-        assert not (flags & (F_GREC | F_GSRC)) or kind == K_GATHER or gf != un
-        if flags & F_GREC:
-            assert _holds(tag[gf], "u", row + 1, N), (row, tag[gf])
-            rec_rows.append(row)
-        if flags & F_GSRC:
-            assert _holds(tag[gf], "mu", row + 1, N), (row, tag[gf])
-            src_rows.append(row)
-        if kind == K_STEP:
-            n = step
-            assert un != uc and 0 <= un < nf
-            assert _holds(tag[up], "u", n - 1, N) and _holds(tag[uc], "u", n, N), (n, tag[up], tag[uc])
-            assert bool(flags & F_INJECT) == has_src
-            assert not adj_seen or n < max(adj_seen)  # recompute only below the adjoint front
-            tag[un] = ("u", n + 1)
-            if n == N - 1:
-                fwd_done = True
-        elif kind == K_ADJ:
-            m = step + 1
-            assert fwd_done and un != uc
-            assert _holds(tag[up], "mu", m + 2, N) and _holds(tag[uc], "mu", m + 1, N), (m, tag[up], tag[uc])
-            assert _holds(tag[uf], "u", m - 1, N), (m, tag[uf])
-            assert bool(flags & F_INJECT) == has_rec
-            assert not adj_seen or m == adj_seen[-1] - 1
-            assert gpart is None or gp == gpart
-            gpart = gp
-            adj_seen.append(m)
-            tag[un] = ("mu", m)
-            if flags & F_FLUSH:
-                flushes.append(m - 1)
-            if flags & F_FIRST:
-                firsts.append(m - 1)
-        else:
-            assert kind == K_GATHER and flags & (F_GREC | F_GSRC)
+    for ph in range(int(ops[-1, 10]) + 1):
+        group = ops[ops[:, 10] == ph].tolist()
+        assert 1 <= len(group) <= 2
+        # ops of one phase run concurrently: no op writes what another op of the phase touches
+        writes = [o[4] for o in group if o[0] != K_GATHER]
+        touched = [[f for f in (o[2], o[3], o[4], o[5], o[7]) if f >= 0] for o in group]
+        for i, w in enumerate(writes):
+            for j, t in enumerate(touched):
+                if j != i:
+                    assert w not in t, (ph, group)
+        pending = []
+        for kind, flags, up, uc, un, uf, gp, gf, step, row, _ in group:
+            # gather and stencil of one op run concurrently: the gathered field is not written
+            assert not (flags & (F_GREC | F_GSRC)) or kind == K_GATHER or gf != un
+            if flags & F_GREC:
+                assert _holds(tag[gf], "u", row + 1, N), (row, tag[gf])
+                rec_rows.append(row)
+            if flags & F_GSRC:
+                assert _holds(tag[gf], "mu", row + 1, N), (row, tag[gf])
+                src_rows.append(row)
+            if kind == K_STEP:
+                n = step
+                assert un != uc and 0 <= un < nf
+                assert _holds(tag[up], "u", n - 1, N) and _holds(tag[uc], "u", n, N), (n, tag[up], tag[uc])
+                assert bool(flags & F_INJECT) == has_src
+                pending.append((un, ("u", n + 1)))
+                if n == N - 1:
+                    fwd_done = True
+            elif kind == K_ADJ:
+                m = step + 1
+                assert fwd_done and un != uc
+                assert _holds(tag[up], "mu", m + 2, N) and _holds(tag[uc], "mu", m + 1, N), (m, tag[up], tag[uc])
+                assert _holds(tag[uf], "u", m - 1, N), (m, tag[uf])
+                assert bool(flags & F_INJECT) == has_rec
+                assert not adj_seen or m == adj_seen[-1] - 1
+                assert gpart is None or gp == gpart
+                gpart = gp
+                adj_seen.append(m)
+                pending.append((un, ("mu", m)))
+                if flags & F_FLUSH:
+                    flushes.append(m - 1)
+                if flags & F_FIRST:
+                    firsts.append(m - 1)
+            else:
+                assert kind == K_GATHER and flags & (F_GREC | F_GSRC)
+        for f, t in pending:
+            tag[f] = t
     # every adjoint step once, newest first; flushes where the gradient plan flushes
     assert adj_seen == list(range(N, 0, -1))
     assert flushes == [m - 1 for m in range(N, 0, -1) if (m - 1) % 32 == 0]
@@ -72,7 +85,7 @@ def replay(N, seg, has_src, has_rec):
     assert sorted(rec_rows) == (list(range(N)) if has_rec else [])
     assert sorted(src_rows) == (list(range(N)) if has_src else [])
     # the imaging partial never doubles as a wave field
-    for kind, flags, up, uc, un, uf, gp, gf, step, row in ops.tolist():
+    for kind, flags, up, uc, un, uf, gp, gf, step, row, _ in ops.tolist():
         assert gpart is None or gpart not in (up, uc, un, uf, gf)
     return ops, nf
 
@@ -91,12 +104,21 @@ def test_schedule_without_points(has_src, has_rec):
 @pytest.mark.parametrize("N,seg", [(10, 1), (10, 3), (3000, 0), (5000, 0)])
 def test_schedule_length_and_fields(N, seg):
     ops, nf = shot_ops(N, seg)
-    C = seg or int(np.argmin([(2 * -(-N // c) if c > 1 else -(-N // c) + 1) + c - 1 for c in range(1, N + 1)]) + 1)
+
+    def fields(C):
+        nck = -(-N // C)
+        return (nck + 1 if C == 1 else 2 * nck) + (2 if nck >= 2 else 1) * (C - 1) + 5
+
+    C = seg or 1 + int(np.argmin([fields(c) for c in range(1, N + 1)]))
     nck = -(-N // C)
-    # forward + lone gather (unless merged into a recompute) + recompute + adjoint + final gather
-    merged = (N - (nck - 1) * C) > 1
-    assert len(ops) == N + (0 if merged else 1) + (N - nck) + N + 1
-    assert nf == (nck + 1 if C == 1 else 2 * nck) + (C - 1) + 5
+    last = N - (nck - 1) * C  # steps of the last segment
+    # ops: forward + lone gather (unless merged into a recompute) + recompute + adjoint + final gather
+    assert len(ops) == N + (0 if last > 1 else 1) + (N - nck) + N + 1
+    assert nf == fields(C)
+    # phases: forward, the last segment's recompute alone, then each adjoint segment together with
+    # the previous segment's recompute, then the final gather
+    phases = N + max(last - 1, 1) + sum(max(min(C, N - k * C), C - 1 if k else 0) for k in range(nck)) + 1
+    assert ops[-1, 10] + 1 == phases
 
 
 def test_shot_ops_rejects_bad_arguments():
@@ -104,7 +126,7 @@ def test_shot_ops_rejects_bad_arguments():
     cnt, nf = ctypes.c_int64(), ctypes.c_int32()
     assert lib.wfd_batch_shot_ops(0, 0, 1, 1, None, 0, ctypes.byref(cnt), ctypes.byref(nf)) == _lib.WFD_EINVAL
     assert lib.wfd_batch_shot_ops(10, 11, 1, 1, None, 0, ctypes.byref(cnt), ctypes.byref(nf)) == _lib.WFD_EINVAL
-    out = np.zeros((2, 10), np.int32)
+    out = np.zeros((2, 11), np.int32)
     rc = lib.wfd_batch_shot_ops(10, 0, 1, 1, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), 2,
                                 ctypes.byref(cnt), ctypes.byref(nf))
     assert rc == _lib.WFD_EINVAL and "needed" in _lib.last_error()
