@@ -175,6 +175,11 @@ def uneven_main(args):
         diff = max(float((a[1] - s_).norm() / s_.norm()) for a, s_ in zip(grads, seq))
     nshots = len(shots)
     gbs = 64.0 * cs / t_run / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "uneven_batch_traffic.json")
+    if os.path.exists(tpath) and nshots == 64 and args.uneven_scale == 1.0:
+        with open(tpath) as f:
+            traffic = json.load(f)["bytes_per_run"]
     out = {
         "metric": "FWI gradient s/shot (uneven batch, one persistent launch)",
         "value": round(t_batch / nshots, 5),
@@ -193,8 +198,9 @@ def uneven_main(args):
                    "shots": nshots, "cell_steps": cs, "phases": info["phases"], "blocks": info["blocks"],
                    "store_gb": round(info["bytes"] / 1e9, 2), "tiles_per_class": classes},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "note": "64 algorithmic B per cell-step (forward 16 + recompute 20 + adjoint 28)"},
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "note": "64 algorithmic B per cell-step (forward 16 + recompute 20 + adjoint 28); "
+                             "traffic = FETCH/WRITE_SIZE bytes of one persistent launch"},
         "gcell_updates_per_s": round(cs / t_run / 1e9, 2),
         "batch_run_s": round(t_run, 4),
         "finalize_s": round(t_fin, 4),
