@@ -297,8 +297,8 @@ def volume_main(args):
         tg = time.perf_counter() - t0
         out["gradient"] = {"seconds": round(tg, 3), "checkpoint_spacing": plan.seg, "checkpoints": plan.nckpt,
                            "plan_GB": round(plan.bytes / 1e9, 2), "finite": bool(torch.isfinite(gk).all()),
-                           "bytes_per_cell_step": 60,
-                           "achieved_GB_per_s": round(60 * cells * nsteps / tg / 1e9, 1)}
+                           "bytes_per_cell_step": 64,
+                           "achieved_GB_per_s": round(64 * cells * nsteps / tg / 1e9, 1)}
     print(json.dumps(out))
 
 
