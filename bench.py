@@ -312,6 +312,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="use the default launch shape")
     ap.add_argument("--workload", choices=["forward", "fwi", "copy", "3d", "uneven"], default="forward")
     ap.add_argument("--launch", default="", help="force VARIANT,ROWS instead of tuning")
+    ap.add_argument("--two-step", type=int, default=0, help="forward: two steps per launch, ROWS output rows per wave")
     ap.add_argument("--n3d", type=int, default=512)
     ap.add_argument("--steps3d", type=int, default=2000)
     ap.add_argument("--grad", action="store_true", help="3D: also time the checkpointed gradient")
@@ -355,13 +356,14 @@ def main():
         tuned = g.tune(med, steps=40)
     state = (g.zeros(), g.zeros())
     # warm-up steps advance the same shot; the timed steps continue it
-    solver.run(g, med, args.warmup, src=src, sig=w["sig"], rec=rec, state=state, graph_chunk=args.graph)
+    solver.run(g, med, args.warmup, src=src, sig=w["sig"], rec=rec, state=state, graph_chunk=args.graph,
+               two_step=args.two_step)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     t_start, t_end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t_start.record(stream)
     traces, _ = solver.run(g, med, args.steps, src=src, sig=w["sig"], rec=rec, state=state, step0=args.warmup,
-                           graph_chunk=args.graph)
+                           graph_chunk=args.graph, two_step=args.two_step)
     t_end.record(stream)
     torch.cuda.synchronize()
     ms = t_start.elapsed_time(t_end)

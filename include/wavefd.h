@@ -139,6 +139,16 @@ int wfd_forward(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med,
                 const wfd_points* src, const wfd_points* rec, int64_t step0, int64_t nsteps,
                 wfd_stream_t stream);
 
+/* nsteps steps like wfd_forward, two per launch (2D): each launch reads u^{n-1}, u^n and kc once
+ * and writes u^{n+1}, u^{n+2} (20 bytes per node per two steps instead of 32), recomputing a
+ * 2K-row / 8-column halo of step 1; bit-identical to single steps.  u_c, u_d are work fields of
+ * the same size (all four distinct).  On return u_a = u^{N-1}, u_b = u^N.  rows = output rows
+ * per wave (8, 16, 32; 0 = 16).  Replaces temporal_step (step2D_wave.cu:244-304) and its 4-array
+ * rotation (2D_wave_cuda_temporal.py:110-112) for the general update. */
+int wfd_forward2(const wfd_grid* g, void* u_a, void* u_b, void* u_c, void* u_d,
+                 const wfd_medium* med, const wfd_points* src, const wfd_points* rec,
+                 int64_t step0, int64_t nsteps, int32_t rows, wfd_stream_t stream);
+
 /* Same as wfd_forward, replayed from a captured hipGraph of `chunk` steps (chunk even, the
  * reference's 20-step graph of 2D_wave_cuda_graph.py:73-79,94-95).  The step index travels in
  * device memory, so one graph serves every chunk; the remainder runs as plain launches. */

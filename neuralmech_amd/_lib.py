@@ -111,6 +111,7 @@ SIGNATURES = {
     "wfd_step": (ctypes.c_int, [_GP, _P, _P, _P, _MP, _PP, _PP, _I64, _P]),
     "wfd_forward": (ctypes.c_int, [_GP, _P, _P, _MP, _PP, _PP, _I64, _I64, _P]),
     "wfd_forward_graph": (ctypes.c_int, [_GP, _P, _P, _MP, _PP, _PP, _I64, _I64, ctypes.c_int32, _P]),
+    "wfd_forward2": (ctypes.c_int, [_GP, _P, _P, _P, _P, _MP, _PP, _PP, _I64, _I64, ctypes.c_int32, _P]),
     "wfd_adjoint_step": (ctypes.c_int, [_GP, _P, _P, _P, _MP, _PP, _PP, _I64, _P, _P, _P]),
     "wfd_grad_flush": (ctypes.c_int, [_GP, _P, _P, _P]),
     "wfd_check_medium_async": (ctypes.c_int, [_GP, _MP, _P, _P]),

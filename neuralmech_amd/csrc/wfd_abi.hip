@@ -231,6 +231,8 @@ struct Launch {
     bool img;
     LaunchCfg cfg;
     int64_t gather_off;
+    void* un2 = nullptr;  // two-step launch: u^{n+2} (un receives u^{n+1})
+    int t2rows = 16;
 };
 
 template <typename T>
@@ -299,6 +301,13 @@ int launch_t(const Launch& L, hipStream_t s)
     a.step = L.step;
     a.step_dev = L.step_dev;
     hipError_t e;
+    if (L.un2) {  // two steps: the gather blocks record both input levels
+        a.un2 = static_cast<T*>(L.un2) + off;
+        a.ngb *= 2;
+        e = launch_step2d_t2<T>(a, g->order, varc, sponge, L.t2rows, g->nshots, s);
+        if (e != hipSuccess) return hip_fail(e, "two-step kernel launch");
+        return WFD_OK;
+    }
     if (!L.un) {  // gather only: L.uc is the field, row = L.step + L.gather_off
         e = launch_gather<T>(a, s);
         if (e != hipSuccess) return hip_fail(e, "gather kernel launch");
@@ -517,6 +526,67 @@ int wfd_forward(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med, 
         std::swap(p, c);
     }
     if (nsteps > 0) return wfd::gather_launch(g, c, rec, step0 + nsteps - 1, (hipStream_t)stream);
+    return WFD_OK;
+}
+
+int wfd_forward2(const wfd_grid* g, void* u_a, void* u_b, void* u_c, void* u_d, const wfd_medium* med,
+                 const wfd_points* src, const wfd_points* rec, int64_t step0, int64_t nsteps, int32_t rows,
+                 wfd_stream_t stream)
+{
+    int rc = check_step_args(g, u_a, u_b, u_a, med);
+    if (rc) return rc;
+    if (g->ndim != 2) return fail(WFD_EINVAL, "wfd_forward2: 2D grids only");
+    if (nsteps < 0) return fail(WFD_EINVAL, "nsteps must be >= 0");
+    if (rows != 0 && rows != 8 && rows != 16 && rows != 32) return fail(WFD_EINVAL, "wfd_forward2: rows must be 0, 8, 16 or 32");
+    if (!u_c || !u_d) return fail(WFD_EINVAL, "wfd_forward2: NULL work field");
+    const void* f[4] = {u_a, u_b, u_c, u_d};
+    for (int i = 0; i < 4; i++)
+        for (int j = i + 1; j < 4; j++)
+            if (f[i] == f[j]) return fail(WFD_EINVAL, "wfd_forward2: the four fields must be distinct");
+    const uintptr_t align = 4 * esize(g);
+    if (((uintptr_t)u_c | (uintptr_t)u_d) % align) return fail(WFD_EINVAL, "fields must be %zu-byte aligned", (size_t)align);
+    hipStream_t st = (hipStream_t)stream;
+    void* P = u_a;  // u^{n-1}
+    void* C = u_b;  // u^n
+    void* X = u_c;
+    void* Y = u_d;
+    int64_t k = 0;
+    if (nsteps % 2) {  // one single step first; it records u^n into row n-1
+        rc = wfd::step_launch(g, P, C, P, med, src, rec, step0, -1, nullptr, nullptr, false, st);
+        if (rc) return rc;
+        std::swap(P, C);
+        k = 1;
+    }
+    for (; k + 2 <= nsteps; k += 2) {
+        // u^{n+1} -> X, u^{n+2} -> Y; the launch records u^{n-1}, u^n into rows n-2, n-1
+        Launch L{g, P, C, X, med, src, rec, step0 + k, nullptr, nullptr, nullptr, false, default_cfg(g), -1};
+        L.un2 = Y;
+        L.t2rows = rows ? rows : 16;
+        rc = do_launch(L, st);
+        if (rc) return rc;
+        std::swap(P, X);
+        std::swap(C, Y);
+    }
+    if (nsteps == 0) return WFD_OK;
+    if (nsteps >= 2) {
+        rc = wfd::gather_launch(g, P, rec, step0 + nsteps - 2, st);
+        if (rc) return rc;
+    }
+    rc = wfd::gather_launch(g, C, rec, step0 + nsteps - 1, st);
+    if (rc) return rc;
+    // leave (u^{N-1}, u^N) in (u_a, u_b)
+    const size_t bytes = (size_t)g->elems * (size_t)g->nshots * esize(g);
+    auto copy = [&](void* dst, const void* from) {
+        return hipMemcpyAsync(dst, from, bytes, hipMemcpyDeviceToDevice, st);
+    };
+    if (P == u_b && C == u_a) {  // swapped: go through a work field
+        HIP_TRY(copy(u_c, u_a), "wfd_forward2: copy of u^N");
+        HIP_TRY(copy(u_a, u_b), "wfd_forward2: copy of u^{N-1}");
+        HIP_TRY(copy(u_b, u_c), "wfd_forward2: copy of u^N");
+    } else {
+        if (P != u_a) HIP_TRY(copy(u_a, P), "wfd_forward2: copy of u^{N-1}");
+        if (C != u_b) HIP_TRY(copy(u_b, C), "wfd_forward2: copy of u^N");
+    }
     return WFD_OK;
 }
 

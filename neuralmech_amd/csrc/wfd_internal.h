@@ -61,6 +61,9 @@ struct StepArgs {
     uint32_t ngb;        // gather blocks (256 points each)
     uint32_t gx, gy;     // stencil block grid (x fastest); the launch is 1D: ngb + gx*gy*gz blocks
     int32_t nshots;
+    // Two-step launches: un receives u^{n+1}, un2 u^{n+2}; their gather blocks record up (u^{n-1})
+    // and uc (u^n) into trace rows step + gather_off - 1 and step + gather_off.
+    T* un2;
 };
 
 // Position of this block in a step launch: a gather block or a stencil tile.  The hardware hands
@@ -131,6 +134,12 @@ hipError_t launch_step3d(const StepArgs<T>& a, int order, bool varc, bool sponge
 
 template <typename T>
 hipError_t launch_gather(const StepArgs<T>& a, hipStream_t s);
+
+// Two steps per launch (2D): u^{n+1} -> a.un, u^{n+2} -> a.un2 from (a.up = u^{n-1}, a.uc = u^n);
+// all four fields distinct.  rows = output rows per wave (8, 16 or 32).
+template <typename T>
+hipError_t launch_step2d_t2(const StepArgs<T>& a, int order, bool varc, bool sponge, int rows, int nshots,
+                            hipStream_t s);
 
 hipError_t launch_add_i64(int64_t* p, int64_t v, hipStream_t s);
 hipError_t launch_set_i64(int64_t* p, int64_t v, hipStream_t s);

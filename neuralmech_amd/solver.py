@@ -295,12 +295,14 @@ def _sig_tensor(grid: WaveGrid, pts: Optional[PointSet], sig, nsteps: int):
 
 
 def run(grid: WaveGrid, medium: Medium, nsteps: int, src: Optional[PointSet] = None, sig=None,
-        rec: Optional[PointSet] = None, state=None, step0: int = 0, graph_chunk: int = 0):
+        rec: Optional[PointSet] = None, state=None, step0: int = 0, graph_chunk: int = 0, two_step: int = 0):
     """Advance ``nsteps`` steps; returns ``(traces, (u_prev, u_cur))``.
 
     ``traces[s, n, r] = u^{step0+n+1}`` at receiver ``r`` of shot ``s``; ``state`` is
     ``(u_prev, u_cur)`` layout tensors (rest state when omitted, updated in place otherwise).
-    ``graph_chunk`` > 0 replays a captured hipGraph of that many steps.
+    ``graph_chunk`` > 0 replays a captured hipGraph of that many steps; ``two_step`` > 0 (2D)
+    runs two steps per launch with that many output rows per wave (8, 16 or 32), using two
+    scratch fields; the results are the same bits.
     """
     lib = _lib.load()
     if state is None:
@@ -316,6 +318,13 @@ def run(grid: WaveGrid, medium: Medium, nsteps: int, src: Optional[PointSet] = N
     rp = rec.struct(traces) if rec is not None else None
     args = (ctypes.byref(grid.c), _ptr(ua), _ptr(ub), ctypes.byref(medium.c),
             ctypes.byref(sp) if sp else None, ctypes.byref(rp) if rp else None, step0, nsteps)
+    if two_step:
+        wc, wd = grid.zeros(), grid.zeros()
+        check(lib.wfd_forward2(ctypes.byref(grid.c), _ptr(ua), _ptr(ub), _ptr(wc), _ptr(wd), ctypes.byref(medium.c),
+                               ctypes.byref(sp) if sp else None, ctypes.byref(rp) if rp else None, step0, nsteps,
+                               int(two_step), _stream_ptr(grid.device)))
+        out = None if traces is None else traces[:, step0:]
+        return out, (ua, ub)  # (u^{N-1}, u^N) in place
     if graph_chunk:
         check(lib.wfd_forward_graph(*args, int(graph_chunk), _stream_ptr(grid.device)))
     else:
