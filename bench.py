@@ -42,7 +42,7 @@ def workload(n=2048, nsteps=4000, dtype=None):
     return dict(grid=g, medium=med, src=src, rec=rec, sig=sig, v=v, dt=dt, h=h, beta=beta, n=n, nsteps=nsteps)
 
 
-def cpu_baseline(w, budget_s=12.0):
+def cpu_baseline(w, budget_s=15.0):
     """C oracle (scalar restatement, OpenMP over rows) on the same model, bounded number of steps."""
     import numpy as np
 
@@ -53,10 +53,12 @@ def cpu_baseline(w, budget_s=12.0):
     rec = [(45, int(j)) for j in np.round(np.linspace(1, w["n"] - 2, 512)).astype(int)]
     kw = dict(beta_profiles=w["beta"], src_nodes=[(50, w["n"] // 2)], rec_nodes=rec, nthreads=threads)
     sig = w["sig"][:, None].astype(np.float32)
+    c_oracle.forward(kc, 8, w["h"], 2, src_sig=sig[:2], **kw)  # thread start-up
     t0 = time.perf_counter()
-    c_oracle.forward(kc, 8, w["h"], 2, src_sig=sig[:2], **kw)
+    c_oracle.forward(kc, 8, w["h"], 20, src_sig=sig[:20], **kw)
     probe = time.perf_counter() - t0
-    steps = int(max(4, min(400, budget_s / max(probe / 2, 1e-4))))
+    # about budget_s of host work, at most the whole shot
+    steps = int(max(20, min(len(sig), budget_s / max(probe / 20, 1e-5))))
     t0 = time.perf_counter()
     c_oracle.forward(kc, 8, w["h"], steps, src_sig=sig[:steps], **kw)
     el = time.perf_counter() - t0

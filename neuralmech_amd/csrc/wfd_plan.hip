@@ -110,7 +110,7 @@ int wfd_plan_create(wfd_plan** out, const wfd_grid* g, int64_t nsteps, int64_t s
     // the store must start (and so stay) zero.
     e = hipMemset(p->mem, 0, p->bytes);
     if (e != hipSuccess) {
-        hipFree(p->mem);
+        (void)hipFree(p->mem);
         delete p;
         return plan_fail(WFD_EHIP, "wfd_plan_create: clearing the checkpoint store failed");
     }
@@ -133,7 +133,7 @@ int wfd_plan_create(wfd_plan** out, const wfd_grid* g, int64_t nsteps, int64_t s
     p->mu[1] = take();
     p->gpart = take();
     if ((size_t)(q - p->mem) > p->bytes) {
-        hipFree(p->mem);
+        (void)hipFree(p->mem);
         delete p;
         return plan_fail(WFD_EINVAL, "wfd_plan_create: internal layout error");
     }
