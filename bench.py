@@ -216,6 +216,53 @@ def uneven_main(args):
     print(json.dumps(out))
 
 
+def roofline_main(args):
+    """The reference's roofline harness (2D_wave_roofline.py) over the reference-layout kernels:
+    copy ceiling, fp32 FMA peak, Gflop/s at 18 flop per dof on 1600 x 3200 for 2000 steps."""
+    from neuralmech_amd import harness
+
+    r = harness.roofline(steps=args.steps if args.steps != 4000 else 2000)
+    padded = next(k for k in r["kernels"] if k["kernel"] == "padded")
+    out = {"metric": "Gflop/s at 18 flop/dof, padded_step drop-in (1600x3200, 2000 steps)",
+           "value": round(padded["gflops"], 1), "unit": "Gflop/s", "n_gpus": 1, "steps": r["steps"], "warmup": 20,
+           "ms_per_step": round(padded["s_per_step"] * 1e3, 5), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (uniform random fields, as the reference)",
+           "config": {"workload": "reference roofline harness, 1600x3200 nodes, order 2, constant c"},
+           "bandwidth_GBs": round(r["bandwidth_GBs"], 1), "peak_Gflops": round(r["peak_Gflops"], 1),
+           "kernels": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in x.items()} for x in r["kernels"]]}
+    if args.csv_dir:
+        harness.save_csv(os.path.join(args.csv_dir, "wave_roofline.csv"), kernel=[k["kernel"] for k in r["kernels"]],
+                         intensity=[k["intensity"] for k in r["kernels"]], gflops=[k["gflops"] for k in r["kernels"]])
+        harness.save_csv(os.path.join(args.csv_dir, "wave_roofline_ceilings.csv"), bandwidth=[r["bandwidth_GBs"]],
+                         peak=[r["peak_Gflops"]])
+    print(json.dumps(out))
+
+
+def scaling_main(args):
+    """The reference's grid-size sweep (2D_wave_scaling.py): dofs against s/step for the padded
+    (fp32) and mixed (fp16) drop-ins, 20-step graphs."""
+    from neuralmech_amd import harness
+
+    res = {}
+    for version, kind in enumerate(("padded", "mixed")):
+        dofs, t = harness.scaling(kind, max_resolution=args.scaling_max, steps=args.scaling_steps)
+        res[kind] = (dofs, t)
+        if args.csv_dir:
+            harness.save_csv(os.path.join(args.csv_dir, f"wave_scaling_{version}.csv"), x=dofs, y=t)
+    d, t = res["padded"]
+    thr = [a / b for a, b in zip(d, t)]
+    best = max(range(len(thr)), key=lambda i: thr[i])
+    out = {"metric": "peak dofs/s over grid sizes, padded_step drop-in", "value": round(thr[best] / 1e9, 3),
+           "unit": "Gdofs/s", "n_gpus": 1, "steps": args.scaling_steps, "warmup": 20,
+           "ms_per_step": round(t[best] * 1e3, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic (uniform random fields)",
+           "config": {"workload": f"reference scaling sweep, square grids up to {max(d) ** 0.5 + 2:.0f}^2",
+                      "peak_at_dofs": d[best]},
+           "mixed_peak_Gdofs": round(max(a / b for a, b in zip(*res["mixed"])) / 1e9, 3),
+           "largest_dofs": max(d)}
+    print(json.dumps(out))
+
+
 def copy_main(args):
     """Device-to-device copy ceiling (the reference measures the same with a 1 GiB copy,
     2D_wave_roofline.py:46-57), at 1 GiB (HBM) and 64 MiB (fits the Infinity Cache); also the
@@ -312,7 +359,8 @@ def main():
     ap.add_argument("--graph", type=int, default=0, help="replay a hipGraph of this many steps (even)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch shape")
-    ap.add_argument("--workload", choices=["forward", "fwi", "copy", "3d", "uneven"], default="forward")
+    ap.add_argument("--workload", choices=["forward", "fwi", "copy", "3d", "uneven", "roofline", "scaling"],
+                    default="forward")
     ap.add_argument("--launch", default="", help="force VARIANT,ROWS instead of tuning")
     ap.add_argument("--two-step", type=int, default=0, help="forward: two steps per launch, ROWS output rows per wave")
     ap.add_argument("--n3d", type=int, default=512)
@@ -326,6 +374,9 @@ def main():
     ap.add_argument("--max-active", type=int, default=0, help="uneven: cells in flight (0 = default)")
     ap.add_argument("--no-phases", action="store_true", help="uneven: skip the per-phase launch timing")
     ap.add_argument("--no-sequential", action="store_true", help="uneven: skip the shot-by-shot baseline")
+    ap.add_argument("--csv-dir", default="", help="roofline / scaling: write the reference's CSV files here")
+    ap.add_argument("--scaling-max", type=int, default=100000, help="scaling: largest grid side")
+    ap.add_argument("--scaling-steps", type=int, default=400, help="scaling: steps timed per grid")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the shot runs on a single GPU and this build has no "
@@ -342,6 +393,12 @@ def main():
         return
     if args.workload == "uneven":
         uneven_main(args)
+        return
+    if args.workload == "roofline":
+        roofline_main(args)
+        return
+    if args.workload == "scaling":
+        scaling_main(args)
         return
 
     import torch

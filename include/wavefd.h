@@ -309,6 +309,14 @@ int wfd_ref_scaled_step(const int16_t* q0, const float* s0, const int16_t* q1, c
                         float* s2, double wavespeed, double dt, double dx, double dy, int32_t N, int32_t M,
                         int32_t M_padded, wfd_stream_t stream);
 
+/* ------------------------------------------------------------------- measurement probe */
+/* Peak fp32 FMA rate: `blocks` x 256 threads each run `n` iterations of 4 dependent fused
+ * multiply-adds (8 flop) in registers and write one float to out[blocks * 256].  Time it with
+ * events on `stream`.  Replaces the fma_chain RawKernel of the reference's roofline harness
+ * (projects/18_simple_wave_fd/2D_wave_roofline.py:55-68); built without packed fp32
+ * instructions, like every kernel here. */
+int wfd_fma_peak(float* out, int32_t blocks, int32_t n, wfd_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

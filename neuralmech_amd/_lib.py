@@ -134,6 +134,7 @@ SIGNATURES = {
     "wfd_batch_info": (ctypes.c_int, [_P, _I64P, _I64P, ctypes.POINTER(ctypes.c_size_t), _I64P]),
     "wfd_batch_shot_info": (ctypes.c_int, [_P, ctypes.c_int32, _I32P, _I32P, _I32P, _I64P, _I64P, _I64P]),
     "wfd_batch_shot_ops": (ctypes.c_int, [_I64, _I64, ctypes.c_int32, ctypes.c_int32, _I32P, _I64, _I64P, _I32P]),
+    "wfd_fma_peak": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P]),
     "wfd_ref_step2d": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P]),

@@ -205,6 +205,20 @@ int ref_args(int N, int M, int M_padded, double c, double dt, double dx, double 
     k.dy2 = (float)(dy * dy);
     return WFD_OK;
 }
+// Peak fp32 FMA rate probe: per thread a chain of fused multiply-adds held in registers, 4 per
+// iteration (8 flop).  The chain is dependent, so throughput comes from the other waves of the SIMD.
+__global__ __launch_bounds__(256) void fma_chain_kernel(float* out, float s, int n)
+{
+    float x = (float)threadIdx.x, y = s;
+    for (int i = 0; i < n; i++) {
+        x = __builtin_fmaf(x, y, s);
+        y = __builtin_fmaf(y, x, s);
+        x = __builtin_fmaf(x, y, s);
+        y = __builtin_fmaf(y, x, s);
+    }
+    out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = x + y;
+}
+
 int launched(const char* what)
 {
     hipError_t e = hipGetLastError();
@@ -288,6 +302,13 @@ int wfd_ref_mixed_step(const void* u0, const void* u1, void* u2, double wavespee
     hipLaunchKernelGGL(ref_mixed_kernel, grid, dim3(256), 0, (hipStream_t)stream, (const __half*)u0, (const __half*)u1,
                        (__half*)u2, k, N, M, (int64_t)M_padded);
     return launched("wfd_ref_mixed_step");
+}
+
+int wfd_fma_peak(float* out, int32_t blocks, int32_t n, wfd_stream_t stream)
+{
+    if (!out || blocks < 1 || n < 0) return set_error(WFD_EINVAL, "wfd_fma_peak: bad arguments");
+    hipLaunchKernelGGL(fma_chain_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, out, 1e-9f, n);
+    return launched("wfd_fma_peak");
 }
 
 int wfd_ref_scaled_step(const int16_t* q0, const float* s0, const int16_t* q1, const float* s1, int16_t* q2,

@@ -96,3 +96,15 @@ def test_indicator_other_than_one_is_refused():
     wave._check_indicator(s, np.ones(s.Nx))
     with pytest.raises(NotImplementedError):
         wave._check_indicator(s, np.full(s.Nx, 0.5))
+
+
+def test_save_csv_writes_the_reference_columns(tmp_path):
+    from neuralmech_amd import harness
+
+    harness.save_csv(tmp_path / "wave_scaling_0.csv", x=[1, 4, 9], y=[1e-6, 2e-6, 3e-6])
+    lines = (tmp_path / "wave_scaling_0.csv").read_text().splitlines()
+    assert lines[0] == "x y" and lines[1].split() == ["1", "1e-06"] and len(lines) == 4
+    import pytest
+
+    with pytest.raises(ValueError):
+        harness.save_csv(tmp_path / "bad.csv", x=[1, 2], y=[1])

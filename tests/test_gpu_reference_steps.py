@@ -137,3 +137,16 @@ def test_scaled_int16_bitexact():
     assert np.array_equal(gs1.cpu().numpy(), as1)
     dec = slice_ref.scaled_decode(aq1, as1)
     assert np.isfinite(dec).all() and math.isfinite(float(np.abs(dec).max()))
+
+
+def test_roofline_and_scaling_harness_small():
+    import math
+
+    from neuralmech_amd import harness
+
+    r = harness.roofline(nx=66, ny=260, steps=80)
+    assert r["bandwidth_GBs"] > 100 and r["peak_Gflops"] > 1000
+    assert [k["kernel"] for k in r["kernels"]] == list(harness.KERNELS)
+    assert all(math.isfinite(k["gflops"]) and k["gflops"] > 0 for k in r["kernels"])
+    dofs, t = harness.scaling("padded", max_resolution=40, steps=40)
+    assert dofs and len(dofs) == len(t) and all(x > 0 for x in t)
