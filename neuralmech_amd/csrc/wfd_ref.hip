@@ -38,6 +38,55 @@ __global__ __launch_bounds__(256) void ref_step2d_kernel(const float* u0, const 
     u2[c] = ref_update(u0[c], u1[c], u1[c - ld], u1[c + ld], u1[c - 1], u1[c + 1], k);
 }
 
+// ---------------------------------------------------------------- one step, 4-node chunks
+// For rows whose starts are 16-byte aligned (ld % 4 == 0): a lane owns 4 consecutive nodes, a
+// wave 256 columns of RI consecutive rows, the RI + 2 rows of u1 it needs are loaded once into
+// registers; the row neighbours across chunk boundaries come from lanes l - 1 / l + 1 (one
+// scalar load at each end of the wave).  Nodes outside the interior are never stored: chunks
+// that straddle the boundary column or the padding store element by element.
+template <int RI>
+__global__ __launch_bounds__(256) void ref_step2d_vec_kernel(const float* u0, const float* __restrict__ u1, float* u2,
+                                                             RefCoef k, int N, int M, int64_t ld)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t base = (int64_t)blockIdx.x * 256;
+    const int64_t c = base + 4 * lane;
+    const int64_t r0 = 1 + ((int64_t)blockIdx.y * 4 + wave) * RI;
+    if (r0 >= N - 1) return;  // whole wave
+    const bool in = c < M;
+    float4 w[RI + 2];
+#pragma unroll
+    for (int t = 0; t < RI + 2; t++) {
+        const int64_t r = r0 - 1 + t;
+        w[t] = (in && r < N) ? *reinterpret_cast<const float4*>(u1 + r * ld + c) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < RI; q++) {
+        const int64_t r = r0 + q;
+        if (r >= N - 1) break;  // whole wave
+        const float4 C = w[q + 1], Nn = w[q], S = w[q + 2];
+        float wl = __shfl_up(C.w, 1), er = __shfl_down(C.x, 1);
+        if (lane == 0) wl = base > 0 ? u1[r * ld + base - 1] : 0.f;
+        if (lane == 63) er = base + 256 < ld ? u1[r * ld + base + 256] : 0.f;
+        if (!in) continue;
+        const float4 P = *reinterpret_cast<const float4*>(u0 + r * ld + c);
+        float o[4];
+        o[0] = ref_update(P.x, C.x, Nn.x, S.x, wl, C.y, k);
+        o[1] = ref_update(P.y, C.y, Nn.y, S.y, C.x, C.z, k);
+        o[2] = ref_update(P.z, C.z, Nn.z, S.z, C.y, C.w, k);
+        o[3] = ref_update(P.w, C.w, Nn.w, S.w, C.z, er, k);
+        float* dst = u2 + r * ld + c;
+        if (c >= 1 && c + 3 <= M - 2) {
+            *reinterpret_cast<float4*>(dst) = make_float4(o[0], o[1], o[2], o[3]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                if (c + e >= 1 && c + e <= M - 2) dst[e] = o[e];
+        }
+    }
+}
+
 // ---------------------------------------------------------------- one step, LDS tile
 // 64 x 4 output tile staged with a one-node halo in LDS, as the shared-memory variant of the
 // reference does (step2D_wave.cu:187-240); identical results.
@@ -239,6 +288,14 @@ int wfd_ref_step2d(const float* u0, const float* u1, float* u2, double wavespeed
     int rc = ref_args(N, M, M_padded, wavespeed, dt, dx, dy, k);
     if (rc) return rc;
     if (!u0 || !u1 || !u2 || u1 == u2) return set_error(WFD_EINVAL, "wfd_ref_step2d: NULL or aliased u1/u2");
+    constexpr int RI = 8;
+    const bool vec = variant == 0 && M_padded % 4 == 0 && ((uintptr_t)u0 | (uintptr_t)u1 | (uintptr_t)u2) % 16 == 0;
+    if (vec) {
+        const dim3 g4((M + 255) / 256, (N - 2 + 4 * RI - 1) / (4 * RI));
+        hipLaunchKernelGGL((ref_step2d_vec_kernel<RI>), g4, dim3(256), 0, (hipStream_t)stream, u0, u1, u2, k, N, M,
+                           (int64_t)M_padded);
+        return launched("wfd_ref_step2d");
+    }
     const dim3 grid((M + 63) / 64, (N + 3) / 4);
     if (variant == 1)
         hipLaunchKernelGGL(ref_step2d_lds_kernel, grid, dim3(256), 0, (hipStream_t)stream, u0, u1, u2, k, N, M,

@@ -23,11 +23,16 @@ def _setup(NX=160, NY=320, pad=32):
     return dx, dy, dt, Mp, u
 
 
-@pytest.mark.parametrize("variant,pad", [("padded", 32), ("basic", 1), ("shared", 32)])
-def test_step2d_bitexact_with_numpy_fd_step(variant, pad):
+@pytest.mark.parametrize("variant,pad,NX,NY", [
+    ("padded", 32, 160, 320),   # 4-node chunks; the last chunk holds the boundary column
+    ("padded", 32, 133, 301),   # chunks straddle the boundary and the padding; ragged rows
+    ("padded", 32, 71, 520),    # three 256-column wave spans, the last one partial
+    ("basic", 1, 160, 317),     # unaligned rows: one node per thread
+    ("basic", 1, 97, 516),      # aligned unpadded rows take the chunked path
+    ("shared", 32, 160, 320)])
+def test_step2d_bitexact_with_numpy_fd_step(variant, pad, NX, NY):
     from neuralmech_amd import reference_steps as rs
 
-    NX, NY = 160, 320 if pad == 32 else 317
     dx, dy, dt, Mp, u = _setup(NX, NY, pad)
     a0, a1 = u.copy(), u.copy()
     g0 = torch.as_tensor(u).cuda()
