@@ -90,6 +90,15 @@ class BatchOpts(ctypes.Structure):
     ]
 
 
+class AcMedium(ctypes.Structure):
+    _fields_ = [
+        ("k", ctypes.c_void_p),
+        ("cn", ctypes.c_void_p),
+        ("cw", ctypes.c_void_p),
+        ("beta", ctypes.c_void_p),
+    ]
+
+
 # name -> (restype, argtypes); every symbol declared in include/wavefd.h
 _P = ctypes.c_void_p
 _GP = ctypes.POINTER(Grid)
@@ -135,6 +144,9 @@ SIGNATURES = {
     "wfd_batch_shot_info": (ctypes.c_int, [_P, ctypes.c_int32, _I32P, _I32P, _I32P, _I64P, _I64P, _I64P]),
     "wfd_batch_shot_ops": (ctypes.c_int, [_I64, _I64, ctypes.c_int32, ctypes.c_int32, _I32P, _I64, _I64P, _I32P]),
     "wfd_fma_peak": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P]),
+    "wfd_ac_forward": (ctypes.c_int, [_GP, _P, _P, ctypes.POINTER(AcMedium), _PP, _PP, _PP, _I64, _I64, _P]),
+    "wfd_ac_backward": (ctypes.c_int, [_GP, _P, _P, _P, _P, ctypes.POINTER(AcMedium), _PP, _PP, _PP, _PP, _I64,
+                                       _P, _P, _P]),
     "wfd_ref_step2d": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P]),

@@ -1,0 +1,538 @@
+"""Variable mass / stiffness acoustic wave over ``wfd_ac_forward`` / ``wfd_ac_backward``.
+
+Replaces cuwave's ``AcousticWave`` and its sensitivities as the drivers call them:
+``AcousticWave(Nx, dx, N, dt, threads, precision, space_order, rho1, rho2, kappa1, kappa2)`` and
+``parametrization_jacobian`` (``projects/8_physics_drivers/topopt_acoustic2D.py:119-131``,
+``projects/4_specialized_nns/analog_rnn_train.py:136``), ``response`` / ``response_gradient`` /
+``energy`` (``topopt_acoustic2D.py:167,179-181,194-196``), ``reconstruction_sensitivity`` and
+``reconstruction_nodes`` (``analog_rnn_train.py:142-146``, ``analog_rnn_fixture.py:84-87``).
+
+Model and discretisation (cuwave's source is not available, so these are this build's):
+
+* ``m p_tt = div(s grad p) + f`` with mass ``m = (1 - g)/kappa1 + g/kappa2`` and stiffness
+  ``s = (1 - g)/rho1 + g/rho2`` per node for an indicator ``g`` in [0, 1]: 1/rho and 1/kappa
+  interpolate, so a grey value mixes harmonically (``projects/1_benchmark_2D/README.md:141-143``);
+  the wave speed is ``sqrt(s / m)``.
+* Finite volumes on the nodes.  A face between two nodes carries ``s_f / h^2`` with ``s_f`` the
+  harmonic mean of the two nodes' stiffness (the reciprocal of the mean density), which keeps
+  the explicit step stable across the high-contrast interfaces of a binary design.
+* Closure ``"neumann"`` (no flux; the edges reflect, ``projects/1_benchmark_2D/forward/wave.py:43``):
+  faces to the boundary ring carry nothing, the first / last interior node of an axis owns half a
+  control volume and faces along a wall half an area (bilinear finite volumes with lumped mass), so
+  the walls sit on those nodes, ``dx = L / (n - 3)`` as at the call sites.
+  Closure ``"dirichlet"``: the ring holds zero, every face to it is active (the scalar model's).
+* Leapfrog with the sponge of the scalar model, ``(1+b) p+ = 2p - (1-b) p- + k (C p + f)``,
+  ``k = w dt^2 / m`` (``w`` the control-volume factor), sources divided by the cell volume.
+* Gradients by the hand-written adjoint with boundary reconstruction: the forward field is
+  rebuilt backwards in time from (p^{N-1}, p^N) where nothing damps it, and a strip of damped
+  nodes next to that region is recorded every step, ``(N+1) x strip`` values per shot.  Gradients
+  are therefore available on the interior nodes free of sponge damping and are zero elsewhere.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib, solver
+from ._lib import check
+from .solver import PointSet, WaveGrid, _ptr, _stream_ptr
+
+CLOSURES = ("neumann", "dirichlet")
+_PRECISIONS = {"float32": torch.float32, "float64": torch.float64}
+
+
+# ----------------------------------------------------------------------- coefficients
+def volume_weights(shape, closure: str) -> np.ndarray:
+    """Control-volume factor ``w`` per node: 2 per axis along which a node is the first or last
+    interior node under the no-flux closure, 1 otherwise."""
+    n0, n1 = shape
+    w = np.ones(shape)
+    if closure == "neumann":
+        for i in (1, n0 - 2):
+            w[i, :] *= 2.0
+        for j in (1, n1 - 2):
+            w[:, j] *= 2.0
+    return w
+
+
+def _face_areas(shape, closure: str):
+    """Relative face areas: cn[i, j] couples (i-1, j)-(i, j), cw[i, j] couples (i, j-1)-(i, j).
+
+    No-flux closure: faces with both ends interior; a face running along a wall (both ends on the
+    first / last interior row or column) has half the area, as the control volumes it separates
+    are halved there.  Zero-ring closure: every face with at least one interior end, full area.
+    """
+    n0, n1 = shape
+    an = np.zeros(shape)
+    aw = np.zeros(shape)
+    if closure == "neumann":
+        an[2:n0 - 1, 1:n1 - 1] = 1.0
+        aw[1:n0 - 1, 2:n1 - 1] = 1.0
+        an[:, 1] *= 0.5
+        an[:, n1 - 2] *= 0.5
+        aw[1, :] *= 0.5
+        aw[n0 - 2, :] *= 0.5
+    else:
+        an[1:n0, 1:n1 - 1] = 1.0
+        aw[1:n0 - 1, 1:n1] = 1.0
+    return an, aw
+
+
+def face_operator(mass: torch.Tensor, stiff: torch.Tensor, spacing, dt: float, closure: str):
+    """``(k, cn, cw)`` node / face arrays (float64, differentiable in ``mass`` and ``stiff``)."""
+    if closure not in CLOSURES:
+        raise ValueError(f"closure must be one of {CLOSURES}")
+    shape = tuple(mass.shape)
+    dev = mass.device
+    inter = torch.zeros(shape, dtype=torch.bool, device=dev)
+    inter[1:-1, 1:-1] = True
+    w = torch.as_tensor(volume_weights(shape, closure), device=dev)
+    k = torch.where(inter, dt * dt * w / mass, torch.zeros((), dtype=mass.dtype, device=dev))
+    h0, h1 = (float(h) for h in spacing)
+
+    def harm(a, b):
+        return 2.0 * a * b / (a + b)
+
+    zr = torch.zeros((1, shape[1]), dtype=stiff.dtype, device=dev)
+    zc = torch.zeros((shape[0], 1), dtype=stiff.dtype, device=dev)
+    cn = torch.cat([zr, harm(stiff[:-1, :], stiff[1:, :]) / (h0 * h0)], 0)
+    cw = torch.cat([zc, harm(stiff[:, :-1], stiff[:, 1:]) / (h1 * h1)], 1)
+    an, aw = (torch.as_tensor(a, dtype=stiff.dtype, device=dev) for a in _face_areas(shape, closure))
+    zero = torch.zeros((), dtype=stiff.dtype, device=dev)
+    return k, torch.where(an > 0, cn * an, zero), torch.where(aw > 0, cw * aw, zero)
+
+
+def _profiles(grid: WaveGrid, beta_profiles):
+    """Sponge profiles rounded to the grid dtype, as host float64 arrays (or None)."""
+    if beta_profiles is None:
+        return None
+    if len(beta_profiles) != 2:
+        raise ValueError("one damping profile per axis")
+    npdt = np.float32 if grid.dtype == torch.float32 else np.float64
+    out = []
+    for n, p in zip(grid.shape, beta_profiles):
+        p = np.asarray(torch.as_tensor(p, dtype=torch.float64).cpu(), np.float64).reshape(-1)
+        if p.size != n:
+            raise ValueError("damping profile length must equal the axis node count")
+        if not np.all(np.isfinite(p)) or np.any(p < 0) or np.any(p >= 1):
+            raise ValueError("damping profiles must be finite and in [0, 1)")
+        out.append(p.astype(npdt).astype(np.float64))
+    return out
+
+
+class AcMedium:
+    """Coefficient fields of the variable mass / stiffness model on a 2D order-2 grid.
+
+    ``mass`` / ``stiff`` are node arrays (or numbers) shared by all shots; ``closure`` is
+    ``"neumann"`` or ``"dirichlet"``; ``beta_profiles`` one damping profile per axis.
+    """
+
+    def __init__(self, grid: WaveGrid, mass, stiff, closure: str = "neumann", beta_profiles=None):
+        if grid.ndim != 2 or grid.order != 2:
+            raise ValueError("the variable mass / stiffness model runs on 2D grids of order 2")
+        if closure not in CLOSURES:
+            raise ValueError(f"closure must be one of {CLOSURES}")
+        if closure == "neumann" and min(grid.shape) < 4:
+            raise ValueError("the no-flux closure needs at least 4 nodes per axis")
+        self.grid, self.closure = grid, closure
+        dev = grid.device
+        self.mass = torch.as_tensor(mass, dtype=torch.float64).to(dev).expand(grid.shape).clone()
+        self.stiff = torch.as_tensor(stiff, dtype=torch.float64).to(dev).expand(grid.shape).clone()
+        for name, a in (("mass", self.mass), ("stiffness", self.stiff)):
+            if not bool(torch.isfinite(a).all()) or not bool((a > 0).all()):
+                raise ValueError(f"{name} must be finite and positive on every node")
+        k, cn, cw = face_operator(self.mass, self.stiff, grid.spacing, grid.dt, closure)
+        self.k, self.cn, self.cw = (grid.from_nodes(a, nshots=1, dtype=grid.dtype) for a in (k, cn, cw))
+        prof = _profiles(grid, beta_profiles)
+        self.profiles = prof
+        self.beta = None
+        if prof is not None:
+            self.beta = torch.as_tensor(np.concatenate(prof), dtype=grid.dtype).to(dev)
+        und = np.zeros(grid.shape, bool)
+        if prof is None:
+            und[1:-1, 1:-1] = True
+        else:
+            und[1:-1, 1:-1] = (prof[0][:, None] + prof[1][None, :])[1:-1, 1:-1] == 0
+        self.undamped = und
+        self.c = _lib.AcMedium()
+        self.c.k, self.c.cn, self.c.cw = self.k.data_ptr(), self.cn.data_ptr(), self.cw.data_ptr()
+        self.c.beta = None if self.beta is None else self.beta.data_ptr()
+
+    def strip_nodes(self) -> np.ndarray:
+        """Damped interior nodes with an undamped face neighbour, ``(S, 2)`` in C order."""
+        und = self.undamped
+        nb = np.zeros_like(und)
+        nb[1:, :] |= und[:-1, :]
+        nb[:-1, :] |= und[1:, :]
+        nb[:, 1:] |= und[:, :-1]
+        nb[:, :-1] |= und[:, 1:]
+        inter = np.zeros_like(und)
+        inter[1:-1, 1:-1] = True
+        return np.argwhere(inter & ~und & nb)
+
+    def check_step(self) -> float:
+        """Gershgorin bound of ``dt^2 lambda_max`` of the update (stable below 4)."""
+        k = self.grid.nodes(self.k)[0].double()
+        cn = self.grid.nodes(self.cn)[0].double()
+        cw = self.grid.nodes(self.cw)[0].double()
+        row = cn.clone()
+        row[:-1, :] += cn[1:, :]
+        row += cw
+        row[:, :-1] += cw[:, 1:]
+        return float((2.0 * k * row).max())
+
+
+# ------------------------------------------------------------------------- runs
+def forward(grid: WaveGrid, med: AcMedium, nsteps: int, src: Optional[PointSet] = None, sig=None,
+            rec: Optional[PointSet] = None, strip: Optional[PointSet] = None, state=None, step0: int = 0):
+    """Advance ``nsteps`` steps: ``(traces, store, (p_prev, p_cur))``.
+
+    ``traces[s, n, r]`` = p^{step0+n+1} at receiver r; ``store[s, n, i]`` = p^{step0+n} at strip node
+    i for n = 0..nsteps.  ``state`` is updated in place (rest when omitted).
+    """
+    lib = _lib.load()
+    ua, ub = (grid.zeros(), grid.zeros()) if state is None else state
+    total = step0 + nsteps
+    s_sig = solver._sig_tensor(grid, src, sig, total)
+    traces = None
+    if rec is not None and rec.count:
+        traces = torch.zeros(grid.nshots, total, rec.count, dtype=grid.dtype, device=grid.device)
+    store = None
+    if strip is not None and strip.count:
+        store = torch.zeros(grid.nshots, total + 1, strip.count, dtype=grid.dtype, device=grid.device)
+    sp = src.struct(s_sig) if src is not None else None
+    rp = rec.struct(traces) if rec is not None else None
+    tp = strip.struct(store) if strip is not None else None
+    check(lib.wfd_ac_forward(ctypes.byref(grid.c), _ptr(ua), _ptr(ub), ctypes.byref(med.c),
+                             ctypes.byref(sp) if sp else None, ctypes.byref(rp) if rp else None,
+                             ctypes.byref(tp) if tp else None, int(step0), int(nsteps), _stream_ptr(grid.device)))
+    if nsteps % 2:
+        ua, ub = ub, ua
+    tr = None if traces is None else traces[:, step0:]
+    st = None if store is None else store[:, step0:]
+    return tr, st, (ua, ub)
+
+
+def backward(grid: WaveGrid, med: AcMedium, nsteps: int, state, src: Optional[PointSet], sig, rec: PointSet,
+             dJdtr, strip: Optional[PointSet] = None, store=None):
+    """Adjoint sweep with reconstruction for ``J(traces)`` with ``dJ/dtraces = dJdtr``
+    ``(nshots, nsteps, R)``; ``state = (p^{N-1}, p^N)`` from :func:`forward` (consumed).
+
+    Returns ``(G, mu_src)``: ``G`` float64 ``(3, nshots, n0, n1)`` (mass term, north faces, west
+    faces; see ``wfd_ac_backward``) and ``mu_src`` ``(nshots, nsteps, S)`` or None.
+    """
+    lib = _lib.load()
+    g = grid
+    dJ = torch.as_tensor(dJdtr, device=g.device)
+    if dJ.shape != (g.nshots, nsteps, rec.count):
+        raise ValueError(f"dJ/dtraces must be (nshots, nsteps, receivers) = {(g.nshots, nsteps, rec.count)}")
+    adj = torch.zeros(g.nshots, nsteps + 1, rec.count, dtype=g.dtype, device=g.device)
+    adj[:, 1:] = (-dJ).to(g.dtype)
+    s_sig = solver._sig_tensor(g, src, sig, nsteps)
+    sp = src.struct(s_sig) if src is not None else None
+    mu_src = None
+    if src is not None and src.count:
+        mu_src = torch.zeros(g.nshots, nsteps, src.count, dtype=g.dtype, device=g.device)
+    mp = src.struct(mu_src) if mu_src is not None else None
+    ap = rec.struct(adj)
+    if strip is not None and strip.count and store is None:
+        raise ValueError("the strip needs the values the forward recorded")
+    tp = strip.struct(store.contiguous()) if strip is not None and strip.count else None
+    mu_a, mu_b = g.zeros(), g.zeros()
+    work = torch.zeros(3 * g.nshots * g.elems, dtype=g.dtype, device=g.device)
+    G = torch.zeros(3, g.nshots, g.elems, dtype=torch.float64, device=g.device)
+    ua, ub = state
+    check(lib.wfd_ac_backward(ctypes.byref(g.c), _ptr(ua), _ptr(ub), _ptr(mu_a), _ptr(mu_b), ctypes.byref(med.c),
+                              ctypes.byref(sp) if sp else None, ctypes.byref(ap) if ap else None,
+                              ctypes.byref(tp) if tp else None, ctypes.byref(mp) if mp else None, int(nsteps),
+                              _ptr(work), _ptr(G), _stream_ptr(g.device)))
+    Gn = torch.stack([g.nodes(G[a]) for a in range(3)])
+    return Gn, mu_src
+
+
+def material_gradient(med: AcMedium, G: torch.Tensor, mu_src, sig, src: Optional[PointSet]):
+    """``(dJ/dmass, dJ/dstiff)`` per shot ``(nshots, n0, n1)`` float64 from the sweep's sums:
+    ``dJ/dk = -(G[0] + sum_n mu^{n+1} sig^n) / k``, ``dJ/dcn = G[1]``, ``dJ/dcw = G[2]``, carried to
+    mass and stiffness through the coefficient map; zero off the undamped interior."""
+    g = med.grid
+    Gm = G[0].clone()
+    if mu_src is not None:
+        s_sig = solver._sig_tensor(g, src, sig, mu_src.shape[1])
+        contrib = (mu_src.double() * s_sig.double()[:, :, : src.count]).sum(1)
+        for s, f in enumerate(src.nodes_flat):
+            if len(f):
+                Gm[s].reshape(-1).index_add_(0, torch.as_tensor(f, device=Gm.device), contrib[s, : len(f)])
+    und = torch.as_tensor(med.undamped, device=Gm.device)
+    dm, ds = [], []
+    with torch.enable_grad():  # also inside an autograd.Function's backward
+        m = med.mass.detach().clone().requires_grad_(True)
+        st = med.stiff.detach().clone().requires_grad_(True)
+        k, cn, cw = face_operator(m, st, g.spacing, g.dt, med.closure)
+        kd = k.detach()
+        for s in range(g.nshots):
+            dk = torch.where(und, -Gm[s] / torch.where(und, kd, torch.ones_like(kd)), torch.zeros_like(kd))
+            gm_, gs_ = torch.autograd.grad([k, cn, cw], [m, st], [dk, G[1][s], G[2][s]], retain_graph=True)
+            dm.append(torch.where(und, gm_, torch.zeros_like(gm_)))
+            ds.append(torch.where(und, gs_, torch.zeros_like(gs_)))
+    return torch.stack(dm), torch.stack(ds)
+
+
+class _AcTraces(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, mass, stiff, grid, closure, beta_profiles, src, sig, rec):
+        med = AcMedium(grid, mass.detach(), stiff.detach(), closure, beta_profiles)
+        strip = PointSet(grid, [tuple(int(c) for c in p) for p in med.strip_nodes()])
+        nsteps = int(sig.shape[-2])
+        traces, store, state = forward(grid, med, nsteps, src, sig, rec, strip)
+        ctx.keep = (grid, med, src, sig, rec, strip, store, state, nsteps)
+        return traces
+
+    @staticmethod
+    def backward(ctx, grad_traces):
+        grid, med, src, sig, rec, strip, store, state, nsteps = ctx.keep
+        G, mu_src = backward(grid, med, nsteps, state, src, sig, rec, grad_traces.contiguous(), strip, store)
+        dm, ds = material_gradient(med, G, mu_src, sig, src)
+        ctx.keep = None
+        return dm.sum(0), ds.sum(0), None, None, None, None, None, None
+
+
+def acoustic_traces(mass, stiff, grid: WaveGrid, src: Optional[PointSet], sig, rec: PointSet,
+                    closure: str = "neumann", beta_profiles=None) -> torch.Tensor:
+    """Receiver traces ``(nshots, nsteps, R)`` of the variable mass / stiffness model, differentiable
+    in ``mass`` and ``stiff`` (node arrays) through the hand-written reconstruction adjoint."""
+    sig = torch.as_tensor(sig, device=grid.device)
+    return _AcTraces.apply(torch.as_tensor(mass, dtype=torch.float64, device=grid.device),
+                           torch.as_tensor(stiff, dtype=torch.float64, device=grid.device),
+                           grid, closure, beta_profiles, src, sig, rec)
+
+
+# --------------------------------------------------------------------- cuwave facade
+@dataclass(frozen=True)
+class AcousticWave:
+    """Simulation descriptor of cuwave's ``AcousticWave`` (fields as at
+    ``topopt_acoustic2D.py:119-131``); node conventions as ``ScalarWave``.  ``damping`` takes the
+    per-axis profiles of :func:`neuralmech_amd.boundary.sponge`."""
+
+    Nx: Sequence[int]
+    dx: Sequence[float]
+    N: int
+    dt: float
+    threads: Optional[Sequence[int]] = None
+    precision: str = "float32"
+    space_order: int = 2
+    rho1: float = 1.0
+    rho2: float = 1.0
+    kappa1: float = 1.0
+    kappa2: float = 1.0
+    damping: Optional[tuple] = None
+    boundary: str = "neumann"
+    device: str = "cuda"
+    _cache: dict = field(default_factory=dict, compare=False, repr=False)
+
+    def __post_init__(self):
+        object.__setattr__(self, "Nx", tuple(int(n) for n in self.Nx))
+        dx = self.dx if np.ndim(self.dx) else [self.dx] * len(self.Nx)
+        object.__setattr__(self, "dx", tuple(float(h) for h in dx))
+        if len(self.Nx) != 2 or len(self.dx) != 2:
+            raise ValueError("AcousticWave is two-dimensional")
+        if self.precision not in _PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
+        if self.space_order != 2:
+            raise NotImplementedError("AcousticWave is implemented for space_order = 2")
+        if self.boundary not in CLOSURES:
+            raise ValueError(f"boundary must be one of {CLOSURES}")
+        for name in ("rho1", "rho2", "kappa1", "kappa2"):
+            if not float(getattr(self, name)) > 0:
+                raise ValueError(f"{name} must be > 0")
+        limit = solver.stable_dt(self.dx, self.max_wavespeed, 2)
+        if self.dt > limit * (1 + 1e-12):
+            raise ValueError(f"dt = {self.dt:.6g} exceeds the stable step {limit:.6g} for wave speed "
+                             f"{self.max_wavespeed:.6g}")
+
+    @property
+    def max_wavespeed(self) -> float:
+        return float(max(np.sqrt(self.kappa1 / self.rho1), np.sqrt(self.kappa2 / self.rho2)))
+
+    @property
+    def ndim(self) -> int:
+        return 2
+
+    @property
+    def dtype(self) -> torch.dtype:
+        return _PRECISIONS[self.precision]
+
+    @property
+    def Nx_padded(self) -> tuple:
+        return self.Nx
+
+    @property
+    def cell_volume(self) -> float:
+        return float(np.prod(self.dx))
+
+    def _gamma(self, gamma) -> torch.Tensor:
+        g = torch.as_tensor(gamma if gamma is not None else 1.0, dtype=torch.float64).to(self.device)
+        g = g.expand(self.Nx) if g.dim() == 0 else g
+        if tuple(g.shape) != self.Nx:
+            raise ValueError(f"indicator must have the node shape {self.Nx}")
+        return g
+
+    def material(self, gamma):
+        """``(mass, stiffness)`` node arrays: ``1/kappa`` and ``1/rho`` interpolated linearly."""
+        g = self._gamma(gamma)
+        m = (1.0 - g) / self.kappa1 + g / self.kappa2
+        s = (1.0 - g) / self.rho1 + g / self.rho2
+        return m, s
+
+    def parametrization_jacobian(self, gamma):
+        """``(d mass / d gamma, d stiffness / d gamma)`` on the nodes."""
+        g = self._gamma(gamma)
+        return (torch.full_like(g, 1.0 / self.kappa2 - 1.0 / self.kappa1),
+                torch.full_like(g, 1.0 / self.rho2 - 1.0 / self.rho1))
+
+    def grid(self, nshots: int = 1) -> WaveGrid:
+        return WaveGrid(self.Nx, self.dx, self.dt, 2, self.dtype, nshots=nshots, device=self.device)
+
+
+def _node_list(nodes):
+    """Node indices as a list of (i, j) tuples from a (D, S) array, a (S, D) array or a Sensors."""
+    if hasattr(nodes, "nodes"):
+        nodes = nodes.nodes
+    a = np.asarray(nodes.cpu() if torch.is_tensor(nodes) else nodes, dtype=np.int64)
+    if a.ndim == 1:
+        a = a.reshape(2, -1)
+    if a.shape[0] == 2 and a.ndim == 2:
+        a = a.T
+    return [tuple(int(c) for c in p) for p in a]
+
+
+def _source_setup(grid, source, nsteps):
+    """PointSet and ``(1, nsteps, S)`` signal of one ``Source`` (or None)."""
+    if isinstance(source, (list, tuple)):
+        if len(source) != 1:
+            raise ValueError("one Source per call (a list of sources is a batch of shots)")
+        source = source[0]
+    if source is None:
+        return None, None
+    pts = PointSet(grid, [tuple(int(c) for c in p) for p in source.idx.T])
+    sig = np.zeros((1, nsteps, source.signal.shape[1]))
+    n = min(nsteps, source.signal.shape[0])
+    sig[0, :n] = source.signal[:n]
+    return pts, torch.as_tensor(sig)
+
+
+def _material(sim, gamma):
+    return sim.material(gamma)
+
+
+def _closure(sim):
+    return getattr(sim, "boundary", "neumann") if isinstance(sim, AcousticWave) else "dirichlet"
+
+
+def simulate_acoustic(sim, source=None, indicator=None, sensors=None, record_every=None):
+    """``simulate`` for a variable mass / stiffness descriptor (see ``wave.simulate``)."""
+    grid = sim.grid()
+    m, s = _material(sim, indicator)
+    med = AcMedium(grid, m, s, _closure(sim), sim.damping)
+    src, sig = _source_setup(grid, source, sim.N)
+    rec = PointSet(grid, _node_list(sensors)) if sensors is not None else None
+    state = (grid.zeros(), grid.zeros())
+    frames, traces, step = [], [], 0
+    chunk = int(record_every) if record_every else sim.N
+    while step < sim.N:
+        n = min(chunk, sim.N - step)
+        tr, _, state = forward(grid, med, n, src, sig, rec, state=state, step0=step)
+        if tr is not None:
+            traces.append(tr[0])
+        step += n
+        if record_every:
+            frames.append(grid.nodes(state[1])[0].detach().cpu().numpy().copy())
+    u = grid.nodes(state[1])[0].clone()
+    out = [u]
+    if sensors is not None:
+        out.append(torch.cat(traces, 0) if traces else torch.zeros(sim.N, 0, dtype=sim.dtype, device=sim.device))
+    if record_every:
+        out.append(frames)
+    return out[0] if len(out) == 1 else tuple(out)
+
+
+def energy(sim):
+    """Objective ``J = dt sum_t sum_r u^2`` of sensor traces, with ``dJ/du = 2 dt u``."""
+    dt = float(sim.dt)
+
+    def objective(traces):
+        t = torch.as_tensor(traces)
+        return float((t.double() ** 2).sum()) * dt, 2.0 * dt * t.double()
+
+    return objective
+
+
+def response(sim, source, gamma, sensors, objective):
+    """``(cost, traces)``: one forward run and the objective on the sensor traces ``(N, S)``."""
+    grid = sim.grid()
+    m, s = _material(sim, gamma)
+    med = AcMedium(grid, m, s, _closure(sim), sim.damping)
+    src, sig = _source_setup(grid, source, sim.N)
+    rec = PointSet(grid, _node_list(sensors))
+    tr, _, _ = forward(grid, med, sim.N, src, sig, rec)
+    cost, _ = objective(tr[0])
+    return cost, tr[0]
+
+
+def reconstruction_nodes(sim):
+    """``(strip, undamped)``: the strip of nodes the forward records for the reconstruction,
+    ``(2, S)`` node indices, and the mask of nodes where gradients are available."""
+    prof = None if sim.damping is None else [np.asarray(torch.as_tensor(p, dtype=torch.float64).cpu())
+                                               for p in sim.damping]
+    npdt = np.float32 if sim.precision == "float32" else np.float64
+    und = np.zeros(sim.Nx, bool)
+    if prof is None:
+        und[1:-1, 1:-1] = True
+    else:
+        p0, p1 = (np.asarray(p, np.float64).astype(npdt) for p in prof)
+        und[1:-1, 1:-1] = (p0[:, None] + p1[None, :])[1:-1, 1:-1] == 0
+    nb = np.zeros_like(und)
+    nb[1:, :] |= und[:-1, :]
+    nb[:-1, :] |= und[1:, :]
+    nb[:, 1:] |= und[:, :-1]
+    nb[:, :-1] |= und[:, 1:]
+    inter = np.zeros_like(und)
+    inter[1:-1, 1:-1] = True
+    return np.argwhere(inter & ~und & nb).T.copy(), und
+
+
+def reconstruction_sensitivity(sim, source, gamma, nodes, objective):
+    """``(loss, {"mass", "stiff"}, traces, p_N)``: forward with the strip recorded, the objective on
+    the traces at ``nodes`` (``(2, S)``) and the reconstruction adjoint; gradients with respect to
+    the mass and stiffness node arrays (zero where the sponge damps)."""
+    grid = sim.grid()
+    m, s = _material(sim, gamma)
+    med = AcMedium(grid, m, s, _closure(sim), sim.damping)
+    src, sig = _source_setup(grid, source, sim.N)
+    rec = PointSet(grid, _node_list(nodes))
+    strip = PointSet(grid, [tuple(int(c) for c in p) for p in med.strip_nodes()])
+    tr, store, state = forward(grid, med, sim.N, src, sig, rec, strip)
+    traces = tr[0]
+    loss, dJ = objective(traces)
+    p_last = grid.nodes(state[1])[0].clone()
+    G, mu_src = backward(grid, med, sim.N, state, src, sig, rec, torch.as_tensor(dJ)[None], strip, store)
+    dm, ds = material_gradient(med, G, mu_src, sig, src)
+    return loss, {"mass": dm[0], "stiff": ds[0]}, traces, p_last
+
+
+def response_gradient(sim, source, gamma, sensors, objective, adjoint=reconstruction_sensitivity):
+    """``(cost, dcost/dgamma)`` through ``adjoint`` (``reconstruction_sensitivity``) and the
+    indicator parametrisation."""
+    nodes = sensors.nodes if hasattr(sensors, "nodes") else sensors
+    loss, grads, _, _ = adjoint(sim, source, gamma, nodes, objective)
+    d_mass, d_stiff = sim.parametrization_jacobian(gamma)
+    return loss, d_mass * grads["mass"] + d_stiff * grads["stiff"]
+
+
+__all__ = ["AcMedium", "AcousticWave", "acoustic_traces", "backward", "energy", "face_operator", "forward",
+           "material_gradient", "reconstruction_nodes", "reconstruction_sensitivity", "response",
+           "response_gradient", "simulate_acoustic", "volume_weights"]

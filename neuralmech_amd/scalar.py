@@ -88,3 +88,6 @@ class ScalarWave:
         if tuple(v.shape) != self.Nx:
             raise ValueError(f"wavespeed array must have the node shape {self.Nx}")
         return v.to(self.device)
+
+
+from .acoustic import AcousticWave  # noqa: E402,F401  (cuwave.scalar exports both descriptors)

@@ -20,6 +20,7 @@ import torch
 
 from . import solver
 from .autograd import misfit_gradient as _misfit_gradient
+from .acoustic import energy, response, response_gradient  # noqa: F401  (cuwave.utils names)
 from .wave import _check_indicator, _setup
 
 
@@ -70,18 +71,29 @@ def line(p0, p1, count: int) -> np.ndarray:
 
 
 class Sensors:
-    """Receivers at the nodes nearest ``coords``; ``nodes`` is ``(R, D)``."""
+    """Receivers at the nodes nearest ``coords``; ``nodes`` is ``(D, R)`` node indices, the shape
+    the call sites read (``topopt_acoustic2D.py:144,154`` prints ``sensors.shape[1]`` nodes;
+    ``analog_rnn_train.py:143`` passes ``sensors.nodes`` on)."""
 
     def __init__(self, sim, coords):
-        self.nodes = _snap(sim, coords).T.copy()
+        self.nodes = _snap(sim, coords)
 
     @property
     def count(self) -> int:
-        return len(self.nodes)
+        return self.nodes.shape[1]
 
     def traces(self, record):
-        """A recorded ``(N, R)`` trace array (identity: simulate already samples the nodes)."""
+        """A recorded ``(N, R)`` trace array (identity: the solver already samples the nodes)."""
         return record
+
+    def objective(self, fn):
+        """``fn(traces (N, R)) -> (cost, dcost/dtraces)`` as an objective on the traces at
+        ``self.nodes`` (``analog_rnn_train.py:143``)."""
+
+        def objective(traces):
+            return fn(self.traces(traces))
+
+        return objective
 
 
 def measure(sim, sources: Sequence[Source], indicator, sensors: Sensors) -> list:

@@ -48,7 +48,8 @@ def _setup(sim, sources, sensors, nshots):
         sig = torch.as_tensor(sig)
     else:
         src, sig = None, None
-    rec = solver.PointSet(g, [tuple(int(c) for c in p) for p in sensors.nodes]) if sensors is not None else None
+    rec = solver.PointSet(g, [tuple(int(c) for c in p) for p in np.asarray(sensors.nodes).T]) \
+        if sensors is not None else None
     return g, med, src, sig, rec
 
 
@@ -59,6 +60,10 @@ def simulate(sim, source=None, indicator=None, sensors=None, record_every: Optio
     (``traces`` of shape ``(N, R)``), ``(u, snaps)`` with ``record_every`` (host arrays every that many
     steps), or ``(u, traces, frames)`` with both.
     """
+    from .acoustic import AcousticWave, simulate_acoustic
+
+    if isinstance(sim, AcousticWave):
+        return simulate_acoustic(sim, source, indicator, sensors, record_every)
     _check_indicator(sim, indicator)
     sources = [source] if source is not None else []
     g, med, src, sig, rec = _setup(sim, sources, sensors, 1)

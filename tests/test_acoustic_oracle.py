@@ -1,0 +1,350 @@
+"""Pins of the variable mass / stiffness model (oracle/acoustic_ref.py and the coefficient map of
+neuralmech_amd/acoustic.py) by what the mathematics fixes: energy conservation of the undamped
+leapfrog, the closed-form cosine modes of the no-flux closure, the reduction to the reference's
+5-point update for constant coefficients, second-order consistency of the face operator, the
+dot-product identity of the adjoint, the reconstruction against the full-history transpose,
+central finite differences, torch autograd through the coefficient map, and stability of the
+harmonic face average at the contrast of the topology-optimisation driver.  Mutation checks
+plant a mistake in the oracle inputs and watch the pin fail."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from neuralmech_amd import acoustic as ac
+from oracle import acoustic_ref as ar
+from oracle import slice_ref
+
+
+def _smooth(shape, seed, lo, hi):
+    rng = np.random.default_rng(seed)
+    x = np.linspace(0, 1, shape[0])[:, None]
+    y = np.linspace(0, 1, shape[1])[None, :]
+    f = np.zeros(shape)
+    for _ in range(4):
+        a, b, c, d = rng.uniform(0.5, 3.0, 4)
+        f += np.sin(a * math.pi * x + c) * np.cos(b * math.pi * y + d)
+    f = (f - f.min()) / (f.max() - f.min())
+    return lo + (hi - lo) * f
+
+
+def _coeffs(m, s, h, dt, closure):
+    k, cn, cw = ac.face_operator(torch.as_tensor(m, dtype=torch.float64), torch.as_tensor(s, dtype=torch.float64),
+                                 h, dt, closure)
+    return k.numpy(), cn.numpy(), cw.numpy()
+
+
+def _dt(m, s, h, safety=0.5):
+    cmax = float(np.sqrt((s / m).max()))
+    return safety * min(h) / cmax / math.sqrt(2)
+
+
+def _sponge(n, width, beta):
+    p = np.zeros(n)
+    for e in range(width):
+        v = beta * ((width - e) / width) ** 2
+        p[1 + e] = max(p[1 + e], v)
+        p[n - 2 - e] = max(p[n - 2 - e], v)
+    return p
+
+
+@pytest.mark.parametrize("closure", ["neumann", "dirichlet"])
+def test_energy_conserved_without_damping(closure):
+    shape, h = (23, 31), (0.1, 0.08)
+    m = _smooth(shape, 1, 0.5, 3.0)
+    s = _smooth(shape, 2, 0.2, 2.0)
+    dt = _dt(m, s, h, 0.7)
+    k, cn, cw = _coeffs(m, s, h, dt, closure)
+    rng = np.random.default_rng(3)
+    p0, p1 = np.zeros(shape), np.zeros(shape)
+    p0[1:-1, 1:-1] = rng.standard_normal((shape[0] - 2, shape[1] - 2))
+    p1[1:-1, 1:-1] = p0[1:-1, 1:-1] + 0.1 * rng.standard_normal((shape[0] - 2, shape[1] - 2))
+    D = 1.0 / k[1:-1, 1:-1]
+    energies = []
+    state = (p0, p1)
+    for _ in range(300):
+        _, _, state = ar.forward_np(k, cn, cw, 1, dtype=np.float64, state=state)
+        a, b = state
+        q, _ = ar.face_q(a, cn, cw)
+        v = (b - a)[1:-1, 1:-1]
+        energies.append(float((v * D * v).sum() - (b[1:-1, 1:-1] * q).sum()))
+    e = np.array(energies)
+    assert e[0] > 0
+    assert np.abs(e - e[0]).max() / e[0] < 1e-10
+
+
+def _mode_run(k, cn, cw, phi, omega, steps):
+    state = (math.cos(omega) * phi, phi.copy())
+    err = 0.0
+    for n in range(1, steps + 1):
+        _, _, state = ar.forward_np(k, cn, cw, 1, dtype=np.float64, state=state)
+        err = max(err, float(np.abs(state[1] - math.cos(n * omega) * phi).max()))
+    return err
+
+
+def test_neumann_cosine_modes_closed_form():
+    """Constant m, s under the no-flux closure: cos(a pi (i-1)/(n0-3)) cos(b pi (j-1)/(n1-3)) on the
+    interior is an exact mode with eigenvalue (dt^2 s/m) sum 4/h^2 sin^2(theta/2); the half
+    control volumes of the first / last interior nodes are what make it exact."""
+    shape, h, m, s = (17, 22), (0.1, 0.13), 2.0, 3.0
+    dt = 0.4 * min(h) / math.sqrt(s / m) / math.sqrt(2)
+    M = np.full(shape, m)
+    S = np.full(shape, s)
+    k, cn, cw = _coeffs(M, S, h, dt, "neumann")
+    a, b = 2, 3
+    ta, tb = a * math.pi / (shape[0] - 3), b * math.pi / (shape[1] - 3)
+    i = np.arange(shape[0])[:, None]
+    j = np.arange(shape[1])[None, :]
+    phi = np.cos(ta * (i - 1)) * np.cos(tb * (j - 1))
+    phi[0, :] = phi[-1, :] = phi[:, 0] = phi[:, -1] = 0.0
+    lam = dt * dt * s / m * (4 / h[0] ** 2 * math.sin(ta / 2) ** 2 + 4 / h[1] ** 2 * math.sin(tb / 2) ** 2)
+    omega = math.acos(1 - lam / 2)
+    assert _mode_run(k, cn, cw, phi, omega, 200) < 1e-10
+    # mutation: full control volumes at the walls (no factor 2) -- the mode is no longer exact
+    k_bad = np.zeros(shape)
+    k_bad[1:-1, 1:-1] = dt * dt / m
+    assert _mode_run(k_bad, cn, cw, phi, omega, 200) > 1e-3
+
+
+def test_dirichlet_constant_coefficients_reduce_to_reference_step():
+    """m = 1/c^2, s = 1 with the ring held at zero is the reference's 5-point update
+    (2D_wave_cupy.py:54-62, restated in slice_ref.fd_step_reference)."""
+    NX, NY = 40, 57
+    dx, dy = 1.0 / (NX - 1), 2.0 / (NY - 1)
+    c = 1.3
+    dt = slice_ref.reference_dt(dx, dy, c, 0.9)
+    u = slice_ref.gaussian_ic(NX, NY, center=(0.5, 0.9), sigma=0.08)
+    u[0, :] = u[-1, :] = u[:, 0] = u[:, -1] = 0.0
+    k, cn, cw = _coeffs(np.full((NX, NY), 1 / c ** 2), np.ones((NX, NY)), (dx, dy), dt, "dirichlet")
+    _, _, (_, pa) = ar.forward_np(k, cn, cw, 60, dtype=np.float64, state=(u.copy(), u.copy()))
+    a0, a1 = u.copy(), u.copy()
+    for _ in range(60):
+        slice_ref.fd_step_reference(a0, a1, a0, dt, dx, dy, c)
+        a0, a1 = a1, a0
+    assert np.linalg.norm(pa - a1) / np.linalg.norm(a1) < 1e-12
+
+
+def _div_s_grad_p_error(n, harmonic=True):
+    h = 1.0 / (n - 1)
+    x = np.linspace(0, 1, n)[:, None]
+    y = np.linspace(0, 1, n)[None, :]
+    s = 1.0 + 0.4 * np.sin(2 * x + 1) * np.cos(3 * y)
+    p = np.sin(2 * x + y) + np.cos(x * y)
+    sx = 0.8 * np.cos(2 * x + 1) * np.cos(3 * y)
+    sy = -1.2 * np.sin(2 * x + 1) * np.sin(3 * y)
+    px = 2 * np.cos(2 * x + y) - y * np.sin(x * y)
+    py = np.cos(2 * x + y) - x * np.sin(x * y)
+    pxx = -4 * np.sin(2 * x + y) - y * y * np.cos(x * y)
+    pyy = -np.sin(2 * x + y) - x * x * np.cos(x * y)
+    exact = sx * px + sy * py + s * (pxx + pyy)
+    if harmonic:
+        _, cn, cw = _coeffs(np.ones((n, n)), s, (h, h), 1.0, "dirichlet")
+    else:  # mutation: the face takes the stiffness of one end only (first order)
+        cn = np.zeros((n, n))
+        cw = np.zeros((n, n))
+        cn[1:, :] = s[:-1, :] / h ** 2
+        cw[:, 1:] = s[:, :-1] / h ** 2
+    q, _ = ar.face_q(p, cn, cw)
+    return float(np.abs(q - exact[1:-1, 1:-1]).max())
+
+
+def test_face_operator_is_second_order_consistent():
+    e = [_div_s_grad_p_error(n) for n in (33, 65, 129)]
+    assert e[0] / e[1] > 3.5 and e[1] / e[2] > 3.5
+    bad = [_div_s_grad_p_error(n, harmonic=False) for n in (33, 65, 129)]
+    assert bad[1] / bad[2] < 2.5  # the one-sided face converges only to first order
+
+
+def _problem(closure="neumann", shape=(26, 33), N=48, seed=5, sponge=True):
+    h = (0.05, 0.04)
+    m = _smooth(shape, seed, 0.6, 2.5)
+    s = _smooth(shape, seed + 1, 0.3, 1.8)
+    dt = _dt(m, s, h, 0.6)
+    k, cn, cw = _coeffs(m, s, h, dt, closure)
+    beta = (_sponge(shape[0], 5, 0.12), _sponge(shape[1], 6, 0.1)) if sponge else (None, None)
+    src = [(12, 15), (12, 15), (8, 20)]
+    t = np.arange(N) * dt
+    rng = np.random.default_rng(seed)
+    sig = np.stack([slice_ref.ricker(t - 0.1 * i, 1.0 / (12 * dt)) for i in range(3)], 1) * rng.uniform(0.5, 2, 3)
+    rec = [(7, 7), (15, 25), (18, 9), (10, 16)]
+    strip = ar.strip_nodes(beta[0], beta[1], shape, np.float64) if sponge else np.zeros((0, 2), np.int64)
+    return dict(m=m, s=s, h=h, dt=dt, k=k, cn=cn, cw=cw, beta=beta, src=src, sig=sig, rec=rec, strip=strip,
+                N=N, shape=shape, closure=closure)
+
+
+def _backward64(P, dJ):
+    tr, store, last = ar.forward_np(P["k"], P["cn"], P["cw"], P["N"], P["beta"], P["src"], P["sig"], P["rec"],
+                                    P["strip"], dtype=np.float64)
+    adj = np.zeros((P["N"] + 1, len(P["rec"])))
+    adj[1:] = -dJ(tr)
+    G, mu_src = ar.backward_np(P["k"], P["cn"], P["cw"], P["N"], last, P["beta"], P["src"], P["sig"], P["rec"],
+                               adj, P["strip"], store, dtype=np.float64)
+    return tr, G, mu_src
+
+
+@pytest.mark.parametrize("closure", ["neumann", "dirichlet"])
+def test_adjoint_dot_product(closure):
+    """<traces(sig), y> = <sig, -mu_src(y)>: the adjoint recursion, its injection and the source
+    gather are the transpose of the forward map."""
+    P = _problem(closure)
+    y = np.random.default_rng(9).standard_normal((P["N"], len(P["rec"])))
+    tr, G, mu_src = _backward64(P, lambda tr: y)
+    lhs = float((tr * y).sum())
+    rhs = float((P["sig"] * -mu_src).sum())
+    assert abs(lhs - rhs) / abs(lhs) < 1e-12
+    assert np.isfinite(G).all()
+
+
+def _dk_from_G(P, G, mu_src):
+    Gm = G[0].copy()
+    for s, p in enumerate(P["src"]):
+        Gm[p] += (mu_src[:, s] * P["sig"][:, s]).sum()
+    und = ar.undamped_mask(P["beta"][0], P["beta"][1], P["shape"], np.float64)
+    dk = np.zeros(P["shape"])
+    dk[und] = -Gm[und] / P["k"][und]
+    return dk, und
+
+
+@pytest.mark.parametrize("closure", ["neumann", "dirichlet"])
+def test_reconstruction_equals_full_history_transpose(closure):
+    P = _problem(closure)
+    obs = 0.7 * ar.traces_exact(P["k"], P["cn"], P["cw"], P["N"], P["beta"], P["src"], P["sig"], P["rec"])
+    tr, G, mu_src = _backward64(P, lambda tr: tr - obs)
+    dk, und = _dk_from_G(P, G, mu_src)
+    ek, ecn, ecw = ar.adjoint_exact(P["k"], P["cn"], P["cw"], P["N"], P["beta"], P["src"], P["sig"], P["rec"],
+                                    tr - obs)
+    assert np.abs(dk[und] - ek[und]).max() <= 1e-9 * np.abs(ek[und]).max()
+    # faces with an undamped end
+    fn = und.copy()
+    fn[1:, :] |= und[:-1, :]
+    fw = und.copy()
+    fw[:, 1:] |= und[:, :-1]
+    for got, ref, mask in ((G[1], ecn, fn), (G[2], ecw, fw)):
+        assert np.abs(got[mask] - ref[mask]).max() <= 1e-9 * np.abs(ref[mask]).max()
+
+
+def test_gradient_central_differences():
+    P = _problem("neumann", N=40)
+    obs = 0.5 * ar.traces_exact(P["k"], P["cn"], P["cw"], P["N"], P["beta"], P["src"], P["sig"], P["rec"])
+
+    def J(k, cn, cw):
+        tr = ar.traces_exact(k, cn, cw, P["N"], P["beta"], P["src"], P["sig"], P["rec"])
+        return 0.5 * float(((tr - obs) ** 2).sum())
+
+    tr = ar.traces_exact(P["k"], P["cn"], P["cw"], P["N"], P["beta"], P["src"], P["sig"], P["rec"])
+    ek, ecn, ecw = ar.adjoint_exact(P["k"], P["cn"], P["cw"], P["N"], P["beta"], P["src"], P["sig"], P["rec"],
+                                    tr - obs)
+    rng = np.random.default_rng(11)
+    und = ar.undamped_mask(P["beta"][0], P["beta"][1], P["shape"], np.float64)
+    cand = np.argwhere(und)
+    checks = 0
+    for which, ref in (("k", ek), ("cn", ecn), ("cw", ecw)):
+        for idx in cand[rng.choice(len(cand), 4, replace=False)]:
+            idx = tuple(idx)
+            arrs = {"k": P["k"].copy(), "cn": P["cn"].copy(), "cw": P["cw"].copy()}
+            base = arrs[which][idx]
+            if base == 0:
+                continue
+            eps = 1e-5 * abs(base)
+            arrs[which][idx] = base + eps
+            jp = J(arrs["k"], arrs["cn"], arrs["cw"])
+            arrs[which][idx] = base - eps
+            jm = J(arrs["k"], arrs["cn"], arrs["cw"])
+            fd = (jp - jm) / (2 * eps)
+            assert abs(fd - ref[idx]) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(ref).max()), (which, idx)
+            checks += 1
+    assert checks >= 10
+
+
+@pytest.mark.parametrize("closure", ["neumann", "dirichlet"])
+def test_material_gradient_matches_torch_autograd(closure):
+    """The coefficient map's chain rule (harmonic faces, volume weights, source term) in
+    ac.material_gradient against torch autograd through a float64 torch forward."""
+    from neuralmech_amd.solver import PointSet, WaveGrid
+
+    P = _problem(closure, N=36)
+    obs = 0.6 * ar.traces_exact(P["k"], P["cn"], P["cw"], P["N"], P["beta"], P["src"], P["sig"], P["rec"])
+    tr, G, mu_src = _backward64(P, lambda tr: tr - obs)
+    grid = WaveGrid(P["shape"], P["h"], P["dt"], 2, torch.float64, device="cpu")
+    med = ac.AcMedium(grid, P["m"], P["s"], closure, P["beta"])
+    src = PointSet(grid, P["src"])
+    dm, ds = ac.material_gradient(med, torch.as_tensor(G)[:, None], torch.as_tensor(mu_src)[None],
+                                  torch.as_tensor(P["sig"])[None], src)
+    m = torch.as_tensor(P["m"]).requires_grad_(True)
+    s = torch.as_tensor(P["s"]).requires_grad_(True)
+    k, cn, cw = ac.face_operator(m, s, P["h"], P["dt"], closure)
+    trt = ar.forward_torch(k, cn, cw, P["N"], P["beta"], P["src"], P["sig"], P["rec"])
+    J = 0.5 * ((trt - torch.as_tensor(obs)) ** 2).sum()
+    gm, gs = torch.autograd.grad(J, [m, s])
+    und = torch.as_tensor(med.undamped)
+    for got, ref in ((dm[0], gm), (ds[0], gs)):
+        ref = torch.where(und, ref, torch.zeros_like(ref))
+        assert float((got - ref).abs().max()) <= 1e-9 * float(ref.abs().max())
+
+
+def _contrast_run(harmonic):
+    """Binary air / aluminium design (topopt_acoustic2D.py:48-49), dt at half the CFL limit of the
+    faster phase, no sponge: the undamped scheme must stay bounded."""
+    shape, h = (40, 44), (0.05, 0.05)
+    rng = np.random.default_rng(2)
+    g = (rng.random(shape) > 0.5).astype(float)
+    rho1, rho2, k1, k2 = 1.204, 2643.0, 1.419e5, 6.87e8
+    m = (1 - g) / k1 + g / k2
+    s = (1 - g) / rho1 + g / rho2
+    cmax = max(math.sqrt(k1 / rho1), math.sqrt(k2 / rho2))
+    dt = 0.5 * min(h) / cmax / math.sqrt(2)
+    k, cn, cw = _coeffs(m, s, h, dt, "neumann")
+    if not harmonic:  # mutation: arithmetic mean of the stiffness on faces
+        a = torch.as_tensor(s)
+        cn = cn.copy()
+        cw = cw.copy()
+        mn = cn != 0
+        mw = cw != 0
+        an = np.zeros(shape)
+        aw = np.zeros(shape)
+        an[1:, :] = 0.5 * (s[:-1, :] + s[1:, :]) / h[0] ** 2
+        aw[:, 1:] = 0.5 * (s[:, :-1] + s[:, 1:]) / h[1] ** 2
+        cn = np.where(mn, an, 0)
+        cw = np.where(mw, aw, 0)
+        del a
+    p0 = np.zeros(shape)
+    p0[1:-1, 1:-1] = np.random.default_rng(4).standard_normal((shape[0] - 2, shape[1] - 2))
+    with np.errstate(over="ignore", invalid="ignore"):
+        _, _, (_, p) = ar.forward_np(k, cn, cw, 400, dtype=np.float64, state=(p0, p0.copy()))
+    return float(np.abs(p).max())
+
+
+def test_harmonic_faces_stable_at_high_contrast():
+    assert _contrast_run(True) < 50.0
+    big = _contrast_run(False)
+    assert not (big < 1e6)  # the arithmetic face mean is unstable at this contrast
+
+
+def test_facade_descriptor_and_reconstruction_nodes():
+    from neuralmech_amd import boundary
+
+    sim = ac.AcousticWave((40, 30), (0.1, 0.1), 100, 1e-4, rho1=1.2, rho2=2643.0, kappa1=1.4e5, kappa2=6.9e8,
+                          device="cpu")
+    g = torch.linspace(0, 1, 40 * 30, dtype=torch.float64).reshape(40, 30)
+    m, s = sim.material(g)
+    assert torch.allclose(m[0, 0], torch.tensor(1 / 1.4e5, dtype=torch.float64))
+    assert torch.allclose(s[-1, -1], torch.tensor(1 / 2643.0, dtype=torch.float64))
+    dm, ds = sim.parametrization_jacobian(g)
+    assert torch.allclose(dm, torch.full_like(g, 1 / 6.9e8 - 1 / 1.4e5))
+    assert torch.allclose(ds, torch.full_like(g, 1 / 2643.0 - 1 / 1.2))
+    with pytest.raises(ValueError):
+        ac.AcousticWave((40, 30), (0.1, 0.1), 100, 1.0, rho1=1.2, rho2=2643.0, kappa1=1.4e5, kappa2=6.9e8)
+    from dataclasses import replace
+
+    sim2 = replace(sim, damping=boundary.sponge(sim, None, 6, 0.1))
+    strip, und = ac.reconstruction_nodes(sim2)
+    assert strip.shape[0] == 2
+    ref = ar.strip_nodes(np.asarray(sim2.damping[0]), np.asarray(sim2.damping[1]), sim2.Nx)
+    assert np.array_equal(strip.T, ref)
+    z0 = int((np.asarray(sim2.damping[0])[1:-1] == 0).sum())
+    z1 = int((np.asarray(sim2.damping[1])[1:-1] == 0).sum())
+    assert 0 < und.sum() == z0 * z1 < (40 - 2) * (30 - 2)
+    # the strip is the ring of damped nodes around the undamped block: 2 (rows + cols) of it
+    assert strip.shape[1] == 2 * (z0 + z1)

@@ -57,7 +57,7 @@ def test_point_source_snaps_and_scales():
     many = utils.shots(s, utils.line((0.1, 0.0), (0.3, 0.0), 5), np.ones(100))
     assert len(many) == 5 and [int(m.idx[0, 0]) for m in many] == [11, 16, 21, 26, 31]
     sens = utils.Sensors(s, utils.line((0.0, 0.1), (0.0, 0.5), 3))
-    assert sens.nodes.shape == (3, 2) and sens.count == 3
+    assert sens.nodes.shape == (2, 3) and sens.count == 3
 
 
 def test_sponge_profiles_and_padding():

@@ -36,7 +36,7 @@ def test_simulate_matches_oracle_bitexact():
     torch.cuda.synchronize()
     kc = ((torch.as_tensor(v) * sim.dt) ** 2).float().numpy()
     src = [tuple(sources[1].idx[:, 0])]
-    rec = [tuple(p) for p in sensors.nodes]
+    rec = [tuple(p) for p in sensors.nodes.T]
     tr_o, _, uc_o = c_oracle.forward(kc, sim.space_order, sim.dx, sim.N, beta_profiles=sim.damping, src_nodes=src,
                                      src_sig=sources[1].signal.astype(np.float32), rec_nodes=rec)
     assert np.array_equal(u.cpu().numpy(), uc_o)
