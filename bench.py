@@ -351,6 +351,81 @@ def volume_main(args):
     print(json.dumps(out))
 
 
+def acoustic_main(args):
+    """Variable mass / stiffness model at the analog-RNN driver's size
+    (projects/4_specialized_nns/analog_rnn_fixture.py:22-45: 3000 x 1500 region, 10 m sponge on
+    every edge, air and polymer foam, fp32, order 2): forward steps with the reconstruction strip
+    recorded, then the reconstruction adjoint over the same steps."""
+    import numpy as np
+    import torch
+
+    from neuralmech_amd import acoustic as ac
+    from neuralmech_amd import boundary, media
+    from neuralmech_amd.solver import PointSet
+
+    res, lengths = (3000, 1500), (200.0, 100.0)
+    dx = tuple(lengths[d] / (res[d] - 3) for d in range(2))
+    Nx, width, origin, region = boundary.pad_for_sponge(res, dx, 10.0)
+    rho1, rho2, k1, k2 = 1.204, 30.0, 1.419e5, 9.72e5
+    cmax = max(np.sqrt(k1 / rho1), np.sqrt(k2 / rho2))
+    from neuralmech_amd import wave
+
+    dt = 0.99 * wave.stable_dt(dx, cmax, 2)
+    nsteps = args.acoustic_steps
+    sim = ac.AcousticWave(Nx, dx, nsteps, dt, precision="float32", rho1=rho1, rho2=rho2, kappa1=k1, kappa2=k2)
+    prof = boundary.sponge(sim, None, width, 0.05)
+    rng = np.random.default_rng(0)
+    gamma = np.zeros(Nx)
+    blk = (rng.random(((res[0] // 2) // 25, res[1] // 25)) > 0.5).repeat(25, 0).repeat(25, 1)
+    x0, y0 = region[0].start + res[0] // 4, region[1].start
+    gamma[x0:x0 + blk.shape[0], y0:y0 + blk.shape[1]] = blk
+    grid = sim.grid()
+    m, s = sim.material(torch.as_tensor(gamma))
+    med = ac.AcMedium(grid, m, s, "neumann", prof)
+    src = PointSet(grid, [(region[0].start + 150, region[1].start + 750)])
+    rec = PointSet(grid, [(region[0].stop - 150, region[1].start + j) for j in (375, 750, 1125)])
+    sn = med.strip_nodes()
+    strip = PointSet(grid, [tuple(int(c) for c in p) for p in sn])
+    sig = torch.as_tensor(media.ricker(np.arange(nsteps) * dt, 50.0)[None, :, None] / (dx[0] * dx[1]))
+    cells = (Nx[0] - 2) * (Nx[1] - 2)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ac.forward(grid, med, 20, src, sig[:, :20], rec, strip)  # warm-up
+    torch.cuda.synchronize()
+    ev[0].record()
+    tr, store, state = ac.forward(grid, med, nsteps, src, sig, rec, strip)
+    ev[1].record()
+    dJ = 2.0 * tr.double()
+    ev[2].record()
+    G, mu_src = ac.backward(grid, med, nsteps, state, src, sig, rec, dJ, strip, store)
+    ev[3].record()
+    torch.cuda.synchronize()
+    tf = ev[0].elapsed_time(ev[1]) / 1e3
+    tb = ev[2].elapsed_time(ev[3]) / 1e3
+    und = int(med.undamped.sum())
+    fwd_bytes = 24.0 * cells
+    bwd_bytes = 60.0 * und + 32.0 * (cells - und)  # imaging only where nothing damps
+    out = {
+        "metric": "Gcell-updates/s, variable mass/stiffness forward and reconstruction adjoint",
+        "value": round(cells * nsteps / tf / 1e9, 2), "unit": "Gcell-updates/s", "n_gpus": 1, "steps": nsteps,
+        "warmup": 20, "ms_per_step": round(tf / nsteps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded binary air/foam design, Ricker source)",
+        "config": {"workload": f"AcousticWave {Nx[0]}x{Nx[1]} nodes (3000x1500 region + {width}-node sponge), "
+                               "order 2, no-flux walls, air/foam", "strip_nodes": int(len(sn)),
+                   "strip_MB_per_1000_steps": round(1001 * len(sn) * 4 / 1e6, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(fwd_bytes * nsteps / tf / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(fwd_bytes * nsteps / tf / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "note": "forward: 24 algorithmic B per cell-step"},
+        "backward": {"gcell_updates_per_s": round(cells * nsteps / tb / 1e9, 2),
+                     "ms_per_step": round(tb / nsteps * 1e3, 4),
+                     "achieved_GB_per_s": round(bwd_bytes * nsteps / tb / 1e9, 1),
+                     "frac": round(bwd_bytes * nsteps / tb / 1e9 / HBM_PEAK_GBS, 4),
+                     "note": "60 B per undamped cell-step (7 fields read, 2 written, 3 fp32 partial sums "
+                             "read and written), 32 B per damped cell-step",
+                     "finite": bool(torch.isfinite(G).all())},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -359,7 +434,8 @@ def main():
     ap.add_argument("--graph", type=int, default=0, help="replay a hipGraph of this many steps (even)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch shape")
-    ap.add_argument("--workload", choices=["forward", "fwi", "copy", "3d", "uneven", "roofline", "scaling"],
+    ap.add_argument("--workload", choices=["forward", "fwi", "copy", "3d", "uneven", "roofline", "scaling",
+                                           "acoustic"],
                     default="forward")
     ap.add_argument("--launch", default="", help="force VARIANT,ROWS instead of tuning")
     ap.add_argument("--two-step", type=int, default=0, help="forward: two steps per launch, ROWS output rows per wave")
@@ -377,6 +453,7 @@ def main():
     ap.add_argument("--csv-dir", default="", help="roofline / scaling: write the reference's CSV files here")
     ap.add_argument("--scaling-max", type=int, default=100000, help="scaling: largest grid side")
     ap.add_argument("--scaling-steps", type=int, default=400, help="scaling: steps timed per grid")
+    ap.add_argument("--acoustic-steps", type=int, default=1000, help="acoustic: forward / adjoint steps timed")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the shot runs on a single GPU and this build has no "
@@ -399,6 +476,9 @@ def main():
         return
     if args.workload == "scaling":
         scaling_main(args)
+        return
+    if args.workload == "acoustic":
+        acoustic_main(args)
         return
 
     import torch

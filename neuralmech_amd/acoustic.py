@@ -410,7 +410,28 @@ def _node_list(nodes):
     return [tuple(int(c) for c in p) for p in a]
 
 
-def _source_setup(grid, source, nsteps):
+def _scale_of(sim):
+    """Per-node factor on source signals (``ScalarWave`` with an indicator: ``rho c^2``)."""
+    f = getattr(sim, "source_scale", None)
+    return None if f is None else np.asarray(f().detach().cpu(), np.float64)
+
+
+def _shots(grid, sources, nsteps, scale=None):
+    """PointSet (one list per shot) and ``(nshots, nsteps, width)`` signal of ``Source`` objects."""
+    nodes = [[tuple(int(c) for c in p) for p in src.idx.T] for src in sources]
+    pts = PointSet(grid, nodes, per_shot=True) if len(sources) > 1 else PointSet(grid, nodes[0])
+    width = max(src.signal.shape[1] for src in sources)
+    sig = np.zeros((len(sources), nsteps, width))
+    for i, src in enumerate(sources):
+        n = min(nsteps, src.signal.shape[0])
+        col = src.signal[:n]
+        if scale is not None:
+            col = col * scale[tuple(src.idx)][None, :]
+        sig[i, :n, : src.signal.shape[1]] = col
+    return pts, torch.as_tensor(sig)
+
+
+def _source_setup(grid, source, nsteps, scale=None):
     """PointSet and ``(1, nsteps, S)`` signal of one ``Source`` (or None)."""
     if isinstance(source, (list, tuple)):
         if len(source) != 1:
@@ -418,14 +439,12 @@ def _source_setup(grid, source, nsteps):
         source = source[0]
     if source is None:
         return None, None
-    pts = PointSet(grid, [tuple(int(c) for c in p) for p in source.idx.T])
-    sig = np.zeros((1, nsteps, source.signal.shape[1]))
-    n = min(nsteps, source.signal.shape[0])
-    sig[0, :n] = source.signal[:n]
-    return pts, torch.as_tensor(sig)
+    return _shots(grid, [source], nsteps, scale)
 
 
 def _material(sim, gamma):
+    if not isinstance(sim, AcousticWave) and (len(sim.Nx) != 2 or sim.space_order != 2):
+        raise NotImplementedError("the density indicator is implemented for 2D grids of space_order 2")
     return sim.material(gamma)
 
 
@@ -433,12 +452,49 @@ def _closure(sim):
     return getattr(sim, "boundary", "neumann") if isinstance(sim, AcousticWave) else "dirichlet"
 
 
+def _medium(sim, gamma, nshots=1):
+    grid = sim.grid(nshots=nshots)
+    m, s = _material(sim, gamma)
+    return grid, AcMedium(grid, m, s, _closure(sim), sim.damping)
+
+
+def measure_indicator(sim, sources, gamma, sensors) -> list:
+    """Traces ``(N, R)`` of every source, all shots in one launch per step."""
+    sources = list(sources)
+    grid, med = _medium(sim, gamma, len(sources))
+    src, sig = _shots(grid, sources, sim.N, _scale_of(sim))
+    rec = PointSet(grid, _node_list(sensors))
+    tr, _, _ = forward(grid, med, sim.N, src, sig, rec)
+    return [tr[k] for k in range(len(sources))]
+
+
+def shots_misfit(sim, sources, gamma, sensors, observed, gradient: bool = True):
+    """``1/2 sum (d - d_obs)^2`` over shots, and with ``gradient`` its derivative in the indicator:
+    one batched forward (strip recorded) and one reconstruction adjoint over all shots."""
+    sources = list(sources)
+    grid, med = _medium(sim, gamma, len(sources))
+    src, sig = _shots(grid, sources, sim.N, _scale_of(sim))
+    rec = PointSet(grid, _node_list(sensors))
+    sn = med.strip_nodes() if gradient else np.zeros((0, 2), np.int64)
+    strip = PointSet(grid, [tuple(int(c) for c in p) for p in sn]) if len(sn) else None
+    tr, store, state = forward(grid, med, sim.N, src, sig, rec, strip)
+    obs = torch.as_tensor(observed, dtype=torch.float64).to(grid.device)
+    if obs.shape != tr.shape:
+        raise ValueError(f"observed records must be (shots, N, receivers) = {tuple(tr.shape)}")
+    r = tr.double() - obs
+    cost = 0.5 * float((r * r).sum())
+    if not gradient:
+        return cost
+    G, mu_src = backward(grid, med, sim.N, state, src, sig, rec, r, strip, store)
+    dm, ds = material_gradient(med, G, mu_src, sig, src)
+    d_mass, d_stiff = sim.parametrization_jacobian(gamma)
+    return cost, d_mass * dm.sum(0) + d_stiff * ds.sum(0)
+
+
 def simulate_acoustic(sim, source=None, indicator=None, sensors=None, record_every=None):
     """``simulate`` for a variable mass / stiffness descriptor (see ``wave.simulate``)."""
-    grid = sim.grid()
-    m, s = _material(sim, indicator)
-    med = AcMedium(grid, m, s, _closure(sim), sim.damping)
-    src, sig = _source_setup(grid, source, sim.N)
+    grid, med = _medium(sim, indicator)
+    src, sig = _source_setup(grid, source, sim.N, _scale_of(sim))
     rec = PointSet(grid, _node_list(sensors)) if sensors is not None else None
     state = (grid.zeros(), grid.zeros())
     frames, traces, step = [], [], 0
@@ -473,10 +529,8 @@ def energy(sim):
 
 def response(sim, source, gamma, sensors, objective):
     """``(cost, traces)``: one forward run and the objective on the sensor traces ``(N, S)``."""
-    grid = sim.grid()
-    m, s = _material(sim, gamma)
-    med = AcMedium(grid, m, s, _closure(sim), sim.damping)
-    src, sig = _source_setup(grid, source, sim.N)
+    grid, med = _medium(sim, gamma)
+    src, sig = _source_setup(grid, source, sim.N, _scale_of(sim))
     rec = PointSet(grid, _node_list(sensors))
     tr, _, _ = forward(grid, med, sim.N, src, sig, rec)
     cost, _ = objective(tr[0])
@@ -509,10 +563,8 @@ def reconstruction_sensitivity(sim, source, gamma, nodes, objective):
     """``(loss, {"mass", "stiff"}, traces, p_N)``: forward with the strip recorded, the objective on
     the traces at ``nodes`` (``(2, S)``) and the reconstruction adjoint; gradients with respect to
     the mass and stiffness node arrays (zero where the sponge damps)."""
-    grid = sim.grid()
-    m, s = _material(sim, gamma)
-    med = AcMedium(grid, m, s, _closure(sim), sim.damping)
-    src, sig = _source_setup(grid, source, sim.N)
+    grid, med = _medium(sim, gamma)
+    src, sig = _source_setup(grid, source, sim.N, _scale_of(sim))
     rec = PointSet(grid, _node_list(nodes))
     strip = PointSet(grid, [tuple(int(c) for c in p) for p in med.strip_nodes()])
     tr, store, state = forward(grid, med, sim.N, src, sig, rec, strip)
@@ -534,5 +586,5 @@ def response_gradient(sim, source, gamma, sensors, objective, adjoint=reconstruc
 
 
 __all__ = ["AcMedium", "AcousticWave", "acoustic_traces", "backward", "energy", "face_operator", "forward",
-           "material_gradient", "reconstruction_nodes", "reconstruction_sensitivity", "response",
-           "response_gradient", "simulate_acoustic", "volume_weights"]
+           "material_gradient", "measure_indicator", "reconstruction_nodes", "reconstruction_sensitivity",
+           "response", "response_gradient", "shots_misfit", "simulate_acoustic", "volume_weights"]

@@ -12,8 +12,11 @@ Conventions of this build (cuwave's own are not available to check against):
   ``dx = L / (n - 3)`` convention of ``helper.py:51`` and ``plate.py:40``.
 * ``Nx_padded`` equals ``Nx``: node arrays have the node shape; the device layout's padding is
   internal to the solver.
-* ``wavespeed`` is a number or a node array; ``density`` is accepted and does not enter the
-  scalar model (``u_tt = v^2 lap u``).
+* ``wavespeed`` is a number or a node array.  Without an indicator the model is
+  ``u_tt = v^2 lap u`` and ``density`` does not enter; with a density indicator ``gamma`` (``simulate``,
+  ``measure``, ``misfit``, ``misfit_gradient``) it is the variable-density form
+  ``gamma rho u_tt = div(gamma rho v^2 grad u) + rho v^2 f`` on the zero boundary ring, run by the
+  variable mass / stiffness kernels (2D, order 2).
 * ``threads`` is accepted for compatibility; launch shapes are chosen by the solver's tuner.
 """
 
@@ -79,6 +82,41 @@ class ScalarWave:
     @property
     def cell_volume(self) -> float:
         return float(np.prod(self.dx))
+
+    # ---- density indicator (the variable-density form the indicator of misfit_gradient scales)
+    def _rho(self) -> torch.Tensor:
+        r = torch.as_tensor(self.density, dtype=torch.float64).to(self.device)
+        return r.expand(self.Nx) if r.dim() == 0 else r
+
+    def _gamma(self, gamma) -> torch.Tensor:
+        g = torch.as_tensor(gamma if gamma is not None else 1.0, dtype=torch.float64).to(self.device)
+        g = g.expand(self.Nx) if g.dim() == 0 else g
+        if tuple(g.shape) != self.Nx:
+            raise ValueError(f"indicator must have the node shape {self.Nx}")
+        return g
+
+    def material(self, gamma):
+        """``(mass, stiffness)`` of ``rho u_tt = div(rho c^2 grad u)`` with the density scaled by the
+        indicator: ``(gamma rho, gamma rho c^2)`` (2D)."""
+        g = self._gamma(gamma)
+        rho = self._rho()
+        return g * rho, g * rho * self.velocity() ** 2
+
+    def parametrization_jacobian(self, gamma):
+        """``(d mass / d gamma, d stiffness / d gamma) = (rho, rho c^2)``."""
+        self._gamma(gamma)
+        rho = self._rho()
+        return rho.clone(), rho * self.velocity() ** 2
+
+    def source_scale(self) -> torch.Tensor:
+        """Sources enter as ``rho c^2 f`` so that a unit indicator reproduces ``u_tt = c^2 (lap u + f)``
+        of the constant-density model where ``c`` is constant."""
+        return self._rho() * self.velocity() ** 2
+
+    def grid(self, nshots: int = 1):
+        from .solver import WaveGrid
+
+        return WaveGrid(self.Nx, self.dx, self.dt, 2, self.dtype, nshots=nshots, device=self.device)
 
     def velocity(self) -> torch.Tensor:
         """Wave speed as a float64 node array on the device."""

@@ -6,8 +6,8 @@ over the wavefd solver.
 Conventions of this build: points snap to the nearest node (node ``i`` at ``(i - 1) dx``,
 clipped to the interior); source signals are divided by the cell volume, as cuwave's
 ``point_source`` does (``9_structural_optimization/README.md:87-88``); the model argument of
-``misfit`` / ``misfit_gradient`` is the wave speed node array (cuwave differentiates a density
-indicator there, which this solver does not implement).
+``misfit`` / ``misfit_gradient`` is the density indicator (variable-density model, see
+``scalar.ScalarWave.material``); ``wavespeed_misfit(_gradient)`` take the wave speed instead.
 """
 
 from __future__ import annotations
@@ -21,7 +21,7 @@ import torch
 from . import solver
 from .autograd import misfit_gradient as _misfit_gradient
 from .acoustic import energy, response, response_gradient  # noqa: F401  (cuwave.utils names)
-from .wave import _check_indicator, _setup
+from .wave import _setup, _unit_indicator
 
 
 @dataclass
@@ -97,8 +97,12 @@ class Sensors:
 
 
 def measure(sim, sources: Sequence[Source], indicator, sensors: Sensors) -> list:
-    """Traces ``(N, R)`` of every source, all shots batched in one launch per time step."""
-    _check_indicator(sim, indicator)
+    """Traces ``(N, R)`` of every source, all shots batched in one launch per time step.  A density
+    indicator other than 1 (or an ``AcousticWave``) runs the variable mass / stiffness model."""
+    from .acoustic import AcousticWave, measure_indicator
+
+    if isinstance(sim, AcousticWave) or not _unit_indicator(indicator):
+        return measure_indicator(sim, list(sources), indicator, sensors)
     g, med, src, sig, rec = _setup(sim, list(sources), sensors, len(sources))
     tr, _ = solver.run(g, med, sim.N, src=src, sig=sig, rec=rec)
     return [tr[k] for k in range(len(sources))]
@@ -114,8 +118,25 @@ def _observed(observed, sim, nshots):
     return obs.to(sim.device, torch.float64)
 
 
-def misfit(sim, sources, model, sensors, observed) -> float:
-    """``1/2 sum (d - d_obs)^2`` over all shots for wave speed ``model``."""
+def misfit(sim, sources, gamma, sensors, observed) -> float:
+    """``1/2 sum (d - d_obs)^2`` over all shots for the density indicator ``gamma``
+    (``9_structural_optimization/fwi_penalization.py:139``; ``16_neuralopt/neuralfwi.py:60``)."""
+    from .acoustic import shots_misfit
+
+    return shots_misfit(sim, sources, gamma, sensors, _observed(observed, sim, len(sources)), gradient=False)
+
+
+def misfit_gradient(sim, sources, gamma, sensors, observed):
+    """``(cost, dcost/dgamma)`` on the nodes for the density indicator ``gamma``
+    (``projects/18_fwi/2D_fwi.py:40``): one batched forward recording the reconstruction strip and
+    one hand-written adjoint sweep over all shots (zero gradient where a sponge damps)."""
+    from .acoustic import shots_misfit
+
+    return shots_misfit(sim, sources, gamma, sensors, _observed(observed, sim, len(sources)), gradient=True)
+
+
+def wavespeed_misfit(sim, sources, model, sensors, observed) -> float:
+    """``1/2 sum (d - d_obs)^2`` over all shots for the wave speed node array ``model``."""
     from dataclasses import replace
 
     s2 = replace(sim, wavespeed=torch.as_tensor(model, dtype=torch.float64))
@@ -124,9 +145,9 @@ def misfit(sim, sources, model, sensors, observed) -> float:
     return 0.5 * float((r * r).sum())
 
 
-def misfit_gradient(sim, sources, model, sensors, observed):
-    """``(cost, dcost/dv)`` on the nodes: one batched forward with sqrt(n) checkpoints and one
-    hand-written adjoint pass over all shots."""
+def wavespeed_misfit_gradient(sim, sources, model, sensors, observed):
+    """``(cost, dcost/dv)`` in the wave speed parametrisation: one batched forward with sqrt(n)
+    checkpoints and one hand-written adjoint pass over all shots."""
     g, _, src, sig, rec = _setup(sim, list(sources), sensors, len(sources))
     v = torch.as_tensor(model, dtype=torch.float64, device=sim.device)
     return _misfit_gradient(v, g, src, sig, rec, _observed(observed, sim, len(sources)), sim.damping)

@@ -25,14 +25,11 @@ def grid_coords(Nx, dx, dtype=np.float64):
     return [a.astype(dtype) for a in np.meshgrid(*axes, indexing="ij")]
 
 
-def _check_indicator(sim, indicator):
+def _unit_indicator(indicator) -> bool:
+    """True when there is no density indicator to apply (None or 1 everywhere)."""
     if indicator is None:
-        return
-    g = torch.as_tensor(indicator)
-    if not bool(torch.all(g == 1)):
-        raise NotImplementedError(
-            "a density indicator (the variable-density parametrisation of cuwave's misfit_gradient) "
-            "is not implemented by this solver; pass the model as ScalarWave(wavespeed=...)")
+        return True
+    return bool(torch.all(torch.as_tensor(indicator) == 1))
 
 
 def _setup(sim, sources, sensors, nshots):
@@ -56,15 +53,15 @@ def _setup(sim, sources, sensors, nshots):
 def simulate(sim, source=None, indicator=None, sensors=None, record_every: Optional[int] = None):
     """Run ``sim.N`` steps from rest.
 
-    Returns ``u`` (final field, node array on the device), ``(u, traces)`` with ``sensors``
+    A density ``indicator`` other than 1 (or an ``AcousticWave``) runs the variable mass / stiffness
+    model.  Returns ``u`` (final field, node array on the device), ``(u, traces)`` with ``sensors``
     (``traces`` of shape ``(N, R)``), ``(u, snaps)`` with ``record_every`` (host arrays every that many
     steps), or ``(u, traces, frames)`` with both.
     """
     from .acoustic import AcousticWave, simulate_acoustic
 
-    if isinstance(sim, AcousticWave):
+    if isinstance(sim, AcousticWave) or not _unit_indicator(indicator):
         return simulate_acoustic(sim, source, indicator, sensors, record_every)
-    _check_indicator(sim, indicator)
     sources = [source] if source is not None else []
     g, med, src, sig, rec = _setup(sim, sources, sensors, 1)
     state = (g.zeros(), g.zeros())

@@ -6,6 +6,7 @@ from dataclasses import replace
 
 import numpy as np
 import pytest
+import torch
 
 from neuralmech_amd import boundary, media, signals, utils, wave
 from neuralmech_amd.scalar import ScalarWave
@@ -90,12 +91,20 @@ def test_sineburst_window():
     assert r[0] == 1.0
 
 
-def test_indicator_other_than_one_is_refused():
+def test_density_indicator_parametrisation():
     s = _sim()
-    wave._check_indicator(s, None)
-    wave._check_indicator(s, np.ones(s.Nx))
-    with pytest.raises(NotImplementedError):
-        wave._check_indicator(s, np.full(s.Nx, 0.5))
+    assert wave._unit_indicator(None) and wave._unit_indicator(np.ones(s.Nx))
+    assert not wave._unit_indicator(np.full(s.Nx, 0.5))
+    g = torch.full(s.Nx, 0.25, dtype=torch.float64)
+    m, k = s.material(g)
+    v = s.velocity()
+    rho = float(torch.as_tensor(s.density, dtype=torch.float64))
+    assert torch.allclose(m, 0.25 * rho * torch.ones_like(m))
+    assert torch.allclose(k, 0.25 * rho * v ** 2)
+    dm, dk = s.parametrization_jacobian(g)
+    assert torch.allclose(dm, torch.full_like(g, rho)) and torch.allclose(dk, rho * v ** 2)
+    with pytest.raises(ValueError):
+        s.material(np.ones((3, 3)))
 
 
 def test_save_csv_writes_the_reference_columns(tmp_path):
