@@ -16,7 +16,7 @@ Model and discretisation (cuwave's source is not available, so these are this bu
 * Finite volumes on the nodes.  A face between two nodes carries ``s_f / h^2`` with ``s_f`` the
   harmonic mean of the two nodes' stiffness (the reciprocal of the mean density), which keeps
   the explicit step stable across the high-contrast interfaces of a binary design.
-* Closure ``"neumann"`` (no flux; the edges reflect, ``projects/1_benchmark_2D/forward/wave.py:43``):
+* Closure ``"neumann"`` (no flux; the edges reflect, ``projects/1_benchmark_2D/forward/wave.py:44``):
   faces to the boundary ring carry nothing, the first / last interior node of an axis owns half a
   control volume and faces along a wall half an area (bilinear finite volumes with lumped mass), so
   the walls sit on those nodes, ``dx = L / (n - 3)`` as at the call sites.

@@ -285,7 +285,7 @@ def test_material_gradient_matches_torch_autograd(closure):
 
 
 def _contrast_run(harmonic):
-    """Binary air / aluminium design (topopt_acoustic2D.py:48-49), dt at half the CFL limit of the
+    """Binary air / aluminium design (topopt_acoustic2D.py:51-52), dt at half the CFL limit of the
     faster phase, no sponge: the undamped scheme must stay bounded."""
     shape, h = (40, 44), (0.05, 0.05)
     rng = np.random.default_rng(2)
