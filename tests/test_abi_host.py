@@ -163,3 +163,39 @@ def test_layout_views_round_trip_on_cpu(shape):
     g.nodes(mask).fill_(False)
     assert (f[mask] == 0).all()
     assert g.interior_cells == int(np.prod([n - 2 for n in shape]))
+
+
+def _ac_medium():
+    m = _lib.AcMedium()
+    m.k = m.cn = m.cw = 0x1000  # never dereferenced: every call below fails validation first
+    return m
+
+
+@pytest.mark.parametrize("case", ["null_grid", "null_medium", "ndim3", "order4", "alias", "neg_steps",
+                                  "null_coef", "bad_points"])
+def test_acoustic_entry_points_validate_arguments(case):
+    lib = _lib.load()
+    shape, order = ((6, 7, 8), 2) if case == "ndim3" else ((12, 14), 4 if case == "order4" else 2)
+    rc, g = _grid(shape, [1.0] * len(shape), order=order)
+    assert rc == 0
+    med = _ac_medium()
+    if case == "null_coef":
+        med.cn = None
+    a, b = ctypes.c_void_p(0x2000), ctypes.c_void_p(0x3000 if case != "alias" else 0x2000)
+    pts = None
+    if case == "bad_points":
+        p = _lib.Points()
+        p.count, p.nrows = 1, 3  # the grid has 12 CSR rows
+        p.row_ptr = p.col = p.id = p.row = p.data = 0x4000
+        p.width, p.nsteps, p.shot_stride = 1, 4, 4
+        pts = ctypes.byref(p)
+    gp = None if case == "null_grid" else ctypes.byref(g)
+    mp = None if case == "null_medium" else ctypes.byref(med)
+    steps = -1 if case == "neg_steps" else 4
+    rc = lib.wfd_ac_forward(gp, a, b, mp, pts, None, None, 0, steps, None)
+    assert rc == _lib.WFD_EINVAL, _lib.last_error()
+    c, d = ctypes.c_void_p(0x5000), ctypes.c_void_p(0x6000)
+    rc = lib.wfd_ac_backward(gp, a, b, c, d, mp, pts, None, None, None, steps, ctypes.c_void_p(0x7000),
+                             ctypes.c_void_p(0x8000), None)
+    assert rc == _lib.WFD_EINVAL, _lib.last_error()
+    assert _lib.last_error()
