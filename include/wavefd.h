@@ -309,19 +309,25 @@ int wfd_ref_scaled_step(const int16_t* q0, const float* s0, const int16_t* q1, c
                         float* s2, double wavespeed, double dt, double dx, double dy, int32_t N, int32_t M,
                         int32_t M_padded, wfd_stream_t stream);
 
-/* ------------------------------------- variable mass / stiffness model (2D, order 2) */
+/* ---------------------------------- variable mass / stiffness model (2D, orders 2..8) */
 /* m p_tt = div(s grad p) + f with a mass m and stiffness s per node, cuwave's AcousticWave
  * (projects/8_physics_drivers/topopt_acoustic2D.py:119-131; 1/rho and 1/kappa interpolated by
- * the indicator, projects/1_benchmark_2D/README.md:141-143).  The library sees it as a symmetric
- * face operator, per interior node x:
- *     q = (cn[x] (p[N]-p[x]) + cn[S] (p[S]-p[x])) + (cw[x] (p[W]-p[x]) + cw[E] (p[E]-p[x]))
+ * the indicator, projects/1_benchmark_2D/README.md:141-143) and ScalarWave's density indicator
+ * (projects/18_fwi/helper.py:31-33).  The library sees it as a symmetric bond operator with
+ * K = order/2 bonds per side and axis, per interior node x:
+ *     qx = cN_1 dN_1 + cS_1 dS_1;  for m = 2..K: qx = qx + (cN_m dN_m + cS_m dS_m)
+ *     qy = cW_1 dW_1 + cE_1 dE_1;  for m = 2..K: qy = qy + (cW_m dW_m + cE_m dE_m)
+ *     q = qx + qy,   dN_m = p[x - m rows] - p[x] (S: + m rows, W / E: -+ m columns)
  *     t = 2 p[x] - (1 - b) p_prev[x]     (2 p[x] - p_prev[x] without sponge)
  *     num = t + k[x] q;  per source at x, in list order: num = num + k[x] sig
  *     p_next[x] = num / (1 + b)          (num without sponge)
- * cn[y] couples y to its north neighbour (row - 1), cw[y] to its west neighbour (column - 1);
- * the caller folds mass, control-volume weights, spacing, closure and the face averaging of s
- * into k and the face coefficients (zero faces make a no-flux boundary).  All three are fields
- * in the halo layout of the grid's dtype, shared by every shot; beta is as in wfd_medium. */
+ * with cN_m = cn_m[x], cS_m = cn_m[x + m rows], cW_m = cw_m[x], cE_m = cw_m[x + m columns]:
+ * cn_m[y] couples y to the node m rows above it, cw_m[y] to the node m columns left of it.  The
+ * field is zero outside the node box.  k, cn and cw are fields in the halo layout of the grid's
+ * dtype shared by every shot: k one field, cn and cw K fields each (`elems` apart, field m-1 for
+ * distance m), entries up to K-1 rows / columns past the last node included.  The caller folds
+ * mass, control-volume weights, spacing, closure and the averaging of s onto bonds into them (a
+ * zero bond carries nothing).  beta is as in wfd_medium. */
 typedef struct wfd_ac_medium {
     const void* k;
     const void* cn;
@@ -332,8 +338,9 @@ typedef struct wfd_ac_medium {
 /* nsteps steps from (u_a = p^{n-1}, u_b = p^n), n = step0, rotating in place as wfd_forward.
  * Records trace row n = p^{n+1} at `rec` (as wfd_forward) and row n = p^n at `strip` for
  * n = step0 .. step0 + nsteps (the values the reconstruction of wfd_ac_backward restores).
- * Replaces the forward of cuwave.wave.simulate / cuwave.utils.response for AcousticWave
- * (topopt_acoustic2D.py:179-181; projects/1_benchmark_2D/forward/wave.py:126). */
+ * Replaces the forward of cuwave.wave.simulate / cuwave.utils.response / measure for
+ * AcousticWave and a density indicator (topopt_acoustic2D.py:179-181;
+ * projects/1_benchmark_2D/forward/wave.py:126; projects/18_fwi/plate.py:62-64). */
 int wfd_ac_forward(const wfd_grid* g, void* u_a, void* u_b, const wfd_ac_medium* med,
                    const wfd_points* src, const wfd_points* rec, const wfd_points* strip,
                    int64_t step0, int64_t nsteps, wfd_stream_t stream);
@@ -341,17 +348,19 @@ int wfd_ac_forward(const wfd_grid* g, void* u_a, void* u_b, const wfd_ac_medium*
 /* Adjoint of an nsteps forward from rest with boundary reconstruction: u_a = p^{N-1},
  * u_b = p^N as the forward left them (consumed).  The forward field is rebuilt backwards in
  * time on nodes without sponge damping; `strip` (its data as wfd_ac_forward wrote it) restores
- * the damped nodes next to them.  `adj` injects row n (= -dJ/dp^n at the receivers, rows 0..N)
- * into the adjoint field mu = k lambda; `mu_src` (data rows 0..N-1) receives mu^{n+1} at the
- * sources.  mu_a, mu_b are work fields, g_work 3 fields per shot of the grid dtype, and on
- * return G (float64, 3 fields per shot: [3][nshots][elems]) holds per shot
- *     G[0] = sum_n mu^{n+1} q(p^n)                on undamped interior nodes,
- *     G[1] = sum_n (dp^n)(dmu^{n+1}) of each node's north face,  G[2] the same of west faces,
- * faces summed when either end is an undamped interior node (the far faces of the last interior
- * row / column are stored at the boundary node beyond them).  With J(traces):
- *     dJ/dk = -(G[0] + sum_n mu^{n+1} sig^n) / k,   dJ/dcn = G[1],   dJ/dcw = G[2].
+ * the damped nodes within K nodes of them.  `adj` injects row n (= -dJ/dp^n at the receivers,
+ * rows 0..N) into the adjoint field mu = k lambda; `mu_src` (data rows 0..N-1) receives mu^{n+1}
+ * at the sources.  mu_a, mu_b are work fields, g_work 2K+1 fields per shot of the grid dtype,
+ * and on return G (float64, [2K+1][nshots][elems]) holds per shot
+ *     G[0]   = sum_n mu^{n+1} q(p^n)                  on undamped interior nodes,
+ *     G[m]   = sum_n (dp^n)(dmu^{n+1}) over each node's bond to the node m rows above,
+ *     G[K+m] = the same over bonds to the node m columns left,
+ * bonds summed when either end is an undamped interior node (bonds from the last interior rows /
+ * columns to nodes past them are stored at that far node).  With J(traces):
+ *     dJ/dk = -(G[0] + sum_n mu^{n+1} sig^n) / k,   dJ/dcn_m = G[m],   dJ/dcw_m = G[K+m].
  * Replaces cuwave.sensitivity.reconstruction_sensitivity (analog_rnn_train.py:142-146;
- * strip footprint (N+2) x strip x itemsize, analog_rnn_fixture.py:84-87). */
+ * strip footprint (N+2) x strip x itemsize, analog_rnn_fixture.py:84-87) and the adjoint of
+ * cuwave.utils.misfit_gradient for a density indicator (projects/18_fwi/2D_fwi.py:40). */
 int wfd_ac_backward(const wfd_grid* g, void* u_a, void* u_b, void* mu_a, void* mu_b,
                     const wfd_ac_medium* med, const wfd_points* src, const wfd_points* adj,
                     const wfd_points* strip, const wfd_points* mu_src, int64_t nsteps,

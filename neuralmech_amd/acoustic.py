@@ -32,11 +32,13 @@ Model and discretisation (cuwave's source is not available, so these are this bu
 from __future__ import annotations
 
 import ctypes
+import math
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
 import numpy as np
 import torch
+import torch.nn.functional as F
 
 from . import _lib, solver
 from ._lib import check
@@ -60,51 +62,103 @@ def volume_weights(shape, closure: str) -> np.ndarray:
     return w
 
 
-def _face_areas(shape, closure: str):
-    """Relative face areas: cn[i, j] couples (i-1, j)-(i, j), cw[i, j] couples (i, j-1)-(i, j).
+def central_weights(order: int) -> list:
+    """``a_1 .. a_K`` of the central second difference of ``order`` (closed form,
+    ``a_k = 2 (-1)^(k+1) (K!)^2 / (k^2 (K-k)! (K+k)!)``)."""
+    K = order // 2
+    return [2.0 * (-1) ** (k + 1) * math.factorial(K) ** 2 / (k * k * math.factorial(K - k) * math.factorial(K + k))
+            for k in range(1, K + 1)]
 
-    No-flux closure: faces with both ends interior; a face running along a wall (both ends on the
-    first / last interior row or column) has half the area, as the control volumes it separates
-    are halved there.  Zero-ring closure: every face with at least one interior end, full area.
+
+def _harm(a, b):
+    return 2.0 * a * b / (a + b)
+
+
+def _axis_bonds(s, h: float, order: int, closure: str):
+    """Bond coefficients along axis 0 of the node array ``s``: ``K`` arrays of shape
+    ``(n0 + K - 1, n1)``, entry ``[m-1][i, j]`` the bond between (i-m, j) and (i, j).
+
+    Zero ring: every bond with an interior end, the stiffness beyond the ring continued from it.
+    No flux: bonds between interior nodes, plus the bonds the even reflection of the field about the
+    first / last interior node adds near the walls (a node's bond to a mirror image of another node
+    becomes a bond to that node; bonds to its own image drop out), so that the operator is the
+    unreflected stencil applied to the even extension.
     """
-    n0, n1 = shape
-    an = np.zeros(shape)
-    aw = np.zeros(shape)
-    if closure == "neumann":
-        an[2:n0 - 1, 1:n1 - 1] = 1.0
-        aw[1:n0 - 1, 2:n1 - 1] = 1.0
-        an[:, 1] *= 0.5
-        an[:, n1 - 2] *= 0.5
-        aw[1, :] *= 0.5
-        aw[n0 - 2, :] *= 0.5
-    else:
-        an[1:n0, 1:n1 - 1] = 1.0
-        aw[1:n0 - 1, 1:n1] = 1.0
-    return an, aw
+    K = order // 2
+    a = central_weights(order)
+    n0, n1 = s.shape
+    E0 = n0 + K - 1
+    cols = slice(1, n1 - 1)
+    out = []
+    if closure == "dirichlet":
+        sp = torch.cat([s[:1].expand(K, n1), s, s[-1:].expand(K, n1)], 0)  # row i at i + K
+        for m in range(1, K + 1):
+            i = torch.arange(1, n0 - 1 + m, device=s.device)
+            v = a[m - 1] / (h * h) * _harm(sp[i - m + K][:, cols], sp[i + K][:, cols])
+            f = torch.zeros((E0, n1), dtype=s.dtype, device=s.device)
+            out.append(f.index_put((i[:, None], torch.arange(1, n1 - 1, device=s.device)[None, :]), v))
+        return out
+    L = n0 - 2  # last interior row; the first is 1
+    entries = {m: [] for m in range(1, K + 1)}  # distance -> [(row of the southern end, northern row, weight)]
+    for m in range(1, K + 1):
+        for i in range(1 + m, L + 1):
+            entries[m].append((i, i - m, a[m - 1]))
+    for i in range(2, min(K, L) + 1):  # near the first interior row: images 2 - (i - m)
+        for m in range(i, K + 1):
+            im = 2 - i + m
+            if im > i and im <= L:
+                entries[im - i].append((im, i, a[m - 1]))
+    for i in range(max(L - K + 1, 1), L):  # near the last interior row: images 2L - (i + m)
+        for m in range(L + 1 - i, K + 1):
+            im = 2 * L - i - m
+            if 1 <= im < i:
+                entries[i - im].append((i, im, a[m - 1]))
+    for d in range(1, K + 1):
+        f = torch.zeros((E0, n1), dtype=s.dtype, device=s.device)
+        if entries[d]:
+            rs = torch.as_tensor([e[0] for e in entries[d]], device=s.device)
+            rn = torch.as_tensor([e[1] for e in entries[d]], device=s.device)
+            w = torch.as_tensor([e[2] for e in entries[d]], dtype=s.dtype, device=s.device)[:, None]
+            v = w / (h * h) * _harm(s[rn][:, cols], s[rs][:, cols])
+            f = f.index_put((rs[:, None], torch.arange(1, n1 - 1, device=s.device)[None, :]), v, accumulate=True)
+        out.append(f)
+    return out
 
 
-def face_operator(mass: torch.Tensor, stiff: torch.Tensor, spacing, dt: float, closure: str):
-    """``(k, cn, cw)`` node / face arrays (float64, differentiable in ``mass`` and ``stiff``)."""
+def face_operator(mass: torch.Tensor, stiff: torch.Tensor, spacing, dt: float, closure: str, order: int = 2):
+    """``(k, cn, cw)``: ``k`` on the nodes and the bond coefficients ``(K, n0+K-1, n1+K-1)``
+    (float64, differentiable in ``mass`` and ``stiff``); K = order / 2.
+
+    The bond between nodes x and y at distance m along an axis is ``a_m hm(s_x, s_y) / h^2`` times
+    its relative area (the harmonic mean ``hm`` is the reciprocal of the mean density); with
+    constant stiffness the operator is the central stencil of ``order``.  Under the no-flux closure
+    a bond along a wall (both ends on the first / last interior row or column) has half the area and
+    ``k = w dt^2 / m`` with ``w`` = 2 per axis on those rows / columns (half control volumes)."""
     if closure not in CLOSURES:
         raise ValueError(f"closure must be one of {CLOSURES}")
+    if order not in (2, 4, 6, 8):
+        raise ValueError("order must be 2, 4, 6 or 8")
+    K = order // 2
     shape = tuple(mass.shape)
+    n0, n1 = shape
     dev = mass.device
     inter = torch.zeros(shape, dtype=torch.bool, device=dev)
     inter[1:-1, 1:-1] = True
     w = torch.as_tensor(volume_weights(shape, closure), device=dev)
     k = torch.where(inter, dt * dt * w / mass, torch.zeros((), dtype=mass.dtype, device=dev))
     h0, h1 = (float(h) for h in spacing)
-
-    def harm(a, b):
-        return 2.0 * a * b / (a + b)
-
-    zr = torch.zeros((1, shape[1]), dtype=stiff.dtype, device=dev)
-    zc = torch.zeros((shape[0], 1), dtype=stiff.dtype, device=dev)
-    cn = torch.cat([zr, harm(stiff[:-1, :], stiff[1:, :]) / (h0 * h0)], 0)
-    cw = torch.cat([zc, harm(stiff[:, :-1], stiff[:, 1:]) / (h1 * h1)], 1)
-    an, aw = (torch.as_tensor(a, dtype=stiff.dtype, device=dev) for a in _face_areas(shape, closure))
-    zero = torch.zeros((), dtype=stiff.dtype, device=dev)
-    return k, torch.where(an > 0, cn * an, zero), torch.where(aw > 0, cw * aw, zero)
+    bn = _axis_bonds(stiff, h0, order, closure)        # (n0+K-1, n1) each
+    bw = _axis_bonds(stiff.t(), h1, order, closure)    # (n1+K-1, n0) each, transposed
+    if closure == "neumann":
+        an = torch.ones(n1, dtype=stiff.dtype, device=dev)
+        an[1] = an[n1 - 2] = 0.5
+        aw = torch.ones(n0, dtype=stiff.dtype, device=dev)
+        aw[1] = aw[n0 - 2] = 0.5
+        bn = [f * an[None, :] for f in bn]
+        bw = [f * aw[None, :] for f in bw]
+    cn = torch.stack([F.pad(f, (0, K - 1, 0, 0)) for f in bn])
+    cw = torch.stack([F.pad(f.t(), (0, 0, 0, K - 1)) for f in bw])
+    return k, cn, cw
 
 
 def _profiles(grid: WaveGrid, beta_profiles):
@@ -125,29 +179,44 @@ def _profiles(grid: WaveGrid, beta_profiles):
     return out
 
 
+def ext_view(grid: WaveGrid, field: torch.Tensor, K: int) -> torch.Tensor:
+    """``(b, n0+K-1, n1+K-1)`` view of layout tensors ``(b, elems)``: the nodes plus the K-1 rows and
+    columns past the last node that hold bonds reaching beyond the last interior row / column."""
+    b = field.shape[0]
+    n0, n1 = grid.shape
+    o = grid.offset
+    rows = n0 + K - 1
+    return field[:, o:o + rows * grid.ld].view(b, rows, grid.ld)[:, :, : n1 + K - 1]
+
+
 class AcMedium:
-    """Coefficient fields of the variable mass / stiffness model on a 2D order-2 grid.
+    """Coefficient fields of the variable mass / stiffness model on a 2D grid (order 2..8).
 
     ``mass`` / ``stiff`` are node arrays (or numbers) shared by all shots; ``closure`` is
     ``"neumann"`` or ``"dirichlet"``; ``beta_profiles`` one damping profile per axis.
     """
 
     def __init__(self, grid: WaveGrid, mass, stiff, closure: str = "neumann", beta_profiles=None):
-        if grid.ndim != 2 or grid.order != 2:
-            raise ValueError("the variable mass / stiffness model runs on 2D grids of order 2")
+        if grid.ndim != 2:
+            raise ValueError("the variable mass / stiffness model runs on 2D grids")
         if closure not in CLOSURES:
             raise ValueError(f"closure must be one of {CLOSURES}")
-        if closure == "neumann" and min(grid.shape) < 4:
-            raise ValueError("the no-flux closure needs at least 4 nodes per axis")
-        self.grid, self.closure = grid, closure
+        K = grid.order // 2
+        if closure == "neumann" and min(grid.shape) < 2 * K + 3:
+            raise ValueError(f"the no-flux closure of order {grid.order} needs at least {2 * K + 3} nodes per axis")
+        self.grid, self.closure, self.K = grid, closure, K
         dev = grid.device
         self.mass = torch.as_tensor(mass, dtype=torch.float64).to(dev).expand(grid.shape).clone()
         self.stiff = torch.as_tensor(stiff, dtype=torch.float64).to(dev).expand(grid.shape).clone()
         for name, a in (("mass", self.mass), ("stiffness", self.stiff)):
             if not bool(torch.isfinite(a).all()) or not bool((a > 0).all()):
                 raise ValueError(f"{name} must be finite and positive on every node")
-        k, cn, cw = face_operator(self.mass, self.stiff, grid.spacing, grid.dt, closure)
-        self.k, self.cn, self.cw = (grid.from_nodes(a, nshots=1, dtype=grid.dtype) for a in (k, cn, cw))
+        k, cn, cw = face_operator(self.mass, self.stiff, grid.spacing, grid.dt, closure, grid.order)
+        self.k = grid.from_nodes(k, nshots=1, dtype=grid.dtype)
+        self.cn = grid.zeros(nshots=K)
+        self.cw = grid.zeros(nshots=K)
+        ext_view(grid, self.cn, K).copy_(cn)
+        ext_view(grid, self.cw, K).copy_(cw)
         prof = _profiles(grid, beta_profiles)
         self.profiles = prof
         self.beta = None
@@ -163,28 +232,36 @@ class AcMedium:
         self.c.k, self.c.cn, self.c.cw = self.k.data_ptr(), self.cn.data_ptr(), self.cw.data_ptr()
         self.c.beta = None if self.beta is None else self.beta.data_ptr()
 
+    def bonds(self):
+        """``(k, cn, cw)`` as the kernels use them (grid dtype): node array and ``(K, n0+K-1, n1+K-1)``."""
+        g = self.grid
+        return g.nodes(self.k)[0], ext_view(g, self.cn, self.K), ext_view(g, self.cw, self.K)
+
     def strip_nodes(self) -> np.ndarray:
-        """Damped interior nodes with an undamped face neighbour, ``(S, 2)`` in C order."""
-        und = self.undamped
-        nb = np.zeros_like(und)
-        nb[1:, :] |= und[:-1, :]
-        nb[:-1, :] |= und[1:, :]
-        nb[:, 1:] |= und[:, :-1]
-        nb[:, :-1] |= und[:, 1:]
-        inter = np.zeros_like(und)
-        inter[1:-1, 1:-1] = True
-        return np.argwhere(inter & ~und & nb)
+        """Damped interior nodes within K nodes along an axis of an undamped one, ``(S, 2)``, C order."""
+        return _strip(self.undamped, self.K)
 
     def check_step(self) -> float:
         """Gershgorin bound of ``dt^2 lambda_max`` of the update (stable below 4)."""
-        k = self.grid.nodes(self.k)[0].double()
-        cn = self.grid.nodes(self.cn)[0].double()
-        cw = self.grid.nodes(self.cw)[0].double()
-        row = cn.clone()
-        row[:-1, :] += cn[1:, :]
-        row += cw
-        row[:, :-1] += cw[:, 1:]
+        k, cn, cw = (a.double() for a in self.bonds())
+        n0, n1 = self.grid.shape
+        row = torch.zeros_like(k)
+        for m in range(1, self.K + 1):
+            row += cn[m - 1][:n0, :n1] + cn[m - 1][m:m + n0, :n1]
+            row += cw[m - 1][:n0, :n1] + cw[m - 1][:n0, m:m + n1]
         return float((2.0 * k * row).max())
+
+
+def _strip(und: np.ndarray, K: int) -> np.ndarray:
+    nb = np.zeros_like(und)
+    for m in range(1, K + 1):
+        nb[m:, :] |= und[:-m, :]
+        nb[:-m, :] |= und[m:, :]
+        nb[:, m:] |= und[:, :-m]
+        nb[:, :-m] |= und[:, m:]
+    inter = np.zeros_like(und)
+    inter[1:-1, 1:-1] = True
+    return np.argwhere(inter & ~und & nb)
 
 
 # ------------------------------------------------------------------------- runs
@@ -223,8 +300,9 @@ def backward(grid: WaveGrid, med: AcMedium, nsteps: int, state, src: Optional[Po
     """Adjoint sweep with reconstruction for ``J(traces)`` with ``dJ/dtraces = dJdtr``
     ``(nshots, nsteps, R)``; ``state = (p^{N-1}, p^N)`` from :func:`forward` (consumed).
 
-    Returns ``(G, mu_src)``: ``G`` float64 ``(3, nshots, n0, n1)`` (mass term, north faces, west
-    faces; see ``wfd_ac_backward``) and ``mu_src`` ``(nshots, nsteps, S)`` or None.
+    Returns ``(G, mu_src)``: ``G`` float64 ``(2K+1, nshots, n0+K-1, n1+K-1)`` (mass term on the nodes,
+    then the north and the west bond sums per distance; see ``wfd_ac_backward``) and ``mu_src``
+    ``(nshots, nsteps, S)`` or None.
     """
     lib = _lib.load()
     g = grid
@@ -244,23 +322,26 @@ def backward(grid: WaveGrid, med: AcMedium, nsteps: int, state, src: Optional[Po
         raise ValueError("the strip needs the values the forward recorded")
     tp = strip.struct(store.contiguous()) if strip is not None and strip.count else None
     mu_a, mu_b = g.zeros(), g.zeros()
-    work = torch.zeros(3 * g.nshots * g.elems, dtype=g.dtype, device=g.device)
-    G = torch.zeros(3, g.nshots, g.elems, dtype=torch.float64, device=g.device)
+    nf = 2 * med.K + 1
+    work = torch.zeros(nf * g.nshots * g.elems, dtype=g.dtype, device=g.device)
+    G = torch.zeros(nf, g.nshots, g.elems, dtype=torch.float64, device=g.device)
     ua, ub = state
     check(lib.wfd_ac_backward(ctypes.byref(g.c), _ptr(ua), _ptr(ub), _ptr(mu_a), _ptr(mu_b), ctypes.byref(med.c),
                               ctypes.byref(sp) if sp else None, ctypes.byref(ap) if ap else None,
                               ctypes.byref(tp) if tp else None, ctypes.byref(mp) if mp else None, int(nsteps),
                               _ptr(work), _ptr(G), _stream_ptr(g.device)))
-    Gn = torch.stack([g.nodes(G[a]) for a in range(3)])
-    return Gn, mu_src
+    return torch.stack([ext_view(g, G[a], med.K) for a in range(nf)]), mu_src
 
 
 def material_gradient(med: AcMedium, G: torch.Tensor, mu_src, sig, src: Optional[PointSet]):
-    """``(dJ/dmass, dJ/dstiff)`` per shot ``(nshots, n0, n1)`` float64 from the sweep's sums:
-    ``dJ/dk = -(G[0] + sum_n mu^{n+1} sig^n) / k``, ``dJ/dcn = G[1]``, ``dJ/dcw = G[2]``, carried to
-    mass and stiffness through the coefficient map; zero off the undamped interior."""
+    """``(dJ/dmass, dJ/dstiff)`` per shot ``(nshots, n0, n1)`` float64 from the sweep's sums ``G``
+    ``(2K+1, nshots, n0+K-1, n1+K-1)``: ``dJ/dk = -(G[0] + sum_n mu^{n+1} sig^n) / k``,
+    ``dJ/dcn_m = G[m]``, ``dJ/dcw_m = G[K+m]``, carried to mass and stiffness through the
+    coefficient map; zero off the undamped interior."""
     g = med.grid
-    Gm = G[0].clone()
+    K = med.K
+    n0, n1 = g.shape
+    Gm = G[0][:, :n0, :n1].clone()
     if mu_src is not None:
         s_sig = solver._sig_tensor(g, src, sig, mu_src.shape[1])
         contrib = (mu_src.double() * s_sig.double()[:, :, : src.count]).sum(1)
@@ -272,11 +353,13 @@ def material_gradient(med: AcMedium, G: torch.Tensor, mu_src, sig, src: Optional
     with torch.enable_grad():  # also inside an autograd.Function's backward
         m = med.mass.detach().clone().requires_grad_(True)
         st = med.stiff.detach().clone().requires_grad_(True)
-        k, cn, cw = face_operator(m, st, g.spacing, g.dt, med.closure)
+        k, cn, cw = face_operator(m, st, g.spacing, g.dt, med.closure, g.order)
         kd = k.detach()
         for s in range(g.nshots):
             dk = torch.where(und, -Gm[s] / torch.where(und, kd, torch.ones_like(kd)), torch.zeros_like(kd))
-            gm_, gs_ = torch.autograd.grad([k, cn, cw], [m, st], [dk, G[1][s], G[2][s]], retain_graph=True)
+            gn = torch.stack([G[a][s] for a in range(1, K + 1)])
+            gw = torch.stack([G[a][s] for a in range(K + 1, 2 * K + 1)])
+            gm_, gs_ = torch.autograd.grad([k, cn, cw], [m, st], [dk, gn, gw], retain_graph=True)
             dm.append(torch.where(und, gm_, torch.zeros_like(gm_)))
             ds.append(torch.where(und, gs_, torch.zeros_like(gs_)))
     return torch.stack(dm), torch.stack(ds)
@@ -342,14 +425,14 @@ class AcousticWave:
             raise ValueError("AcousticWave is two-dimensional")
         if self.precision not in _PRECISIONS:
             raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}")
-        if self.space_order != 2:
-            raise NotImplementedError("AcousticWave is implemented for space_order = 2")
+        if self.space_order not in (2, 4, 6, 8):
+            raise ValueError("space_order must be 2, 4, 6 or 8")
         if self.boundary not in CLOSURES:
             raise ValueError(f"boundary must be one of {CLOSURES}")
         for name in ("rho1", "rho2", "kappa1", "kappa2"):
             if not float(getattr(self, name)) > 0:
                 raise ValueError(f"{name} must be > 0")
-        limit = solver.stable_dt(self.dx, self.max_wavespeed, 2)
+        limit = solver.stable_dt(self.dx, self.max_wavespeed, self.space_order)
         if self.dt > limit * (1 + 1e-12):
             raise ValueError(f"dt = {self.dt:.6g} exceeds the stable step {limit:.6g} for wave speed "
                              f"{self.max_wavespeed:.6g}")
@@ -395,7 +478,7 @@ class AcousticWave:
                 torch.full_like(g, 1.0 / self.rho2 - 1.0 / self.rho1))
 
     def grid(self, nshots: int = 1) -> WaveGrid:
-        return WaveGrid(self.Nx, self.dx, self.dt, 2, self.dtype, nshots=nshots, device=self.device)
+        return WaveGrid(self.Nx, self.dx, self.dt, self.space_order, self.dtype, nshots=nshots, device=self.device)
 
 
 def _node_list(nodes):
@@ -443,8 +526,8 @@ def _source_setup(grid, source, nsteps, scale=None):
 
 
 def _material(sim, gamma):
-    if not isinstance(sim, AcousticWave) and (len(sim.Nx) != 2 or sim.space_order != 2):
-        raise NotImplementedError("the density indicator is implemented for 2D grids of space_order 2")
+    if len(sim.Nx) != 2:
+        raise NotImplementedError("the density indicator is implemented for 2D grids")
     return sim.material(gamma)
 
 
@@ -549,14 +632,7 @@ def reconstruction_nodes(sim):
     else:
         p0, p1 = (np.asarray(p, np.float64).astype(npdt) for p in prof)
         und[1:-1, 1:-1] = (p0[:, None] + p1[None, :])[1:-1, 1:-1] == 0
-    nb = np.zeros_like(und)
-    nb[1:, :] |= und[:-1, :]
-    nb[:-1, :] |= und[1:, :]
-    nb[:, 1:] |= und[:, :-1]
-    nb[:, :-1] |= und[:, 1:]
-    inter = np.zeros_like(und)
-    inter[1:-1, 1:-1] = True
-    return np.argwhere(inter & ~und & nb).T.copy(), und
+    return _strip(und, sim.space_order // 2).T.copy(), und
 
 
 def reconstruction_sensitivity(sim, source, gamma, nodes, objective):

@@ -1,23 +1,22 @@
-// Variable mass / stiffness wave on the 2D halo layout (order 2) and its adjoint with boundary
-// reconstruction of the forward field.
+// Variable mass / stiffness wave on the 2D halo layout (orders 2..8) and its adjoint with
+// boundary reconstruction of the forward field.
 //
-// Forward (per interior node x, step n):
-//     q     = (cn[x] (p[N] - p[x]) + cn[S] (p[S] - p[x])) + (cw[x] (p[W] - p[x]) + cw[E] (p[E] - p[x]))
+// Forward (per interior node x, step n; K = order / 2 bonds per side and axis):
+//     q     = sum over rows then columns of sum_m (c_m,near d_m + c_m,far d_m)   (see wavefd.h)
 //     t     = 2 p[x] - (1 - b) p_prev[x]               (2 p[x] - p_prev[x] without sponge)
 //     num   = t + k[x] q;  for each source s at x (list order): num = num + k[x] sig[n][s]
 //     p_next[x] = num / (1 + b)                        (num without sponge)
-// N/S/W/E are the four neighbours; cn[y] is the coefficient of the face between y and its north
-// neighbour, cw[y] of the face between y and its west neighbour, so the face operator C is
-// symmetric: sum_x mu_x (C p)_x = -sum_faces c_f (dp)_f (dmu)_f.  Mass, volume weights, spacing,
-// closure and the averaging of the stiffness onto faces are all in k and c (set by the caller).
+// The bond operator C is symmetric: sum_x mu_x (C p)_x = -sum_bonds c (dp)(dmu).  Mass,
+// volume weights, spacing, closure and the averaging of the stiffness onto bonds are all in k
+// and c (set by the caller).
 //
 // Backward step n (n = N .. 1), from (p^{n+1}, p^n) and (mu^{n+2}, mu^{n+1}):
 //     mu^n      = the forward update of mu with the adjoint sources adj[n] (= -dJ/dp^n)
 //     p^{n-1}   = (2 p^n + k q(p^n)) + k sig[n] - p^{n+1}       on undamped interior nodes
 //     Gm       += mu^{n+1} q(p^n)                                on undamped interior nodes
-//     F_face   += (dp^n)_f (dmu^{n+1})_f                         on faces with an undamped end
-// with p^{n-1} on the damped nodes next to undamped ones ("strip") restored from the values the
-// forward recorded.  Then dJ/dk = -(Gm + sum_n mu^{n+1} sig^n) / k and dJ/dc_f = F_f.
+//     F_bond   += (dp^n)_bond (dmu^{n+1})_bond                   on bonds with an undamped end
+// with p^{n-1} on the damped nodes within K of undamped ones ("strip") restored from the values
+// the forward recorded.  Then dJ/dk = -(Gm + sum_n mu^{n+1} sig^n) / k and dJ/dc = F.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -38,21 +37,20 @@ struct AcArgs {
     const T* mp;  // backward: mu^{n+2} (overwritten in place by mu^n)
     const T* mc;  // backward: mu^{n+1}
     T* mn;
-    const T* k;  // node (0,0) of the shared coefficient fields
-    const T* cn;
-    const T* cw;
+    const T* k;   // node (0,0) of the shared coefficient fields
+    const T* cn;  // K fields, cstride apart: bond to the north neighbour at distance m
+    const T* cw;  // K fields: bond to the west neighbour at distance m
     const T* br;  // sponge profile of the row axis (n0 entries) and column axis (n1), or null
     const T* bc;
-    T* gm;  // imaging partials (backward): mass term, north faces, west faces
-    T* fn;
-    T* fw;
-    double* Gm;  // float64 sums of the partials
-    double* Fn;
-    double* Fw;
+    T* gm;       // imaging partials (backward): mass term, then K north-bond and K west-bond fields
+    double* Gm;  // float64 sums of the partials, same layout
     int32_t flush;  // add the partials to the sums and store zero partials
     int32_t recon;  // backward: reconstruct p^{n-1} and image (0 on the first backward step)
     int64_t shot_stride;
+    int64_t cstride;  // elements between the K bond-coefficient fields (one field)
+    int64_t pstride;  // elements between the 2K + 1 imaging fields (nshots fields)
     int64_t n0, n1, ld;
+    int32_t K;        // bonds per side and axis (order / 2)
     PtsDev<T> src;  // injected into p (forward and reconstruction), row `step`
     PtsDev<T> adj;  // injected into mu (backward), row `step`
     PtsDev<T> ga;   // gather: field ga_f at ga's points -> row step + ga_off
@@ -139,9 +137,39 @@ __device__ __forceinline__ void acc(T* part, double* sum, int64_t o, T add, bool
     }
 }
 
+// Column neighbours at distance 1..K of this lane's node in row offset `o`: from lanes l -+ m,
+// loaded directly by the lanes whose neighbour lies in another wave's columns.  Every lane of
+// the wave takes part in the shuffles.
+template <typename T, int K>
+__device__ __forceinline__ void ac_cols(const T* f, int64_t o, int lane, bool in, T c, T (&w)[K], T (&e)[K])
+{
+#pragma unroll
+    for (int m = 1; m <= K; m++) {
+        w[m - 1] = __shfl_up(c, m);
+        e[m - 1] = __shfl_down(c, m);
+        if (lane < m && in) w[m - 1] = f[o - m];
+        if (lane >= 64 - m && in) e[m - 1] = f[o + m];
+    }
+}
+
+// q = sum over the two axes of sum_m (c_m(north/west) d_m + c_m(south/east) d_m), accumulated
+// m = 1..K per axis, rows before columns (for K = 1: (cN dN + cS dS) + (cW dW + cE dE)).
+template <typename T, int K>
+__device__ __forceinline__ T ac_q(const T (&cn)[K], const T (&cs)[K], const T (&cw)[K], const T (&ce)[K],
+                                  const T (&dn)[K], const T (&ds)[K], const T (&dw)[K], const T (&de)[K])
+{
+    T qx = cn[0] * dn[0] + cs[0] * ds[0];
+#pragma unroll
+    for (int m = 1; m < K; m++) qx = qx + (cn[m] * dn[m] + cs[m] * ds[m]);
+    T qy = cw[0] * dw[0] + ce[0] * de[0];
+#pragma unroll
+    for (int m = 1; m < K; m++) qy = qy + (cw[m] * dw[m] + ce[m] * de[m]);
+    return qx + qy;
+}
+
 // Launch: 8 * ceil(aux / 8) aux blocks then gx * gy * nshots tiles (see step_block); a tile is 64
-// columns x 4 waves x AC_RI rows.
-template <typename T, bool SPONGE, bool BWD>
+// columns x 4 waves x AC_RI rows.  K = order / 2 bonds per side and axis.
+template <typename T, int K, bool SPONGE, bool BWD>
 __global__ __launch_bounds__(256) void ac_kernel(const AcArgs<T> a)
 {
     const StepBlock bp = step_block(a.nga + a.ngb + a.nsc, a.gx, a.gy);
@@ -156,7 +184,7 @@ __global__ __launch_bounds__(256) void ac_kernel(const AcArgs<T> a)
     }
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t n0 = a.n0, n1 = a.n1, ld = a.ld;
+    const int64_t n0 = a.n0, n1 = a.n1, ld = a.ld, cst = a.cstride, pst = a.pstride;
     const int64_t shot = bp.z;
     const int64_t j0 = (int64_t)bp.x * 64, j = j0 + lane;
     const int64_t r0 = 1 + ((int64_t)bp.y * 4 + wave) * AC_RI;
@@ -166,93 +194,124 @@ __global__ __launch_bounds__(256) void ac_kernel(const AcArgs<T> a)
     const int64_t so = shot * a.shot_stride;
     const T* uc = a.uc + so;
     const T* mc = BWD ? a.mc + so : nullptr;
+    const T bcj = SPONGE && in ? a.bc[j] : T(0);
 
-    // column profile of this node and of its west neighbour
-    T bcj = T(0), bcw = T(0);
-    if (SPONGE) {
-        bcj = in ? a.bc[j] : T(0);
-        bcw = __shfl_up(bcj, 1);
-        if (lane == 0) bcw = j0 > 0 ? a.bc[j0 - 1] : T(0);
+    // rolling windows: rows r-K .. r+K of p (and mu), bond coefficients cn_m of rows r .. r+m
+    T P[2 * K + 1], M[2 * K + 1], CN[K][K + 1];
+#pragma unroll
+    for (int t = 0; t < 2 * K; t++) {
+        P[t] = ld0(uc, (r0 - K + t) * ld + j, in);
+        M[t] = BWD ? ld0(mc, (r0 - K + t) * ld + j, in) : T(0);
     }
-    // rolling rows: north neighbour, centre, north-face coefficient of the centre row
-    T pN = ld0(uc, (r0 - 1) * ld + j, in), pC = ld0(uc, r0 * ld + j, in);
-    T mN = T(0), mC = T(0);
-    if (BWD) {
-        mN = ld0(mc, (r0 - 1) * ld + j, in);
-        mC = ld0(mc, r0 * ld + j, in);
-    }
-    T cN = ld0(a.cn, r0 * ld + j, in);
+#pragma unroll
+    for (int m = 1; m <= K; m++)
+#pragma unroll
+        for (int t = 0; t < m; t++) CN[m - 1][t] = ld0(a.cn + (m - 1) * cst, (r0 + t) * ld + j, in);
 #pragma unroll
     for (int s = 0; s < AC_RI; s++) {
         const int64_t r = r0 + s;
         if (r > n0 - 2) break;  // whole wave
         const int64_t o = r * ld + j;
-        const T pS = ld0(uc, o + ld, in);
-        const T cS = ld0(a.cn, o + ld, in);
-        const T cW = ld0(a.cw, o, in);
-        T cE = __shfl_down(cW, 1);
-        if (lane == 63) cE = j0 + 64 < n1 ? a.cw[r * ld + j0 + 64] : T(0);
-        T pW = __shfl_up(pC, 1), pE = __shfl_down(pC, 1);
-        if (lane == 0) pW = j0 > 0 ? uc[r * ld + j0 - 1] : T(0);
-        if (lane == 63) pE = j0 + 64 < n1 ? uc[r * ld + j0 + 64] : T(0);
+        P[2 * K] = ld0(uc, o + K * ld, in);
+        if (BWD) M[2 * K] = ld0(mc, o + K * ld, in);
+#pragma unroll
+        for (int m = 1; m <= K; m++) CN[m - 1][m] = ld0(a.cn + (m - 1) * cst, o + m * ld, in);
+        const T pc = P[K];
+        T pw[K], pe[K], cwv[K], cev[K], cnv[K], csv[K], dN[K], dS[K], dW[K], dE[K];
+        ac_cols<T, K>(uc, o, lane, in, pc, pw, pe);
+#pragma unroll
+        for (int m = 1; m <= K; m++) {
+            const T* cwm = a.cw + (m - 1) * cst;
+            cwv[m - 1] = ld0(cwm, o, in);
+            cev[m - 1] = __shfl_down(cwv[m - 1], m);
+            if (lane >= 64 - m && in) cev[m - 1] = cwm[o + m];
+            cnv[m - 1] = CN[m - 1][0];
+            csv[m - 1] = CN[m - 1][m];
+            dN[m - 1] = P[K - m] - pc;
+            dS[m - 1] = P[K + m] - pc;
+            dW[m - 1] = pw[m - 1] - pc;
+            dE[m - 1] = pe[m - 1] - pc;
+        }
         const T kx = ld0(a.k, o, in);
-        const T dN = pN - pC, dS = pS - pC, dW = pW - pC, dE = pE - pC;
-        const T q = (cN * dN + cS * dS) + (cW * dW + cE * dE);
+        const T q = ac_q<T, K>(cnv, csv, cwv, cev, dN, dS, dW, dE);
         const T b = SPONGE ? a.br[r] + bcj : T(0);
         if (!BWD) {
             const T pp = ld0(a.up + so, o, in);
-            const T t = SPONGE ? T(2) * pC - (T(1) - b) * pp : T(2) * pC - pp;
+            const T t = SPONGE ? T(2) * pc - (T(1) - b) * pp : T(2) * pc - pp;
             T num = t + kx * q;
             num = ac_inject(a.src, shot, r, j0, lane, a.step, kx, num);
             if (inter) a.un[so + o] = SPONGE ? num / (T(1) + b) : num;
         } else {
-            const T mS = ld0(mc, o + ld, in);
-            T mW = __shfl_up(mC, 1), mE = __shfl_down(mC, 1);
-            if (lane == 0) mW = j0 > 0 ? mc[r * ld + j0 - 1] : T(0);
-            if (lane == 63) mE = j0 + 64 < n1 ? mc[r * ld + j0 + 64] : T(0);
-            const T eN = mN - mC, eS = mS - mC, eW = mW - mC, eE = mE - mC;
-            const T qm = (cN * eN + cS * eS) + (cW * eW + cE * eE);
+            const T mcv = M[K];
+            T mw[K], me[K], eN[K], eS[K], eW[K], eE[K];
+            ac_cols<T, K>(mc, o, lane, in, mcv, mw, me);
+#pragma unroll
+            for (int m = 1; m <= K; m++) {
+                eN[m - 1] = M[K - m] - mcv;
+                eS[m - 1] = M[K + m] - mcv;
+                eW[m - 1] = mw[m - 1] - mcv;
+                eE[m - 1] = me[m - 1] - mcv;
+            }
+            const T qm = ac_q<T, K>(cnv, csv, cwv, cev, eN, eS, eW, eE);
             const T mpv = ld0(a.mp + so, o, in);
-            const T tm = SPONGE ? T(2) * mC - (T(1) - b) * mpv : T(2) * mC - mpv;
+            const T tm = SPONGE ? T(2) * mcv - (T(1) - b) * mpv : T(2) * mcv - mpv;
             T numm = tm + kx * qm;
             numm = ac_inject(a.adj, shot, r, j0, lane, a.step, kx, numm);
             if (inter) a.mn[so + o] = SPONGE ? numm / (T(1) + b) : numm;
             if (a.recon) {
                 const bool und = !SPONGE || b == T(0);
-                T rv = T(2) * pC + kx * q;
+                T rv = T(2) * pc + kx * q;
                 rv = ac_inject(a.src, shot, r, j0, lane, a.step, kx, rv);
                 const bool fl = a.flush != 0;
                 if (inter && und) {
                     a.un[so + o] = rv - a.up[so + o];
-                    acc(a.gm + so, a.Gm + so, o, mC * q, fl);
+                    acc(a.gm + so, a.Gm + so, o, mcv * q, fl);
                 }
                 if (inter) {
-                    // north face: imaged when either end is an undamped interior node
-                    const bool nund = r - 1 >= 1 && (!SPONGE || a.br[r - 1] + bcj == T(0));
-                    if (und || nund) acc(a.fn + so, a.Fn + so, o, dN * eN, fl);
-                    const bool wund = j - 1 >= 1 && (!SPONGE || a.br[r] + bcw == T(0));
-                    if (und || wund) acc(a.fw + so, a.Fw + so, o, dW * eW, fl);
-                    // faces to the boundary ring on the far sides are stored at the ring node
-                    if (r == n0 - 2 && und) acc(a.fn + so, a.Fn + so, o + ld, dS * eS, fl);
-                    if (j == n1 - 2 && und) acc(a.fw + so, a.Fw + so, o + 1, dE * eE, fl);
+#pragma unroll
+                    for (int m = 1; m <= K; m++) {
+                        T* fn = a.gm + m * pst + so;
+                        double* Fn = a.Gm + m * pst + so;
+                        T* fw = a.gm + (K + m) * pst + so;
+                        double* Fw = a.Gm + (K + m) * pst + so;
+                        // a bond is imaged when either end is an undamped interior node
+                        const bool nund = r - m >= 1 && (!SPONGE || a.br[r - m] + bcj == T(0));
+                        if (und || nund) acc(fn, Fn, o, dN[m - 1] * eN[m - 1], fl);
+                        const bool wund = j - m >= 1 && (!SPONGE || a.br[r] + a.bc[j - m] == T(0));
+                        if (und || wund) acc(fw, Fw, o, dW[m - 1] * eW[m - 1], fl);
+                        // bonds reaching past the last interior row / column: stored at their far end
+                        if (und && r + m >= n0 - 1) acc(fn, Fn, o + m * ld, dS[m - 1] * eS[m - 1], fl);
+                        if (und && j + m >= n1 - 1) acc(fw, Fw, o + m, dE[m - 1] * eE[m - 1], fl);
+                    }
                 }
             }
-            mN = mC;
-            mC = mS;
         }
-        pN = pC;
-        pC = pS;
-        cN = cS;
+#pragma unroll
+        for (int t = 0; t < 2 * K; t++) {
+            P[t] = P[t + 1];
+            M[t] = M[t + 1];
+        }
+#pragma unroll
+        for (int m = 1; m <= K; m++)
+#pragma unroll
+            for (int t = 0; t < m; t++) CN[m - 1][t] = CN[m - 1][t + 1];
     }
 }
 
-template <typename T, bool SPONGE, bool BWD>
+template <typename T, int K, bool SPONGE, bool BWD>
 hipError_t ac_launch_k(const AcArgs<T>& a, uint32_t tiles, hipStream_t s)
 {
     const uint32_t blocks = step_grid(tiles, a.nga + a.ngb + a.nsc);
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((ac_kernel<T, SPONGE, BWD>), dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((ac_kernel<T, K, SPONGE, BWD>), dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
+}
+
+template <typename T, int K>
+hipError_t ac_launch_kk(const AcArgs<T>& a, uint32_t tiles, bool sp, bool bwd, hipStream_t s)
+{
+    if (bwd) return sp ? ac_launch_k<T, K, true, true>(a, tiles, s) : ac_launch_k<T, K, false, true>(a, tiles, s);
+    return sp ? ac_launch_k<T, K, true, false>(a, tiles, s) : ac_launch_k<T, K, false, false>(a, tiles, s);
 }
 
 template <typename T>
@@ -268,8 +327,12 @@ hipError_t ac_launch(AcArgs<T> a, bool tiles_on, bool bwd, hipStream_t s)
     a.gy = (uint32_t)((a.n0 - 2 + 4 * AC_RI - 1) / (4 * AC_RI));
     const uint32_t tiles = tiles_on ? a.gx * a.gy * (uint32_t)a.nshots : 0u;
     const bool sp = a.br != nullptr;
-    if (bwd) return sp ? ac_launch_k<T, true, true>(a, tiles, s) : ac_launch_k<T, false, true>(a, tiles, s);
-    return sp ? ac_launch_k<T, true, false>(a, tiles, s) : ac_launch_k<T, false, false>(a, tiles, s);
+    switch (a.K) {
+        case 1: return ac_launch_kk<T, 1>(a, tiles, sp, bwd, s);
+        case 2: return ac_launch_kk<T, 2>(a, tiles, sp, bwd, s);
+        case 3: return ac_launch_kk<T, 3>(a, tiles, sp, bwd, s);
+        default: return ac_launch_kk<T, 4>(a, tiles, sp, bwd, s);
+    }
 }
 
 // ------------------------------------------------------------------------------ host side
@@ -301,7 +364,9 @@ int ac_pts(PtsDev<T>& p, const wfd_points* src, int32_t nrows, int nshots, const
 int ac_check(const wfd_grid* g, const wfd_ac_medium* med)
 {
     if (!g || !med) return ac_fail(WFD_EINVAL, "wfd_ac: NULL grid or medium");
-    if (g->ndim != 2 || g->order != 2) return ac_fail(WFD_EINVAL, "wfd_ac: 2D grids of order 2 only");
+    if (g->ndim != 2) return ac_fail(WFD_EINVAL, "wfd_ac: 2D grids only");
+    if (g->order != 2 && g->order != 4 && g->order != 6 && g->order != 8)
+        return ac_fail(WFD_EINVAL, "wfd_ac: order must be 2, 4, 6 or 8");
     if (g->dtype != WFD_F32 && g->dtype != WFD_F64) return ac_fail(WFD_EINVAL, "wfd_ac: bad dtype");
     if (g->nshots < 1 || g->ld <= 0 || g->elems <= 0) return ac_fail(WFD_EINVAL, "wfd_ac: grid not initialised");
     if (!med->k || !med->cn || !med->cw) return ac_fail(WFD_EINVAL, "wfd_ac: NULL coefficient field");
@@ -319,6 +384,9 @@ void ac_base(AcArgs<T>& a, const wfd_grid* g, const wfd_ac_medium* med)
     a.br = med->beta ? (const T*)med->beta : nullptr;
     a.bc = med->beta ? (const T*)med->beta + g->n[0] : nullptr;
     a.shot_stride = g->elems;
+    a.cstride = g->elems;
+    a.pstride = g->elems * g->nshots;
+    a.K = g->order / 2;
     a.n0 = g->n[0];
     a.n1 = g->n[1];
     a.ld = g->ld;
@@ -377,8 +445,9 @@ int ac_backward_t(const wfd_grid* g, void* ua, void* ub, void* ma, void* mb, con
     const int64_t field = g->elems * g->nshots;
     hipError_t e = launch_zero(ma, fbytes, s);
     if (e == hipSuccess) e = launch_zero(mb, fbytes, s);
-    if (e == hipSuccess) e = launch_zero(gwork, 3 * fbytes, s);
-    if (e == hipSuccess) e = launch_zero(G, 3 * field * (int64_t)sizeof(double), s);
+    const int64_t nf = 2 * (g->order / 2) + 1;
+    if (e == hipSuccess) e = launch_zero(gwork, nf * fbytes, s);
+    if (e == hipSuccess) e = launch_zero(G, nf * field * (int64_t)sizeof(double), s);
     if (e != hipSuccess) return ac_fail(WFD_EHIP, "wfd_ac_backward: clearing work fields failed");
     const int64_t off = g->offset;
     T* PA = (T*)ua + off;  // p^{N-1}
@@ -386,11 +455,7 @@ int ac_backward_t(const wfd_grid* g, void* ua, void* ub, void* ma, void* mb, con
     T* MA = (T*)ma + off;
     T* MB = (T*)mb + off;
     a.gm = (T*)gwork + off;
-    a.fn = a.gm + field;
-    a.fw = a.fn + field;
     a.Gm = G + off;
-    a.Fn = a.Gm + field;
-    a.Fw = a.Fn + field;
     a.ga_off = 0;   // mu^{n+1} at the sources into row n
     a.sc_off = -1;  // strip row n - 1 into p^{n-1}
     int64_t imaged = 0;

@@ -16,7 +16,7 @@ Conventions of this build (cuwave's own are not available to check against):
   ``u_tt = v^2 lap u`` and ``density`` does not enter; with a density indicator ``gamma`` (``simulate``,
   ``measure``, ``misfit``, ``misfit_gradient``) it is the variable-density form
   ``gamma rho u_tt = div(gamma rho v^2 grad u) + rho v^2 f`` on the zero boundary ring, run by the
-  variable mass / stiffness kernels (2D, order 2).
+  variable mass / stiffness kernels (2D).
 * ``threads`` is accepted for compatibility; launch shapes are chosen by the solver's tuner.
 """
 
@@ -116,7 +116,7 @@ class ScalarWave:
     def grid(self, nshots: int = 1):
         from .solver import WaveGrid
 
-        return WaveGrid(self.Nx, self.dx, self.dt, 2, self.dtype, nshots=nshots, device=self.device)
+        return WaveGrid(self.Nx, self.dx, self.dt, self.space_order, self.dtype, nshots=nshots, device=self.device)
 
     def velocity(self) -> torch.Tensor:
         """Wave speed as a float64 node array on the device."""

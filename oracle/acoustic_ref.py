@@ -39,15 +39,46 @@ def _interior(a):
 
 
 def face_q(p, cn, cw):
-    """``q = (cN dN + cS dS) + (cW dW + cE dE)`` on the interior nodes (kernel association)."""
+    """``q`` on the interior nodes in the kernels' association, for K bonds per side and axis:
+    ``qx = cN_1 dN_1 + cS_1 dS_1``, then ``qx = qx + (cN_m dN_m + cS_m dS_m)`` for m = 2..K, the
+    same for columns, ``q = qx + qy`` (for K = 1: ``(cN dN + cS dS) + (cW dW + cE dE)``).
+
+    ``cn`` / ``cw`` are ``(K, n0 + K - 1, n1 + K - 1)`` (a single ``(n0, n1)`` array for K = 1):
+    ``cn[m-1][i, j]`` couples nodes (i-m, j) and (i, j), ``cw[m-1][i, j]`` nodes (i, j-m) and (i, j);
+    outside the node box the field is zero.  Returns ``(q, (dN, dS, dW, dE))`` with lists over m."""
+    cn = np.asarray(cn)
+    cw = np.asarray(cw)
+    if cn.ndim == 2:
+        cn, cw = cn[None], cw[None]
+    K = cn.shape[0]
+    n0, n1 = p.shape
+    P = np.zeros((n0 + 2 * K, n1 + 2 * K), p.dtype)
+    P[K:K + n0, K:K + n1] = p
     C = _interior(p)
-    dN = p[:-2, 1:-1] - C
-    dS = p[2:, 1:-1] - C
-    dW = p[1:-1, :-2] - C
-    dE = p[1:-1, 2:] - C
-    cN, cS = cn[1:-1, 1:-1], cn[2:, 1:-1]
-    cW, cE = cw[1:-1, 1:-1], cw[1:-1, 2:]
-    return (cN * dN + cS * dS) + (cW * dW + cE * dE), (dN, dS, dW, dE)
+    r = slice(K + 1, K + n0 - 1)
+    c = slice(K + 1, K + n1 - 1)
+    dN, dS, dW, dE = [], [], [], []
+    for m in range(1, K + 1):
+        dN.append(P[K + 1 - m:K + n0 - 1 - m, c] - C)
+        dS.append(P[K + 1 + m:K + n0 - 1 + m, c] - C)
+        dW.append(P[r, K + 1 - m:K + n1 - 1 - m] - C)
+        dE.append(P[r, K + 1 + m:K + n1 - 1 + m] - C)
+    cN = [cn[m - 1][1:n0 - 1, 1:n1 - 1] for m in range(1, K + 1)]
+    cS = [cn[m - 1][1 + m:n0 - 1 + m, 1:n1 - 1] for m in range(1, K + 1)]
+    cW = [cw[m - 1][1:n0 - 1, 1:n1 - 1] for m in range(1, K + 1)]
+    cE = [cw[m - 1][1:n0 - 1, 1 + m:n1 - 1 + m] for m in range(1, K + 1)]
+    qx = cN[0] * dN[0] + cS[0] * dS[0]
+    for m in range(1, K):
+        qx = qx + (cN[m] * dN[m] + cS[m] * dS[m])
+    qy = cW[0] * dW[0] + cE[0] * dE[0]
+    for m in range(1, K):
+        qy = qy + (cW[m] * dW[m] + cE[m] * dE[m])
+    return qx + qy, (dN, dS, dW, dE)
+
+
+def _bonds(a):
+    a = np.asarray(a)
+    return a[None] if a.ndim == 2 else a
 
 
 def _damping(beta_r, beta_c, shape, dtype):
@@ -72,7 +103,8 @@ def forward_np(k, cn, cw, nsteps, beta=(None, None), src=(), sig=None, rec=(), s
     field after step n at the receivers and store row n the field before step n at the strip nodes.
     """
     dt = np.dtype(dtype).type
-    k, cn, cw = (np.asarray(a, dtype=dtype) for a in (k, cn, cw))
+    k = np.asarray(k, dtype=dtype)
+    cn, cw = (_bonds(a).astype(dtype) for a in (cn, cw))
     shape = k.shape
     B = _damping(beta[0], beta[1], shape, dtype)
     b = None if B is None else _interior(B)
@@ -111,30 +143,36 @@ def undamped_mask(beta_r, beta_c, shape, dtype=np.float32):
     return m
 
 
-def strip_nodes(beta_r, beta_c, shape, dtype=np.float32):
-    """Damped interior nodes with an undamped face neighbour, in C order, ``(S, 2)``."""
+def strip_nodes(beta_r, beta_c, shape, dtype=np.float32, K=1):
+    """Damped interior nodes within K nodes along an axis of an undamped one, in C order, ``(S, 2)``."""
     und = undamped_mask(beta_r, beta_c, shape, dtype)
     inter = np.zeros(shape, bool)
     inter[1:-1, 1:-1] = True
     nb = np.zeros(shape, bool)
-    nb[1:, :] |= und[:-1, :]
-    nb[:-1, :] |= und[1:, :]
-    nb[:, 1:] |= und[:, :-1]
-    nb[:, :-1] |= und[:, 1:]
+    for m in range(1, K + 1):
+        nb[m:, :] |= und[:-m, :]
+        nb[:-m, :] |= und[m:, :]
+        nb[:, m:] |= und[:, :-m]
+        nb[:, :-m] |= und[:, m:]
     return np.argwhere(inter & ~und & nb)
 
 
 def backward_np(k, cn, cw, nsteps, p_last, beta=(None, None), src=(), sig=None, rec=(), adj=None, strip=(),
                 store=None, dtype=np.float32):
-    """The backward sweep of ``wfd_ac_backward``: returns ``(G [3, n0, n1] float64, mu_src [N, S])``.
+    """The backward sweep of ``wfd_ac_backward``: ``(G [2K+1, n0+K-1, n1+K-1] float64, mu_src [N, S])``.
 
-    ``p_last = (p^{N-1}, p^N)``; ``adj`` rows 0..N injected into mu (row n when mu^n is formed);
-    nodes whose forward value is not known to the sweep hold NaN, so any use of them shows.
+    ``G[0]`` the mass term on the nodes, ``G[m]`` / ``G[K+m]`` the north / west bond sums at distance
+    m (bonds past the last interior row / column stored at their far end).  ``p_last = (p^{N-1},
+    p^N)``; ``adj`` rows 0..N injected into mu (row n when mu^n is formed); nodes whose forward value
+    the sweep does not know hold NaN, so any use of them shows.
     """
     dt = np.dtype(dtype).type
-    k, cn, cw = (np.asarray(a, dtype=dtype) for a in (k, cn, cw))
+    k = np.asarray(k, dtype=dtype)
+    cn, cw = (_bonds(a).astype(dtype) for a in (cn, cw))
+    K = cn.shape[0]
     shape = k.shape
     n0, n1 = shape
+    E0, E1 = n0 + K - 1, n1 + K - 1
     B = _damping(beta[0], beta[1], shape, dtype)
     b = None if B is None else _interior(B)
     und = undamped_mask(beta[0], beta[1], shape, dtype)
@@ -149,17 +187,24 @@ def backward_np(k, cn, cw, nsteps, p_last, beta=(None, None), src=(), sig=None, 
     p_cur = np.where(known, np.asarray(p_last[0], dtype), np.nan).astype(dtype)
     mu_p = np.zeros(shape, dtype)
     mu_c = np.zeros(shape, dtype)
-    part = np.zeros((3, n0, n1), dtype)
-    G = np.zeros((3, n0, n1), np.float64)
+    part = np.zeros((2 * K + 1, E0, E1), dtype)
+    G = np.zeros((2 * K + 1, E0, E1), np.float64)
     mu_src = np.zeros((nsteps, len(src)), dtype)
     two, one = dt(2), dt(1)
     kin = _interior(k)
     und_in = _interior(und)
-    # faces: north face of (i, j) imaged when (i, j) or (i-1, j) is undamped interior
-    nund = np.zeros_like(und_in)
-    nund[1:, :] = und_in[:-1, :]
-    wund = np.zeros_like(und_in)
-    wund[:, 1:] = und_in[:, :-1]
+    ni, nj = und_in.shape
+    # bond at distance m to the north / west imaged when either end is an undamped interior node
+    nund, wund = [], []
+    for m in range(1, K + 1):
+        a_ = np.zeros_like(und_in)
+        a_[m:, :] = und_in[:-m, :]
+        nund.append(und_in | a_)
+        w_ = np.zeros_like(und_in)
+        w_[:, m:] = und_in[:, :-m]
+        wund.append(und_in | w_)
+    rows = np.arange(1, n0 - 1)[:, None]
+    cols = np.arange(1, n1 - 1)[None, :]
     imaged = 0
     for n in range(nsteps, 0, -1):
         if n <= nsteps - 1:
@@ -171,7 +216,6 @@ def backward_np(k, cn, cw, nsteps, p_last, beta=(None, None), src=(), sig=None, 
         if rec and adj is not None:
             _inject(numm, k, rec, adj[n])
         mu_n = np.zeros(shape, dtype)
-        mu_n[0, :], mu_n[-1, :], mu_n[:, 0], mu_n[:, -1] = mu_p[0, :], mu_p[-1, :], mu_p[:, 0], mu_p[:, -1]
         _interior(mu_n)[...] = numm / (one + b) if b is not None else numm
         if n < nsteps:
             imaged += 1
@@ -186,17 +230,17 @@ def backward_np(k, cn, cw, nsteps, p_last, beta=(None, None), src=(), sig=None, 
                 _interior(p_prev)[...] = np.where(und_in, rv - _interior(p_next), np.nan)
                 for i, p in enumerate(strip):
                     p_prev[p] = store[n - 1, i]
-                adds = [np.where(und_in, C * q, 0)]
-                adds.append(np.where(und_in | nund, dN * eN, 0))
-                adds.append(np.where(und_in | wund, dW * eW, 0))
-            for a in range(3):
-                v = _interior(part[a]) + adds[a].astype(dtype)
-                _interior(part[a])[...] = v
-            # far faces of the last interior row / column, stored at the boundary node beyond
-            last_r = np.where(und_in[-1, :], dS[-1, :] * eS[-1, :], 0).astype(dtype)
-            part[1, n0 - 1, 1:-1] = part[1, n0 - 1, 1:-1] + last_r
-            last_c = np.where(und_in[:, -1], dE[:, -1] * eE[:, -1], 0).astype(dtype)
-            part[2, 1:-1, n1 - 1] = part[2, 1:-1, n1 - 1] + last_c
+                part[0, 1:n0 - 1, 1:n1 - 1] += np.where(und_in, C * q, 0).astype(dtype)
+                for m in range(1, K + 1):
+                    part[m, 1:n0 - 1, 1:n1 - 1] += np.where(nund[m - 1], dN[m - 1] * eN[m - 1], 0).astype(dtype)
+                    part[K + m, 1:n0 - 1, 1:n1 - 1] += np.where(wund[m - 1], dW[m - 1] * eW[m - 1], 0).astype(dtype)
+                    # bonds past the last interior row / column, stored at their far end
+                    far_r = und_in & (rows + m >= n0 - 1)
+                    ii, jj = np.nonzero(far_r)
+                    part[m, ii + 1 + m, jj + 1] += (dS[m - 1] * eS[m - 1])[ii, jj].astype(dtype)
+                    far_c = und_in & (cols + m >= n1 - 1)
+                    ii, jj = np.nonzero(far_c)
+                    part[K + m, ii + 1, jj + 1 + m] += (dE[m - 1] * eE[m - 1])[ii, jj].astype(dtype)
             if flush:
                 G += part.astype(np.float64)
                 part[...] = 0
@@ -207,25 +251,22 @@ def backward_np(k, cn, cw, nsteps, p_last, beta=(None, None), src=(), sig=None, 
     return G, mu_src
 
 
-def _masked_part_semantics_note():
-    """The kernel adds ``0`` to faces it does not image only where it skips them; the restatement
-    adds a zero instead, which leaves fp32 partial sums unchanged (x + 0 == x exactly)."""
-
-
 # ------------------------------------------------------------------ exact float64 transpose
 def adjoint_exact(k, cn, cw, nsteps, beta=(None, None), src=(), sig=None, rec=(), dJdtr=None):
     """Gradient pieces of ``J(traces)`` in float64 with the whole forward history kept.
 
-    ``dJdtr [N, R]`` is dJ/dtrace (trace row n = p^{n+1}).  Returns ``(dJ/dk, dJ/dcn, dJ/dcw)`` as
-    node / face arrays, imaging every interior node and every face touching one.
+    ``dJdtr [N, R]`` is dJ/dtrace (trace row n = p^{n+1}).  Returns ``(dJ/dk, dJ/dcn, dJ/dcw)``:
+    node array and bond arrays ``(K, n0+K-1, n1+K-1)``, imaging every interior node and every bond
+    touching one.
     """
-    k, cn, cw = (np.asarray(a, np.float64) for a in (k, cn, cw))
+    k = np.asarray(k, np.float64)
+    cn, cw = (_bonds(a).astype(np.float64) for a in (cn, cw))
+    K = cn.shape[0]
     shape = k.shape
     n0, n1 = shape
     src_ = np.asarray(src, np.int64).reshape(-1, 2)
     rec_ = np.asarray(rec, np.int64).reshape(-1, 2)
     sig = None if sig is None else np.asarray(sig, np.float64)
-    # forward history p^0 .. p^N
     hist = [np.zeros(shape), np.zeros(shape)]  # p^{-1}, p^0
     B = _damping(beta[0], beta[1], shape, np.float64)
     b = None if B is None else _interior(B)
@@ -234,16 +275,13 @@ def adjoint_exact(k, cn, cw, nsteps, beta=(None, None), src=(), sig=None, rec=()
         q, _ = face_q(pc, cn, cw)
         t = 2 * _interior(pc) - ((1 - b) * _interior(pp) if b is not None else _interior(pp))
         num = t + _interior(k) * q
-        for s, (i, j) in enumerate(src_):
-            num[i - 1, j - 1] += k[i, j] * sig[n, s]
+        for s_, (i, j) in enumerate(src_):
+            num[i - 1, j - 1] += k[i, j] * sig[n, s_]
         nxt = np.zeros(shape)
         _interior(nxt)[...] = num / (1 + b) if b is not None else num
         hist.append(nxt)
     P = hist[1:]  # P[n] = p^n, n = 0..N
-    mu = [np.zeros(shape), np.zeros(shape)]  # mu^{N+2}, mu^{N+1}
-    dk = np.zeros(shape)
-    dcn = np.zeros(shape)
-    dcw = np.zeros(shape)
+    mu = [np.zeros(shape), np.zeros(shape)]
     mus = {}
     for n in range(nsteps, 0, -1):
         mp, mc = mu[-2], mu[-1]
@@ -256,19 +294,27 @@ def adjoint_exact(k, cn, cw, nsteps, beta=(None, None), src=(), sig=None, rec=()
         _interior(mn)[...] = num / (1 + b) if b is not None else num
         mu.append(mn)
         mus[n] = mn
+    dk = np.zeros(shape)
+    dcn = np.zeros(cn.shape)
+    dcw = np.zeros(cw.shape)
+    rows = np.arange(1, n0 - 1)[:, None]
+    cols = np.arange(1, n1 - 1)[None, :]
     for n in range(nsteps):  # pairs (mu^{n+1}, p^n)
         m1, p = mus[n + 1], P[n]
         q, (dN, dS, dW, dE) = face_q(p, cn, cw)
         _, (eN, eS, eW, eE) = face_q(m1, cn, cw)
         f = np.zeros(shape)
-        for s, (i, j) in enumerate(src_):
-            f[i, j] += sig[n, s]
+        for s_, (i, j) in enumerate(src_):
+            f[i, j] += sig[n, s_]
         with np.errstate(divide="ignore", invalid="ignore"):
             _interior(dk)[...] -= _interior(m1) * (q + _interior(f)) / _interior(k)
-        _interior(dcn)[...] += dN * eN
-        _interior(dcw)[...] += dW * eW
-        dcn[n0 - 1, 1:-1] += dS[-1, :] * eS[-1, :]
-        dcw[1:-1, n1 - 1] += dE[:, -1] * eE[:, -1]
+        for m in range(1, K + 1):
+            dcn[m - 1, 1:n0 - 1, 1:n1 - 1] += dN[m - 1] * eN[m - 1]
+            dcw[m - 1, 1:n0 - 1, 1:n1 - 1] += dW[m - 1] * eW[m - 1]
+            ii, jj = np.nonzero(np.broadcast_to(rows + m >= n0 - 1, (n0 - 2, n1 - 2)))
+            dcn[m - 1, ii + 1 + m, jj + 1] += (dS[m - 1] * eS[m - 1])[ii, jj]
+            ii, jj = np.nonzero(np.broadcast_to(cols + m >= n1 - 1, (n0 - 2, n1 - 2)))
+            dcw[m - 1, ii + 1, jj + 1 + m] += (dE[m - 1] * eE[m - 1])[ii, jj]
     return dk, dcn, dcw
 
 
@@ -278,12 +324,16 @@ def traces_exact(k, cn, cw, nsteps, beta=(None, None), src=(), sig=None, rec=())
 
 
 def forward_torch(k, cn, cw, nsteps, beta=(None, None), src=(), sig=None, rec=()):
-    """The forward map as differentiable torch operations (float64 tensors ``k``, ``cn``, ``cw``):
-    traces ``[N, R]``, for gradients by torch autograd."""
+    """The forward map as differentiable torch operations (float64 tensors ``k`` and bond arrays
+    ``cn``, ``cw`` of shape ``(K, n0+K-1, n1+K-1)`` or ``(n0, n1)``): traces ``[N, R]``."""
     import torch
     import torch.nn.functional as F
 
+    if cn.dim() == 2:
+        cn, cw = cn[None], cw[None]
+    K = cn.shape[0]
     shape = k.shape
+    n0, n1 = shape
     src_ = torch.as_tensor(np.asarray(src, np.int64).reshape(-1, 2))
     rec_ = torch.as_tensor(np.asarray(rec, np.int64).reshape(-1, 2))
     sig = None if sig is None else torch.as_tensor(sig, dtype=torch.float64)
@@ -297,9 +347,14 @@ def forward_torch(k, cn, cw, nsteps, beta=(None, None), src=(), sig=None, rec=()
     kin = k[1:-1, 1:-1]
     out = []
     for n in range(nsteps):
+        P = F.pad(pc, (K, K, K, K))
         C = pc[1:-1, 1:-1]
-        q = (cn[1:-1, 1:-1] * (pc[:-2, 1:-1] - C) + cn[2:, 1:-1] * (pc[2:, 1:-1] - C)) + \
-            (cw[1:-1, 1:-1] * (pc[1:-1, :-2] - C) + cw[1:-1, 2:] * (pc[1:-1, 2:] - C))
+        q = 0
+        for m in range(1, K + 1):
+            q = q + cn[m - 1][1:n0 - 1, 1:n1 - 1] * (P[K + 1 - m:K + n0 - 1 - m, K + 1:K + n1 - 1] - C) \
+                + cn[m - 1][1 + m:n0 - 1 + m, 1:n1 - 1] * (P[K + 1 + m:K + n0 - 1 + m, K + 1:K + n1 - 1] - C) \
+                + cw[m - 1][1:n0 - 1, 1:n1 - 1] * (P[K + 1:K + n0 - 1, K + 1 - m:K + n1 - 1 - m] - C) \
+                + cw[m - 1][1:n0 - 1, 1 + m:n1 - 1 + m] * (P[K + 1:K + n0 - 1, K + 1 + m:K + n1 - 1 + m] - C)
         t = 2 * C - ((1 - b) * pp[1:-1, 1:-1] if b is not None else pp[1:-1, 1:-1])
         f = torch.zeros_like(C)
         if len(src_):

@@ -30,9 +30,9 @@ def _smooth(shape, seed, lo, hi):
     return lo + (hi - lo) * f
 
 
-def _coeffs(m, s, h, dt, closure):
+def _coeffs(m, s, h, dt, closure, order=2):
     k, cn, cw = ac.face_operator(torch.as_tensor(m, dtype=torch.float64), torch.as_tensor(s, dtype=torch.float64),
-                                 h, dt, closure)
+                                 h, dt, closure, order)
     return k.numpy(), cn.numpy(), cw.numpy()
 
 
@@ -50,13 +50,14 @@ def _sponge(n, width, beta):
     return p
 
 
+@pytest.mark.parametrize("order", [2, 4, 8])
 @pytest.mark.parametrize("closure", ["neumann", "dirichlet"])
-def test_energy_conserved_without_damping(closure):
+def test_energy_conserved_without_damping(closure, order):
     shape, h = (23, 31), (0.1, 0.08)
     m = _smooth(shape, 1, 0.5, 3.0)
     s = _smooth(shape, 2, 0.2, 2.0)
-    dt = _dt(m, s, h, 0.7)
-    k, cn, cw = _coeffs(m, s, h, dt, closure)
+    dt = _dt(m, s, h, 0.5)
+    k, cn, cw = _coeffs(m, s, h, dt, closure, order)
     rng = np.random.default_rng(3)
     p0, p1 = np.zeros(shape), np.zeros(shape)
     p0[1:-1, 1:-1] = rng.standard_normal((shape[0] - 2, shape[1] - 2))
@@ -84,28 +85,48 @@ def _mode_run(k, cn, cw, phi, omega, steps):
     return err
 
 
-def test_neumann_cosine_modes_closed_form():
+@pytest.mark.parametrize("order", [2, 4, 6, 8])
+def test_neumann_cosine_modes_closed_form(order):
     """Constant m, s under the no-flux closure: cos(a pi (i-1)/(n0-3)) cos(b pi (j-1)/(n1-3)) on the
-    interior is an exact mode with eigenvalue (dt^2 s/m) sum 4/h^2 sin^2(theta/2); the half
-    control volumes of the first / last interior nodes are what make it exact."""
+    interior is an exact mode with eigenvalue -(dt^2 s/m) sum_d (a_0 + 2 sum_m a_m cos(m theta_d))/h_d^2
+    (for order 2: (dt^2 s/m) sum 4/h^2 sin^2(theta/2)); the half control volumes of the first / last
+    interior nodes, the half wall bonds and the bonds the even reflection adds make it exact."""
     shape, h, m, s = (17, 22), (0.1, 0.13), 2.0, 3.0
     dt = 0.4 * min(h) / math.sqrt(s / m) / math.sqrt(2)
     M = np.full(shape, m)
     S = np.full(shape, s)
-    k, cn, cw = _coeffs(M, S, h, dt, "neumann")
+    k, cn, cw = _coeffs(M, S, h, dt, "neumann", order)
     a, b = 2, 3
     ta, tb = a * math.pi / (shape[0] - 3), b * math.pi / (shape[1] - 3)
     i = np.arange(shape[0])[:, None]
     j = np.arange(shape[1])[None, :]
     phi = np.cos(ta * (i - 1)) * np.cos(tb * (j - 1))
     phi[0, :] = phi[-1, :] = phi[:, 0] = phi[:, -1] = 0.0
-    lam = dt * dt * s / m * (4 / h[0] ** 2 * math.sin(ta / 2) ** 2 + 4 / h[1] ** 2 * math.sin(tb / 2) ** 2)
+    w = [float(x) for x in slice_ref.fd_weights(order)]
+    sym = lambda th: w[0] + 2 * sum(w[q] * math.cos(q * th) for q in range(1, len(w)))  # noqa: E731
+    lam = -dt * dt * s / m * (sym(ta) / h[0] ** 2 + sym(tb) / h[1] ** 2)
     omega = math.acos(1 - lam / 2)
     assert _mode_run(k, cn, cw, phi, omega, 200) < 1e-10
     # mutation: full control volumes at the walls (no factor 2) -- the mode is no longer exact
     k_bad = np.zeros(shape)
     k_bad[1:-1, 1:-1] = dt * dt / m
     assert _mode_run(k_bad, cn, cw, phi, omega, 200) > 1e-3
+
+
+@pytest.mark.parametrize("order", [4, 6, 8])
+def test_dirichlet_constant_coefficients_reduce_to_central_stencil(order):
+    """m = 1/c^2, s = 1 with the field zero outside the interior is the scalar model's central
+    stencil of the same order (slice_ref.forward with kc = c^2 dt^2)."""
+    NX, NY = 40, 57
+    h = (1.0 / (NX - 1), 2.0 / (NY - 1))
+    c = 1.3
+    dt = 0.8 * slice_ref.stable_dt(h, c, order)
+    u = slice_ref.gaussian_ic(NX, NY, center=(0.5, 0.9), sigma=0.08)
+    u[0, :] = u[-1, :] = u[:, 0] = u[:, -1] = 0.0
+    k, cn, cw = _coeffs(np.full((NX, NY), 1 / c ** 2), np.ones((NX, NY)), h, dt, "dirichlet", order)
+    _, _, (_, pa) = ar.forward_np(k, cn, cw, 60, dtype=np.float64, state=(u.copy(), u.copy()))
+    _, _, ref = slice_ref.forward(np.full((NX, NY), (c * dt) ** 2), order, h, 60, u_prev=u.copy(), u_cur=u.copy())
+    assert np.linalg.norm(pa - ref) / np.linalg.norm(ref) < 1e-12
 
 
 def test_dirichlet_constant_coefficients_reduce_to_reference_step():
@@ -126,7 +147,7 @@ def test_dirichlet_constant_coefficients_reduce_to_reference_step():
     assert np.linalg.norm(pa - a1) / np.linalg.norm(a1) < 1e-12
 
 
-def _div_s_grad_p_error(n, harmonic=True):
+def _div_s_grad_p_error(n, harmonic=True, order=2):
     h = 1.0 / (n - 1)
     x = np.linspace(0, 1, n)[:, None]
     y = np.linspace(0, 1, n)[None, :]
@@ -140,38 +161,42 @@ def _div_s_grad_p_error(n, harmonic=True):
     pyy = -np.sin(2 * x + y) - x * x * np.cos(x * y)
     exact = sx * px + sy * py + s * (pxx + pyy)
     if harmonic:
-        _, cn, cw = _coeffs(np.ones((n, n)), s, (h, h), 1.0, "dirichlet")
+        _, cn, cw = _coeffs(np.ones((n, n)), s, (h, h), 1.0, "dirichlet", order)
     else:  # mutation: the face takes the stiffness of one end only (first order)
         cn = np.zeros((n, n))
         cw = np.zeros((n, n))
         cn[1:, :] = s[:-1, :] / h ** 2
         cw[:, 1:] = s[:, :-1] / h ** 2
     q, _ = ar.face_q(p, cn, cw)
-    return float(np.abs(q - exact[1:-1, 1:-1]).max())
+    K = order // 2  # nodes whose stencil reaches past the ring see the zero extension, not p
+    inner = (slice(K - 1, q.shape[0] - K + 1), slice(K - 1, q.shape[1] - K + 1))
+    return float(np.abs(q - exact[1:-1, 1:-1])[inner].max())
 
 
-def test_face_operator_is_second_order_consistent():
-    e = [_div_s_grad_p_error(n) for n in (33, 65, 129)]
-    assert e[0] / e[1] > 3.5 and e[1] / e[2] > 3.5
+@pytest.mark.parametrize("order", [2, 4, 8])
+def test_face_operator_is_second_order_consistent(order):
+    e = [_div_s_grad_p_error(n, order=order) for n in (33, 65, 129)]
+    # at least second order until the error reaches rounding (|exact| ~ 10, 1/h^2 ~ 1e4)
+    assert e[0] / e[1] > 3.5 and (e[1] / e[2] > 3.5 or e[2] < 1e-8)
     bad = [_div_s_grad_p_error(n, harmonic=False) for n in (33, 65, 129)]
     assert bad[1] / bad[2] < 2.5  # the one-sided face converges only to first order
 
 
-def _problem(closure="neumann", shape=(26, 33), N=48, seed=5, sponge=True):
+def _problem(closure="neumann", shape=(26, 33), N=48, seed=5, sponge=True, order=2):
     h = (0.05, 0.04)
     m = _smooth(shape, seed, 0.6, 2.5)
     s = _smooth(shape, seed + 1, 0.3, 1.8)
-    dt = _dt(m, s, h, 0.6)
-    k, cn, cw = _coeffs(m, s, h, dt, closure)
+    dt = _dt(m, s, h, 0.45)
+    k, cn, cw = _coeffs(m, s, h, dt, closure, order)
     beta = (_sponge(shape[0], 5, 0.12), _sponge(shape[1], 6, 0.1)) if sponge else (None, None)
     src = [(12, 15), (12, 15), (8, 20)]
     t = np.arange(N) * dt
     rng = np.random.default_rng(seed)
     sig = np.stack([slice_ref.ricker(t - 0.1 * i, 1.0 / (12 * dt)) for i in range(3)], 1) * rng.uniform(0.5, 2, 3)
     rec = [(7, 7), (15, 25), (18, 9), (10, 16)]
-    strip = ar.strip_nodes(beta[0], beta[1], shape, np.float64) if sponge else np.zeros((0, 2), np.int64)
+    strip = ar.strip_nodes(beta[0], beta[1], shape, np.float64, order // 2) if sponge else np.zeros((0, 2), np.int64)
     return dict(m=m, s=s, h=h, dt=dt, k=k, cn=cn, cw=cw, beta=beta, src=src, sig=sig, rec=rec, strip=strip,
-                N=N, shape=shape, closure=closure)
+                N=N, shape=shape, closure=closure, order=order)
 
 
 def _backward64(P, dJ):
@@ -184,11 +209,12 @@ def _backward64(P, dJ):
     return tr, G, mu_src
 
 
+@pytest.mark.parametrize("order", [2, 4, 8])
 @pytest.mark.parametrize("closure", ["neumann", "dirichlet"])
-def test_adjoint_dot_product(closure):
+def test_adjoint_dot_product(closure, order):
     """<traces(sig), y> = <sig, -mu_src(y)>: the adjoint recursion, its injection and the source
     gather are the transpose of the forward map."""
-    P = _problem(closure)
+    P = _problem(closure, order=order)
     y = np.random.default_rng(9).standard_normal((P["N"], len(P["rec"])))
     tr, G, mu_src = _backward64(P, lambda tr: y)
     lhs = float((tr * y).sum())
@@ -198,7 +224,7 @@ def test_adjoint_dot_product(closure):
 
 
 def _dk_from_G(P, G, mu_src):
-    Gm = G[0].copy()
+    Gm = G[0][:P["shape"][0], :P["shape"][1]].copy()
     for s, p in enumerate(P["src"]):
         Gm[p] += (mu_src[:, s] * P["sig"][:, s]).sum()
     und = ar.undamped_mask(P["beta"][0], P["beta"][1], P["shape"], np.float64)
@@ -207,26 +233,33 @@ def _dk_from_G(P, G, mu_src):
     return dk, und
 
 
+@pytest.mark.parametrize("order", [2, 4, 8])
 @pytest.mark.parametrize("closure", ["neumann", "dirichlet"])
-def test_reconstruction_equals_full_history_transpose(closure):
-    P = _problem(closure)
+def test_reconstruction_equals_full_history_transpose(closure, order):
+    P = _problem(closure, order=order)
     obs = 0.7 * ar.traces_exact(P["k"], P["cn"], P["cw"], P["N"], P["beta"], P["src"], P["sig"], P["rec"])
     tr, G, mu_src = _backward64(P, lambda tr: tr - obs)
     dk, und = _dk_from_G(P, G, mu_src)
     ek, ecn, ecw = ar.adjoint_exact(P["k"], P["cn"], P["cw"], P["N"], P["beta"], P["src"], P["sig"], P["rec"],
                                     tr - obs)
     assert np.abs(dk[und] - ek[und]).max() <= 1e-9 * np.abs(ek[und]).max()
-    # faces with an undamped end
-    fn = und.copy()
-    fn[1:, :] |= und[:-1, :]
-    fw = und.copy()
-    fw[:, 1:] |= und[:, :-1]
-    for got, ref, mask in ((G[1], ecn, fn), (G[2], ecw, fw)):
-        assert np.abs(got[mask] - ref[mask]).max() <= 1e-9 * np.abs(ref[mask]).max()
+    # bonds with an undamped end (north / west bond of a node, or past the far side of one)
+    K = P["order"] // 2
+    n0, n1 = P["shape"]
+    for m in range(1, K + 1):
+        fn = np.zeros((n0 + K - 1, n1 + K - 1), bool)
+        fn[:n0, :n1] = und
+        fn[m:n0, :n1] |= und[:n0 - m, :]
+        fw = np.zeros_like(fn)
+        fw[:n0, :n1] = und
+        fw[:n0, m:n1] |= und[:, :n1 - m]
+        for got, ref, mask in ((G[m], ecn[m - 1], fn), (G[K + m], ecw[m - 1], fw)):
+            assert np.abs(got[mask] - ref[mask]).max() <= 1e-9 * np.abs(ref).max()
 
 
-def test_gradient_central_differences():
-    P = _problem("neumann", N=40)
+@pytest.mark.parametrize("order", [2, 8])
+def test_gradient_central_differences(order):
+    P = _problem("neumann", N=40, order=order)
     obs = 0.5 * ar.traces_exact(P["k"], P["cn"], P["cw"], P["N"], P["beta"], P["src"], P["sig"], P["rec"])
 
     def J(k, cn, cw):
@@ -238,43 +271,46 @@ def test_gradient_central_differences():
                                     tr - obs)
     rng = np.random.default_rng(11)
     und = ar.undamped_mask(P["beta"][0], P["beta"][1], P["shape"], np.float64)
-    cand = np.argwhere(und)
+    K = order // 2
+    cand = [("k", tuple(x)) for x in np.argwhere(und)]
+    for m in range(K):
+        cand += [("cn", (m, *x)) for x in np.argwhere(und & (P["cn"][m][:und.shape[0], :und.shape[1]] != 0))]
+        cand += [("cw", (m, *x)) for x in np.argwhere(und & (P["cw"][m][:und.shape[0], :und.shape[1]] != 0))]
+    refs = {"k": ek, "cn": ecn, "cw": ecw}
     checks = 0
-    for which, ref in (("k", ek), ("cn", ecn), ("cw", ecw)):
-        for idx in cand[rng.choice(len(cand), 4, replace=False)]:
-            idx = tuple(idx)
-            arrs = {"k": P["k"].copy(), "cn": P["cn"].copy(), "cw": P["cw"].copy()}
-            base = arrs[which][idx]
-            if base == 0:
-                continue
-            eps = 1e-5 * abs(base)
-            arrs[which][idx] = base + eps
-            jp = J(arrs["k"], arrs["cn"], arrs["cw"])
-            arrs[which][idx] = base - eps
-            jm = J(arrs["k"], arrs["cn"], arrs["cw"])
-            fd = (jp - jm) / (2 * eps)
-            assert abs(fd - ref[idx]) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(ref).max()), (which, idx)
-            checks += 1
-    assert checks >= 10
+    for which, idx in [cand[i] for i in rng.choice(len(cand), 14, replace=False)]:
+        arrs = {"k": P["k"].copy(), "cn": P["cn"].copy(), "cw": P["cw"].copy()}
+        base = arrs[which][idx]
+        eps = 1e-5 * abs(base)
+        arrs[which][idx] = base + eps
+        jp = J(arrs["k"], arrs["cn"], arrs["cw"])
+        arrs[which][idx] = base - eps
+        jm = J(arrs["k"], arrs["cn"], arrs["cw"])
+        fd = (jp - jm) / (2 * eps)
+        ref = refs[which]
+        assert abs(fd - ref[idx]) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(ref).max()), (which, idx)
+        checks += 1
+    assert checks == 14
 
 
+@pytest.mark.parametrize("order", [2, 4, 6])
 @pytest.mark.parametrize("closure", ["neumann", "dirichlet"])
-def test_material_gradient_matches_torch_autograd(closure):
-    """The coefficient map's chain rule (harmonic faces, volume weights, source term) in
-    ac.material_gradient against torch autograd through a float64 torch forward."""
+def test_material_gradient_matches_torch_autograd(closure, order):
+    """The coefficient map's chain rule (harmonic bonds, volume weights, reflected bonds, source
+    term) in ac.material_gradient against torch autograd through a float64 torch forward."""
     from neuralmech_amd.solver import PointSet, WaveGrid
 
-    P = _problem(closure, N=36)
+    P = _problem(closure, N=36, order=order)
     obs = 0.6 * ar.traces_exact(P["k"], P["cn"], P["cw"], P["N"], P["beta"], P["src"], P["sig"], P["rec"])
     tr, G, mu_src = _backward64(P, lambda tr: tr - obs)
-    grid = WaveGrid(P["shape"], P["h"], P["dt"], 2, torch.float64, device="cpu")
+    grid = WaveGrid(P["shape"], P["h"], P["dt"], order, torch.float64, device="cpu")
     med = ac.AcMedium(grid, P["m"], P["s"], closure, P["beta"])
     src = PointSet(grid, P["src"])
     dm, ds = ac.material_gradient(med, torch.as_tensor(G)[:, None], torch.as_tensor(mu_src)[None],
                                   torch.as_tensor(P["sig"])[None], src)
     m = torch.as_tensor(P["m"]).requires_grad_(True)
     s = torch.as_tensor(P["s"]).requires_grad_(True)
-    k, cn, cw = ac.face_operator(m, s, P["h"], P["dt"], closure)
+    k, cn, cw = ac.face_operator(m, s, P["h"], P["dt"], closure, order)
     trt = ar.forward_torch(k, cn, cw, P["N"], P["beta"], P["src"], P["sig"], P["rec"])
     J = 0.5 * ((trt - torch.as_tensor(obs)) ** 2).sum()
     gm, gs = torch.autograd.grad(J, [m, s])

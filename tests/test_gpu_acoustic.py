@@ -36,16 +36,16 @@ def _profile(n, width, beta):
     return p
 
 
-def _case(dtype, closure, shape=(37, 70), N=64, sponge=True, seed=3):
+def _case(dtype, closure, shape=(37, 70), N=64, sponge=True, seed=3, order=2):
     from neuralmech_amd import acoustic as ac
     from neuralmech_amd.solver import PointSet, WaveGrid
 
     h = (0.05, 0.04)
     m = _smooth(shape, seed, 0.6, 2.5)
     s = _smooth(shape, seed + 1, 0.3, 1.8)
-    dt = 0.6 * min(h) / float(np.sqrt((s / m).max())) / math.sqrt(2)
+    dt = 0.45 * min(h) / float(np.sqrt((s / m).max())) / math.sqrt(2)
     beta = (_profile(shape[0], 6, 0.1), _profile(shape[1], 7, 0.12)) if sponge else None
-    grid = WaveGrid(shape, h, dt, 2, dtype, nshots=2, device="cuda")
+    grid = WaveGrid(shape, h, dt, order, dtype, nshots=2, device="cuda")
     med = ac.AcMedium(grid, m, s, closure, beta)
     # shot 0: two sources on one node (list order matters) and one near a wall; shot 1: one source
     src_nodes = [[(15, 30), (15, 30), (8, 66)], [(20, 12)]]
@@ -60,16 +60,17 @@ def _case(dtype, closure, shape=(37, 70), N=64, sponge=True, seed=3):
     rec = PointSet(grid, rec_nodes)
     strip = PointSet(grid, [tuple(int(c) for c in p) for p in strip_nodes]) if len(strip_nodes) else None
     npdt = np.float32 if dtype == torch.float32 else np.float64
-    coeff = [grid.nodes(f)[0].cpu().numpy().astype(npdt) for f in (med.k, med.cn, med.cw)]
+    coeff = [f.cpu().numpy().astype(npdt) for f in med.bonds()]
     prof = (None, None) if med.profiles is None else tuple(med.profiles)
     return dict(ac=ac, grid=grid, med=med, src=src, rec=rec, strip=strip, sig=sig, N=N, npdt=npdt, coeff=coeff,
                 prof=prof, src_nodes=src_nodes, rec_nodes=rec_nodes, strip_nodes=strip_nodes, shape=shape)
 
 
+@pytest.mark.parametrize("order", [2, 4, 8])
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
 @pytest.mark.parametrize("closure", ["neumann", "dirichlet"])
-def test_forward_bitexact_with_oracle(dtype, closure):
-    C = _case(getattr(torch, dtype), closure)
+def test_forward_bitexact_with_oracle(dtype, closure, order):
+    C = _case(getattr(torch, dtype), closure, order=order)
     ac = C["ac"]
     tr, store, state = ac.forward(C["grid"], C["med"], C["N"], C["src"], torch.as_tensor(C["sig"]), C["rec"],
                                   C["strip"])
@@ -87,10 +88,11 @@ def test_forward_bitexact_with_oracle(dtype, closure):
     assert float(tr.abs().max()) > 0
 
 
+@pytest.mark.parametrize("order", [2, 4, 8])
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
 @pytest.mark.parametrize("closure", ["neumann", "dirichlet"])
-def test_backward_bitexact_with_oracle(dtype, closure):
-    C = _case(getattr(torch, dtype), closure, N=70)
+def test_backward_bitexact_with_oracle(dtype, closure, order):
+    C = _case(getattr(torch, dtype), closure, N=70, order=order)
     ac = C["ac"]
     sig = torch.as_tensor(C["sig"])
     tr, store, state = ac.forward(C["grid"], C["med"], C["N"], C["src"], sig, C["rec"], C["strip"])
@@ -113,14 +115,15 @@ def test_backward_bitexact_with_oracle(dtype, closure):
     assert float(G.abs().max()) > 0
 
 
-def test_fp32_gradient_against_float64_transpose():
+@pytest.mark.parametrize("order", [2, 4])
+def test_fp32_gradient_against_float64_transpose(order):
     """Gradient with respect to mass and stiffness from an fp32 run (reconstruction adjoint)
     within 1e-3 relative L2 of the float64 full-history transpose."""
-    C = _case(torch.float32, "neumann", shape=(61, 90), N=300)
+    C = _case(torch.float32, "neumann", shape=(61, 90), N=300, order=order)
     ac = C["ac"]
     sig = torch.as_tensor(C["sig"])
     k64, cn64, cw64 = (a.numpy() for a in ac.face_operator(C["med"].mass.cpu(), C["med"].stiff.cpu(),
-                                                             C["grid"].spacing, C["grid"].dt, "neumann"))
+                                                             C["grid"].spacing, C["grid"].dt, "neumann", order))
     prof = C["prof"]
     ns = len(C["src_nodes"][0])
     obs = [0.8 * ar.traces_exact(k64, cn64, cw64, C["N"], prof, C["src_nodes"][s],
@@ -135,7 +138,7 @@ def test_fp32_gradient_against_float64_transpose():
                                     C["rec_nodes"], tr64 - obs[0])
     m = C["med"].mass.cpu().clone().requires_grad_(True)
     s = C["med"].stiff.cpu().clone().requires_grad_(True)
-    k, cn, cw = ac.face_operator(m, s, C["grid"].spacing, C["grid"].dt, "neumann")
+    k, cn, cw = ac.face_operator(m, s, C["grid"].spacing, C["grid"].dt, "neumann", order)
     und = torch.as_tensor(C["med"].undamped)
     ekt = torch.where(und, torch.as_tensor(ek), torch.zeros(()))
     rm, rs = torch.autograd.grad([k, cn, cw], [m, s], [ekt, torch.as_tensor(ecn), torch.as_tensor(ecw)])

@@ -82,16 +82,17 @@ def test_wavespeed_misfit_gradient_fp64_matches_central_differences():
         assert fd == pytest.approx(g[cell], rel=1e-5, abs=1e-9 * np.abs(g).max())
 
 
-def test_density_indicator_misfit_gradient_fp64():
-    """cuwave's misfit_gradient(sim, sources, gamma, sensors, observed) (2D_fwi.py:40): gradient in
-    the density indicator by the reconstruction adjoint, against central differences; a unit
-    indicator reproduces the constant-density run."""
+@pytest.mark.parametrize("order", [2, 4, 8])
+def test_density_indicator_misfit_gradient_fp64(order):
+    """cuwave's misfit_gradient(sim, sources, gamma, sensors, observed) (2D_fwi.py:40, orders 4 and
+    8 at its call sites): gradient in the density indicator by the reconstruction adjoint, against
+    central differences; a unit indicator reproduces the constant-density run."""
     from neuralmech_amd import acoustic, boundary, utils, wave
     from neuralmech_amd.scalar import ScalarWave
 
     n, h, N = (44, 52), (1e-3, 1e-3), 150
-    dt = 0.7 * wave.stable_dt(h, 6000.0, 2)
-    sim = ScalarWave(n, h, N, dt, precision="float64", space_order=2, wavespeed=6000.0, density=2700.0)
+    dt = 0.7 * wave.stable_dt(h, 6000.0, order)
+    sim = ScalarWave(n, h, N, dt, precision="float64", space_order=order, wavespeed=6000.0, density=2700.0)
     sim = replace(sim, damping=boundary.sponge(sim, None, 6, 0.1))
     t = np.arange(N) * dt
     from neuralmech_amd import signals
