@@ -222,7 +222,8 @@ __global__ __launch_bounds__(256) void ac_kernel(const AcArgs<T> a)
 #pragma unroll
         for (int m = 1; m <= K; m++) {
             const T* cwm = a.cw + (m - 1) * cst;
-            cwv[m - 1] = ld0(cwm, o, in);
+            // bonds of the last interior columns reach up to K-1 columns past the nodes (< ld)
+            cwv[m - 1] = ld0(cwm, o, j < n1 + K - 1);
             cev[m - 1] = __shfl_down(cwv[m - 1], m);
             if (lane >= 64 - m && in) cev[m - 1] = cwm[o + m];
             cnv[m - 1] = CN[m - 1][0];
