@@ -98,7 +98,7 @@ class Sensors:
 
 def measure(sim, sources: Sequence[Source], indicator, sensors: Sensors) -> list:
     """Traces ``(N, R)`` of every source, all shots batched in one launch per time step.  A density
-    indicator other than 1 (or an ``AcousticWave``) runs the variable mass / stiffness model."""
+    indicator (or an ``AcousticWave``) runs the variable mass / stiffness model."""
     from .acoustic import AcousticWave, measure_indicator
 
     if isinstance(sim, AcousticWave) or not _unit_indicator(indicator):

@@ -26,10 +26,10 @@ def grid_coords(Nx, dx, dtype=np.float64):
 
 
 def _unit_indicator(indicator) -> bool:
-    """True when there is no density indicator to apply (None or 1 everywhere)."""
-    if indicator is None:
-        return True
-    return bool(torch.all(torch.as_tensor(indicator) == 1))
+    """True when there is no density indicator (None): the constant-density kernels run.  Any
+    indicator array, ones included, selects the variable-density form, so that runs with and
+    without voids share one discretisation (``projects/18_fwi/2D_tfm.py:29-30`` subtracts them)."""
+    return indicator is None
 
 
 def _setup(sim, sources, sensors, nshots):
@@ -53,8 +53,7 @@ def _setup(sim, sources, sensors, nshots):
 def simulate(sim, source=None, indicator=None, sensors=None, record_every: Optional[int] = None):
     """Run ``sim.N`` steps from rest.
 
-    A density ``indicator`` other than 1 (or an ``AcousticWave``) runs the variable mass / stiffness
-    model.  Returns ``u`` (final field, node array on the device), ``(u, traces)`` with ``sensors``
+    A density ``indicator`` (or an ``AcousticWave``) runs the variable mass / stiffness model.  Returns ``u`` (final field, node array on the device), ``(u, traces)`` with ``sensors``
     (``traces`` of shape ``(N, R)``), ``(u, snaps)`` with ``record_every`` (host arrays every that many
     steps), or ``(u, traces, frames)`` with both.
     """
