@@ -426,6 +426,71 @@ def acoustic_main(args):
     print(json.dumps(out))
 
 
+def plate_main(args):
+    """The plate FWI of projects/18_fwi (plate.py:19-47, 2D_fwi.py:21-43): 512 x 512 nodes, 32
+    elements on the top edge, a void indicator measured at order 8, every fourth element firing;
+    one iteration = the density-indicator misfit gradient of all 8 shots at order 4 (batched
+    forward recording the reconstruction strip + reconstruction adjoint), timed over a few
+    iterations of plain gradient descent."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from neuralmech_amd import geometry, signals, utils, wave
+    from neuralmech_amd.scalar import ScalarWave
+
+    Nx, L, T, safety = (512, 512), (1.0, 1.0), 2.5, 0.9
+    dx = tuple(l / (n - 3) for l, n in zip(L, Nx))
+    elements = utils.line((0.1, L[1]), (0.9, L[1]), 32)
+
+    def simulation(order):
+        N = math.ceil(T / (safety * wave.stable_dt(dx, 1.0, order))) + 1
+        sim = ScalarWave(Nx, dx, N, T / (N - 1), (4, 64), precision="float32", space_order=order, wavespeed=1.0,
+                         density=1.0)
+        t = np.linspace(0, T, N)
+        return sim, t, utils.shots(sim, elements, signals.sineburst(t, 1.0, 20.0, 2)), utils.Sensors(sim, elements)
+
+    sim_obs, t_obs, src_obs, sens_obs = simulation(8)
+    sim, t, src, sens = simulation(4)
+    coords = wave.grid_coords(Nx, dx)
+    void = np.zeros(Nx, bool)
+    for k, y in enumerate((0.15, 0.25, 0.35, 0.45)):  # four voids stacked along the array axis
+        void |= geometry.circle(coords, (0.5, y), 0.02 + 0.01 * k)
+    truth = np.where(void, 1e-4, 1.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    observed = utils.measure(sim_obs, src_obs[::4], truth, sens_obs)
+    torch.cuda.synchronize()
+    t_meas = time.perf_counter() - t0
+    observed = [utils.resample(r, t_obs[1], t[1], sim.N) for r in observed]
+    gamma = torch.ones(Nx, dtype=torch.float64, device="cuda")
+    utils.misfit_gradient(sim, src[::4][:1], gamma, sens, observed[:1])  # warm-up
+    iters = max(1, args.plate_iters)
+    costs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        J, g = utils.misfit_gradient(sim, src[::4], gamma, sens, observed)
+        costs.append(J)
+        gamma = (gamma - 0.1 * g / g.abs().max()).clamp(1e-3, 1.0)
+    torch.cuda.synchronize()
+    t_it = (time.perf_counter() - t0) / iters
+    cells = (Nx[0] - 2) * (Nx[1] - 2) * 8 * sim.N
+    out = {
+        "metric": "plate FWI ms per iteration (density-indicator misfit gradient, 8 shots)",
+        "value": round(t_it * 1e3, 2), "unit": "ms/iteration", "n_gpus": 1, "steps": iters, "warmup": 1,
+        "ms_per_step": round(t_it / sim.N * 1e3, 4), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (four circular voids, gamma 1e-4; sine-burst array)",
+        "config": {"workload": f"plate FWI: {Nx[0]}x{Nx[1]} nodes, order 4 inversion / order 8 measurement, "
+                               f"32 elements, 8 shots x {sim.N} steps", "steps_per_shot": sim.N,
+                   "measurement_order8_s": round(t_meas, 3)},
+        "gcell_updates_per_s_forward_plus_adjoint": round(2 * cells / t_it / 1e9, 2),
+        "misfit_per_iteration": [round(c, 6) for c in costs],
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -435,7 +500,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch shape")
     ap.add_argument("--workload", choices=["forward", "fwi", "copy", "3d", "uneven", "roofline", "scaling",
-                                           "acoustic"],
+                                           "acoustic", "plate"],
                     default="forward")
     ap.add_argument("--launch", default="", help="force VARIANT,ROWS instead of tuning")
     ap.add_argument("--two-step", type=int, default=0, help="forward: two steps per launch, ROWS output rows per wave")
@@ -454,6 +519,7 @@ def main():
     ap.add_argument("--scaling-max", type=int, default=100000, help="scaling: largest grid side")
     ap.add_argument("--scaling-steps", type=int, default=400, help="scaling: steps timed per grid")
     ap.add_argument("--acoustic-steps", type=int, default=1000, help="acoustic: forward / adjoint steps timed")
+    ap.add_argument("--plate-iters", type=int, default=5, help="plate: gradient iterations timed")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the shot runs on a single GPU and this build has no "
@@ -479,6 +545,9 @@ def main():
         return
     if args.workload == "acoustic":
         acoustic_main(args)
+        return
+    if args.workload == "plate":
+        plate_main(args)
         return
 
     import torch

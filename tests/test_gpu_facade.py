@@ -117,3 +117,37 @@ def test_density_indicator_misfit_gradient_fp64(order):
         gm[cell] -= eps
         fd = (utils.misfit(sim, sources, gp, sensors, obs) - utils.misfit(sim, sources, gm, sensors, obs)) / (2 * eps)
         assert fd == pytest.approx(float(grad[cell]), rel=1e-5, abs=1e-6 * float(grad.abs().max()))
+
+
+def test_plate_fwi_flow_with_density_indicator():
+    """projects/18_fwi/plate.py + 2D_fwi.py at a reduced size: a void measured at order 8 with a
+    density indicator, resampled onto the order-4 inversion grid, then one indicator gradient of
+    all shots; the steepest-descent step lowers the misfit."""
+    import math
+
+    from neuralmech_amd import geometry, signals, utils, wave
+    from neuralmech_amd.scalar import ScalarWave
+
+    Nx, L, T = (96, 96), (1.0, 1.0), 1.6
+    dx = tuple(l / (n - 3) for l, n in zip(L, Nx))
+    elements = utils.line((0.1, L[1]), (0.9, L[1]), 8)
+
+    def simulation(order):
+        N = math.ceil(T / (0.9 * wave.stable_dt(dx, 1.0, order))) + 1
+        sim = ScalarWave(Nx, dx, N, T / (N - 1), (4, 64), precision="float32", space_order=order, wavespeed=1.0,
+                         density=1.0)
+        t = np.linspace(0, T, N)
+        return sim, t, utils.shots(sim, elements, signals.sineburst(t, 1.0, 8.0, 2)), utils.Sensors(sim, elements)
+
+    sim_obs, t_obs, src_obs, sens_obs = simulation(8)
+    sim, t, src, sens = simulation(4)
+    void = geometry.circle(wave.grid_coords(Nx, dx), (0.5, 0.45), 0.12)
+    truth = np.where(void, 1e-4, 1.0)
+    observed = utils.measure(sim_obs, src_obs[::2], truth, sens_obs)
+    observed = [utils.resample(r, t_obs[1], t[1], sim.N) for r in observed]
+    gamma = torch.ones(Nx, dtype=torch.float64, device="cuda")
+    J0, g = utils.misfit_gradient(sim, src[::2], gamma, sens, observed)
+    assert J0 > 0 and torch.isfinite(g).all() and float(g.abs().max()) > 0
+    step = 0.05 / float(g.abs().max())
+    J1 = utils.misfit(sim, src[::2], (gamma - step * g).clamp(1e-3, 1.0), sens, observed)
+    assert J1 < J0
