@@ -491,6 +491,91 @@ def plate_main(args):
     print(json.dumps(out))
 
 
+def tfm_main(args):
+    """The TFM imager of projects/18_fwi (2D_tfm.py, 2D_tfm_scaling.py): the plate's full matrix
+    capture (32 elements, order 8, voids minus no voids) from this solver, its analytic signal,
+    then the delay-and-sum image at sides 32 .. --tfm-max (powers of two), the reciprocal kernel
+    form; the array form and the NumPy restatement of it on the host (up to 256^2, as the
+    reference's cpu run) beside it."""
+    import math
+
+    import numpy as np
+    import torch
+
+    from neuralmech_amd import geometry, signals, tfm, utils, wave
+    from neuralmech_amd.scalar import ScalarWave
+    from oracle import tfm_ref
+
+    Nx, L, T, safety, E = (512, 512), (1.0, 1.0), 2.5, 0.9, 32
+    dx = tuple(l / (n - 3) for l, n in zip(L, Nx))
+    elements = utils.line((0.1, L[1]), (0.9, L[1]), E)
+    N = math.ceil(T / (safety * wave.stable_dt(dx, 1.0, 8))) + 1
+    sim = ScalarWave(Nx, dx, N, T / (N - 1), (4, 64), precision="float32", space_order=8, wavespeed=1.0, density=1.0)
+    t = np.linspace(0, T, N)
+    sources = utils.shots(sim, elements, signals.sineburst(t, 1.0, 20.0, 2))
+    sensors = utils.Sensors(sim, elements)
+    coords = wave.grid_coords(Nx, dx)
+    void = np.zeros(Nx, bool)
+    for k, yv in enumerate((0.15, 0.25, 0.35, 0.45)):
+        void |= geometry.circle(coords, (0.5, yv), 0.02 + 0.01 * k)
+    truth = np.where(void, 1e-4, 1.0)
+    torch.cuda.synchronize()
+    t0_ = time.perf_counter()
+    scattered = torch.stack(utils.measure(sim, sources, truth, sensors))
+    scattered -= torch.stack(utils.measure(sim, sources, np.ones(Nx), sensors))
+    torch.cuda.synchronize()
+    t_fmc = time.perf_counter() - t0_
+    h = tfm.analytic_signal(scattered.double(), axis=1).transpose(1, 2).contiguous().to(torch.complex64)
+    t0 = 2 / (2 * 20.0)
+    dt = t[1]
+    sizes = [2 ** k for k in range(5, int(math.log2(args.tfm_max)) + 1)]
+    rows = []
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for n in sizes:
+        x, y = torch.meshgrid(torch.linspace(0, L[0], n, device="cuda"), torch.linspace(0, L[1], n, device="cuda"),
+                              indexing="ij")
+        img = tfm.tfm_kernel(h, elements, t0, dt, 1.0, x, y)
+        times = []
+        for _ in range(3):
+            e0.record()
+            tfm.tfm_kernel(h, elements, t0, dt, 1.0, x, y)
+            e1.record()
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1) / 1e3)
+        row = {"pixels": n * n, "kernel_s": min(times)}
+        if n <= 256:
+            ref = tfm_ref.tfm_reciprocal(h.cpu().numpy(), np.asarray(elements, np.float32), np.float32(t0),
+                                         np.float32(dt), np.float32(1.0), x.cpu().numpy(), y.cpu().numpy())
+            row["kernel_rel_diff"] = float(np.abs(img.cpu().numpy() - ref).max() / ref.max())
+        rows.append(row)
+    n_big = sizes[-1]
+    pairs = E * (E + 1) // 2
+    best = rows[-1]["kernel_s"]
+    # host baseline: the reference's array form in NumPy at 256^2 (its cpu run stops there)
+    xs, ys = np.meshgrid(np.linspace(0, 1, 256), np.linspace(0, 1, 256), indexing="ij")
+    hc = h.cpu().numpy()
+    t0_ = time.perf_counter()
+    tfm_ref.tfm_all_pairs(hc, np.asarray(elements), t0, dt, 1.0, xs, ys)
+    t_cpu = time.perf_counter() - t0_
+    out = {
+        "metric": "TFM image time (32-element full matrix capture, reciprocal delay and sum)",
+        "value": round(best * 1e3, 3), "unit": "ms/image", "n_gpus": 1, "steps": 3, "warmup": 1,
+        "ms_per_step": round(best * 1e3, 3), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (plate with four voids, this solver's order-8 full matrix capture)",
+        "config": {"workload": f"TFM {n_big}^2 pixels, {E} elements, {h.shape[2]} samples", "fmc_s": round(t_fmc, 3)},
+        "gpair_samples_per_s": round(n_big * n_big * pairs / best / 1e9, 2),
+        "scaling_rows": rows,
+        "cpu_baseline": {"value": round(t_cpu * 1e3, 1), "unit": "ms/image", "cores": 1, "kind": "port",
+                         "sample": "NumPy array form of tfm.py:39-53 (all pairs), 256^2 pixels, float64"},
+    }
+    if args.csv_dir:
+        from neuralmech_amd import harness
+
+        harness.save_csv(os.path.join(args.csv_dir, "tfm_scaling_kernel.csv"), x=[r["pixels"] for r in rows],
+                         y=[r["kernel_s"] for r in rows])
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -500,7 +585,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-tune", action="store_true", help="use the default launch shape")
     ap.add_argument("--workload", choices=["forward", "fwi", "copy", "3d", "uneven", "roofline", "scaling",
-                                           "acoustic", "plate"],
+                                           "acoustic", "plate", "tfm"],
                     default="forward")
     ap.add_argument("--launch", default="", help="force VARIANT,ROWS instead of tuning")
     ap.add_argument("--two-step", type=int, default=0, help="forward: two steps per launch, ROWS output rows per wave")
@@ -520,6 +605,7 @@ def main():
     ap.add_argument("--scaling-steps", type=int, default=400, help="scaling: steps timed per grid")
     ap.add_argument("--acoustic-steps", type=int, default=1000, help="acoustic: forward / adjoint steps timed")
     ap.add_argument("--plate-iters", type=int, default=5, help="plate: gradient iterations timed")
+    ap.add_argument("--tfm-max", type=int, default=4096, help="tfm: largest image side")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the shot runs on a single GPU and this build has no "
@@ -548,6 +634,9 @@ def main():
         return
     if args.workload == "plate":
         plate_main(args)
+        return
+    if args.workload == "tfm":
+        tfm_main(args)
         return
 
     import torch

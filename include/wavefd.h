@@ -366,6 +366,18 @@ int wfd_ac_backward(const wfd_grid* g, void* u_a, void* u_b, void* mu_a, void* m
                     const wfd_points* strip, const wfd_points* mu_src, int64_t nsteps,
                     void* g_work, double* G, wfd_stream_t stream);
 
+/* ------------------------------------------------------------- total focusing method */
+/* Delay and sum of a full matrix capture onto P pixels: h = analytic traces, complex64 as
+ * (re, im) float pairs, [E][E][Nt] (emitter, receiver, sample); element e at (xe[e], ye[e]),
+ * pixel p at (xp[p], yp[p]); image[p] = |sum_pairs h_er((t0 + (|p-x_e| + |p-x_r|)/c) / dt)|
+ * with linear interpolation in time.  reciprocal = 1: pairs r >= e, r > e weighted 2, samples
+ * with k >= Nt-1 skipped (the tfm RawKernel, projects/18_fwi/tfm.py:12-36); reciprocal = 0: all
+ * E^2 pairs, k clamped to Nt-2 and skipped there (the array form, tfm.py:39-53).  fp32
+ * arithmetic; 1 <= E <= 64; t0 >= 0.  All arrays on the device. */
+int wfd_tfm(const void* h, const float* xe, const float* ye, int32_t E, int32_t Nt, double dt,
+            double t0, double c, const float* xp, const float* yp, int64_t P, float* image,
+            int32_t reciprocal, wfd_stream_t stream);
+
 /* ------------------------------------------------------------------- measurement probe */
 /* Peak fp32 FMA rate: `blocks` x 256 threads each run `n` iterations of 4 dependent fused
  * multiply-adds (8 flop) in registers and write one float to out[blocks * 256].  Time it with
