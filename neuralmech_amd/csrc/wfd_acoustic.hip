@@ -27,7 +27,6 @@
 namespace wfd {
 namespace {
 
-constexpr int AC_RI = 8;  // rows marched per wave
 
 template <typename T>
 struct AcArgs {
@@ -168,8 +167,8 @@ __device__ __forceinline__ T ac_q(const T (&cn)[K], const T (&cs)[K], const T (&
 }
 
 // Launch: 8 * ceil(aux / 8) aux blocks then gx * gy * nshots tiles (see step_block); a tile is 64
-// columns x 4 waves x AC_RI rows.  K = order / 2 bonds per side and axis.
-template <typename T, int K, bool SPONGE, bool BWD>
+// columns x 4 waves x RI rows.  K = order / 2 bonds per side and axis.
+template <typename T, int K, bool SPONGE, bool BWD, int RI>
 __global__ __launch_bounds__(256) void ac_kernel(const AcArgs<T> a)
 {
     const StepBlock bp = step_block(a.nga + a.ngb + a.nsc, a.gx, a.gy);
@@ -187,7 +186,7 @@ __global__ __launch_bounds__(256) void ac_kernel(const AcArgs<T> a)
     const int64_t n0 = a.n0, n1 = a.n1, ld = a.ld, cst = a.cstride, pst = a.pstride;
     const int64_t shot = bp.z;
     const int64_t j0 = (int64_t)bp.x * 64, j = j0 + lane;
-    const int64_t r0 = 1 + ((int64_t)bp.y * 4 + wave) * AC_RI;
+    const int64_t r0 = 1 + ((int64_t)bp.y * 4 + wave) * RI;
     if (r0 > n0 - 2) return;  // whole wave
     const bool in = j < n1;
     const bool inter = j >= 1 && j <= n1 - 2;
@@ -208,7 +207,7 @@ __global__ __launch_bounds__(256) void ac_kernel(const AcArgs<T> a)
 #pragma unroll
         for (int t = 0; t < m; t++) CN[m - 1][t] = ld0(a.cn + (m - 1) * cst, (r0 + t) * ld + j, in);
 #pragma unroll
-    for (int s = 0; s < AC_RI; s++) {
+    for (int s = 0; s < RI; s++) {
         const int64_t r = r0 + s;
         if (r > n0 - 2) break;  // whole wave
         const int64_t o = r * ld + j;
@@ -300,16 +299,25 @@ __global__ __launch_bounds__(256) void ac_kernel(const AcArgs<T> a)
 }
 
 template <typename T, int K, bool SPONGE, bool BWD>
-hipError_t ac_launch_k(const AcArgs<T>& a, uint32_t tiles, hipStream_t s)
+hipError_t ac_launch_k(AcArgs<T> a, bool tiles_on, hipStream_t s)
 {
+    // rows per wave: 8, or 4 when that leaves fewer than ~8 waves per SIMD of the chip
+    // (256 CUs x 4 SIMDs), so that small grids still have loads in flight
+    const uint64_t waves8 = (uint64_t)a.gx * 4 * (uint64_t)((a.n0 - 2 + 31) / 32) * (uint64_t)a.nshots;
+    const int ri = waves8 >= 8192 ? 8 : 4;
+    a.gy = (uint32_t)((a.n0 - 2 + 4 * ri - 1) / (4 * ri));
+    const uint32_t tiles = tiles_on ? a.gx * a.gy * (uint32_t)a.nshots : 0u;
     const uint32_t blocks = step_grid(tiles, a.nga + a.ngb + a.nsc);
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL((ac_kernel<T, K, SPONGE, BWD>), dim3(blocks), dim3(256), 0, s, a);
+    if (ri == 8)
+        hipLaunchKernelGGL((ac_kernel<T, K, SPONGE, BWD, 8>), dim3(blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((ac_kernel<T, K, SPONGE, BWD, 4>), dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 template <typename T, int K>
-hipError_t ac_launch_kk(const AcArgs<T>& a, uint32_t tiles, bool sp, bool bwd, hipStream_t s)
+hipError_t ac_launch_kk(const AcArgs<T>& a, bool tiles, bool sp, bool bwd, hipStream_t s)
 {
     if (bwd) return sp ? ac_launch_k<T, K, true, true>(a, tiles, s) : ac_launch_k<T, K, false, true>(a, tiles, s);
     return sp ? ac_launch_k<T, K, true, false>(a, tiles, s) : ac_launch_k<T, K, false, false>(a, tiles, s);
@@ -325,14 +333,12 @@ hipError_t ac_launch(AcArgs<T> a, bool tiles_on, bool bwd, hipStream_t s)
     a.ngb = a.gb_f ? nb(a.gb, a.nshots) : 0;
     a.nsc = a.sc.count ? nb(a.sc, a.nshots) : 0;
     a.gx = (uint32_t)((a.n1 + 63) / 64);
-    a.gy = (uint32_t)((a.n0 - 2 + 4 * AC_RI - 1) / (4 * AC_RI));
-    const uint32_t tiles = tiles_on ? a.gx * a.gy * (uint32_t)a.nshots : 0u;
     const bool sp = a.br != nullptr;
     switch (a.K) {
-        case 1: return ac_launch_kk<T, 1>(a, tiles, sp, bwd, s);
-        case 2: return ac_launch_kk<T, 2>(a, tiles, sp, bwd, s);
-        case 3: return ac_launch_kk<T, 3>(a, tiles, sp, bwd, s);
-        default: return ac_launch_kk<T, 4>(a, tiles, sp, bwd, s);
+        case 1: return ac_launch_kk<T, 1>(a, tiles_on, sp, bwd, s);
+        case 2: return ac_launch_kk<T, 2>(a, tiles_on, sp, bwd, s);
+        case 3: return ac_launch_kk<T, 3>(a, tiles_on, sp, bwd, s);
+        default: return ac_launch_kk<T, 4>(a, tiles_on, sp, bwd, s);
     }
 }
 

@@ -226,3 +226,42 @@ def test_autograd_function_matches_material_gradient():
         fd = (jp - jm) / (2 * eps)
         g = float(st.grad[i, j])
         assert abs(fd - g) <= 1e-5 * max(abs(fd), 1e-3 * float(st.grad.abs().max())), (i, j, fd, g)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_bitexact_on_launches_with_eight_rows_per_wave(dtype):
+    """16 shots of 258 x 520 nodes: enough waves that each wave marches 8 rows (small launches
+    march 4); shots 0 and 15 against the restatement, forward and backward, order 4."""
+    from neuralmech_amd import acoustic as ac
+    from neuralmech_amd.solver import PointSet, WaveGrid
+
+    shape, order, N, h = (258, 520), 4, 40, (0.05, 0.04)
+    tdt = getattr(torch, dtype)
+    m = _smooth(shape, 9, 0.6, 2.5)
+    s = _smooth(shape, 10, 0.3, 1.8)
+    dt = 0.45 * min(h) / float(np.sqrt((s / m).max())) / math.sqrt(2)
+    beta = (_profile(shape[0], 10, 0.1), _profile(shape[1], 12, 0.1))
+    grid = WaveGrid(shape, h, dt, order, tdt, nshots=16, device="cuda")
+    med = ac.AcMedium(grid, m, s, "neumann", beta)
+    src_nodes, rec_nodes = [(100, 250), (30, 400)], [(40, 40), (200, 300), (250, 510)]
+    t = np.arange(N) * dt
+    sig = np.stack([slice_ref.ricker(t, 1.0 / (10 * dt)), slice_ref.ricker(t - 4 * dt, 1.0 / (12 * dt))], 1)
+    src, rec = PointSet(grid, src_nodes), PointSet(grid, rec_nodes)
+    sn = med.strip_nodes()
+    strip = PointSet(grid, [tuple(int(c) for c in p) for p in sn])
+    sg = torch.as_tensor(sig)
+    tr, store, state = ac.forward(grid, med, N, src, sg, rec, strip)
+    dJ = (tr - 0.5 * tr).to(tr.dtype)
+    G, mu_src = ac.backward(grid, med, N, state, src, sg, rec, dJ, strip, store)
+    torch.cuda.synchronize()
+    npdt = np.float32 if dtype == "float32" else np.float64
+    k, cn, cw = (f.cpu().numpy().astype(npdt) for f in med.bonds())
+    prof = tuple(med.profiles)
+    otr, ostore, last = ar.forward_np(k, cn, cw, N, prof, src_nodes, sig, rec_nodes, sn, dtype=npdt)
+    adj = np.zeros((N + 1, len(rec_nodes)), npdt)
+    adj[1:] = -dJ[0].cpu().numpy()
+    OG, omu = ar.backward_np(k, cn, cw, N, last, prof, src_nodes, sig, rec_nodes, adj, sn, ostore, dtype=npdt)
+    for shot in (0, 15):
+        assert np.array_equal(tr[shot].cpu().numpy(), otr)
+        assert np.array_equal(G[:, shot].cpu().numpy(), OG)
+        assert np.array_equal(mu_src[shot].cpu().numpy(), omu)
