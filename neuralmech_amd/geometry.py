@@ -27,3 +27,27 @@ def circle(coords, center, radius):
     c = [np.asarray(a.get() if hasattr(a, "get") else a, dtype=np.float64) for a in coords]
     r2 = sum((a - float(m)) ** 2 for a, m in zip(c, center))
     return r2 <= float(radius) ** 2 * (1 + 1e-12)
+
+
+def stacked_circles(coords, num: int, max_radius: float, span, position: float, axis: int = 1,
+                    ratio: float = 0.5, order: str = "ascending"):
+    """Mask of ``num`` discs stacked along ``axis`` (``projects/18_fwi/plate.py:56-59``): centres
+    evenly spaced over ``span`` = (start, end) (the middles of ``num`` equal segments) at
+    ``position`` on the other axis, radii from ``ratio * max_radius`` to ``max_radius`` in
+    ``order`` ("ascending" along the axis, or "descending").  cuwave's exact placement is not
+    available; this is this build's reading of the call site's arguments."""
+    import numpy as _np
+
+    radii = max_radius * _np.linspace(ratio, 1.0, int(num))
+    if order == "descending":
+        radii = radii[::-1]
+    lo, hi = (float(v) for v in span)
+    centres = lo + (hi - lo) * (_np.arange(int(num)) + 0.5) / int(num)
+    mask = None
+    for c, r in zip(centres, radii):
+        centre = [0.0, 0.0]
+        centre[axis] = c
+        centre[1 - axis] = float(position)
+        m = circle(coords, centre, r)
+        mask = m if mask is None else mask | m
+    return mask

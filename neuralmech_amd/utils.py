@@ -170,3 +170,9 @@ def resample(record, dt_from: float, dt_to: float, N: int):
     if is_t:
         return torch.as_tensor(out, dtype=record.dtype, device=record.device)
     return out
+
+
+def threshold(gamma, t: float = 0.5):
+    """Binary indicator: 1 where ``gamma >= t``, 0 elsewhere (``topopt_acoustic2D.py:258``)."""
+    g = torch.as_tensor(gamma.get() if hasattr(gamma, "get") else gamma)
+    return (g >= t).to(g.dtype if g.is_floating_point() else torch.float64)

@@ -117,3 +117,28 @@ def test_save_csv_writes_the_reference_columns(tmp_path):
 
     with pytest.raises(ValueError):
         harness.save_csv(tmp_path / "bad.csv", x=[1, 2], y=[1])
+
+
+def test_fwi_driver_helpers():
+    from neuralmech_amd import evals, geometry, optimization
+
+    truth = np.ones((20, 20))
+    truth[5:9, 5:9] = 1e-4
+    g = truth.copy()
+    assert evals.l2_error(g, truth) == 0.0 and evals.f1_score(g, truth) == 1.0
+    g2 = np.ones((20, 20))
+    assert evals.f1_score(g2, truth) == 0.0 and evals.l2_error(g2, truth) > 0
+    # Adam on f(x) = |x - 3|^2 / 2 converges; the first step has size lr in every component
+    opt = optimization.Adam(lr=0.1)
+    x = torch.zeros(5, dtype=torch.float64)
+    x1 = opt.step(x, x - 3.0)
+    assert torch.allclose(x1, torch.full_like(x, 0.1))
+    for _ in range(400):
+        x1 = opt.step(x1, x1 - 3.0)
+    assert float((x1 - 3.0).abs().max()) < 0.05
+    coords = wave.grid_coords((64, 64), (1 / 61, 1 / 61))
+    m = geometry.stacked_circles(coords, 4, 0.05, (0.1, 0.5), 0.5, axis=1, ratio=0.5, order="ascending")
+    assert m.shape == (64, 64) and m.sum() > 0
+    y = coords[1][m]
+    assert y.min() >= 0.1 - 0.05 and y.max() <= 0.5 + 0.05
+    assert torch.equal(utils.threshold(torch.tensor([0.2, 0.5, 0.9])), torch.tensor([0.0, 1.0, 1.0]))
