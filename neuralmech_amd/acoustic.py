@@ -74,10 +74,32 @@ def _harm(a, b):
     return 2.0 * a * b / (a + b)
 
 
+POWER_MEAN_Q = 8  # exponent -q of the segment mean of the stiffness on bonds of orders >= 4
+
+
+def _segment_mean(sp, lo, hi, q: int):
+    """Power mean ``((1/n) sum s^-q)^(-1/q)`` of ``sp`` over rows ``lo[b] .. hi[b]`` (inclusive, all
+    columns): rows gathered per bond, scaled by the largest value first so that ``s^-q`` stays in
+    range (the mean is homogeneous of degree 1, so the scale passes through)."""
+    ref = sp.detach().max()
+    n = (hi - lo + 1).to(sp.dtype)[:, None]
+    acc = torch.zeros((len(lo), sp.shape[1]), dtype=sp.dtype, device=sp.device)
+    for t in range(int((hi - lo).max()) + 1):
+        rows = torch.minimum(lo + t, hi)
+        valid = (lo + t <= hi).to(sp.dtype)[:, None]
+        acc = acc + valid * (sp[rows] / ref) ** (-q)
+    return ref * (acc / n) ** (-1.0 / q)
+
+
 def _axis_bonds(s, h: float, order: int, closure: str):
     """Bond coefficients along axis 0 of the node array ``s``: ``K`` arrays of shape
     ``(n0 + K - 1, n1)``, entry ``[m-1][i, j]`` the bond between (i-m, j) and (i, j).
 
+    Order 2: the harmonic mean of the two ends.  Orders 4..8: the power mean with exponent -8 of
+    every node on the segment between the ends (a reflected bond's segment runs through the wall);
+    the negative bonds of these stencils (a_2, a_4 < 0) then never outweigh the shorter bonds they
+    span (their segment mean is at most (3/2)^(1/8) of any sub-segment's), which keeps the operator
+    negative semi-definite at any contrast; with constant stiffness the mean is that constant.
     Zero ring: every bond with an interior end, the stiffness beyond the ring continued from it.
     No flux: bonds between interior nodes, plus the bonds the even reflection of the field about the
     first / last interior node adds near the walls (a node's bond to a mirror image of another node
@@ -88,39 +110,50 @@ def _axis_bonds(s, h: float, order: int, closure: str):
     a = central_weights(order)
     n0, n1 = s.shape
     E0 = n0 + K - 1
-    cols = slice(1, n1 - 1)
+    cols = torch.arange(1, n1 - 1, device=s.device)[None, :]
+    dev = s.device
+
+    def coef(rows_n, rows_s, lo, hi, sp, off, w):
+        # bonds between rows_n and rows_s (node rows of sp shifted by off), segment rows lo..hi
+        if K == 1:
+            v = _harm(sp[rows_n + off][:, 1:n1 - 1], sp[rows_s + off][:, 1:n1 - 1])
+        else:
+            v = _segment_mean(sp[:, 1:n1 - 1], lo + off, hi + off, POWER_MEAN_Q)
+        return w[:, None] / (h * h) * v
+
     out = []
     if closure == "dirichlet":
         sp = torch.cat([s[:1].expand(K, n1), s, s[-1:].expand(K, n1)], 0)  # row i at i + K
         for m in range(1, K + 1):
-            i = torch.arange(1, n0 - 1 + m, device=s.device)
-            v = a[m - 1] / (h * h) * _harm(sp[i - m + K][:, cols], sp[i + K][:, cols])
-            f = torch.zeros((E0, n1), dtype=s.dtype, device=s.device)
-            out.append(f.index_put((i[:, None], torch.arange(1, n1 - 1, device=s.device)[None, :]), v))
+            i = torch.arange(1, n0 - 1 + m, device=dev)
+            w = torch.full((len(i),), a[m - 1], dtype=s.dtype, device=dev)
+            v = coef(i - m, i, i - m, i, sp, K, w)
+            f = torch.zeros((E0, n1), dtype=s.dtype, device=dev)
+            out.append(f.index_put((i[:, None], cols), v))
         return out
     L = n0 - 2  # last interior row; the first is 1
-    entries = {m: [] for m in range(1, K + 1)}  # distance -> [(row of the southern end, northern row, weight)]
+    entries = {m: [] for m in range(1, K + 1)}  # distance -> [(southern row, northern row, weight, lo, hi)]
     for m in range(1, K + 1):
         for i in range(1 + m, L + 1):
-            entries[m].append((i, i - m, a[m - 1]))
+            entries[m].append((i, i - m, a[m - 1], i - m, i))
     for i in range(2, min(K, L) + 1):  # near the first interior row: images 2 - (i - m)
         for m in range(i, K + 1):
             im = 2 - i + m
             if im > i and im <= L:
-                entries[im - i].append((im, i, a[m - 1]))
+                entries[im - i].append((im, i, a[m - 1], 1, im))
     for i in range(max(L - K + 1, 1), L):  # near the last interior row: images 2L - (i + m)
         for m in range(L + 1 - i, K + 1):
             im = 2 * L - i - m
             if 1 <= im < i:
-                entries[i - im].append((i, im, a[m - 1]))
+                entries[i - im].append((i, im, a[m - 1], im, L))
     for d in range(1, K + 1):
-        f = torch.zeros((E0, n1), dtype=s.dtype, device=s.device)
+        f = torch.zeros((E0, n1), dtype=s.dtype, device=dev)
         if entries[d]:
-            rs = torch.as_tensor([e[0] for e in entries[d]], device=s.device)
-            rn = torch.as_tensor([e[1] for e in entries[d]], device=s.device)
-            w = torch.as_tensor([e[2] for e in entries[d]], dtype=s.dtype, device=s.device)[:, None]
-            v = w / (h * h) * _harm(s[rn][:, cols], s[rs][:, cols])
-            f = f.index_put((rs[:, None], torch.arange(1, n1 - 1, device=s.device)[None, :]), v, accumulate=True)
+            col = lambda k, dt=torch.long: torch.as_tensor([e[k] for e in entries[d]], dtype=dt, device=dev)  # noqa: E731
+            rs, rn, lo, hi = col(0), col(1), col(3), col(4)
+            w = col(2, s.dtype)
+            v = coef(rn, rs, lo, hi, s, 0, w)
+            f = f.index_put((rs[:, None], cols), v, accumulate=True)
         out.append(f)
     return out
 
