@@ -704,6 +704,17 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "step2d_kernel<float, K=4, VARC, SPONGE>", "bytes_per_cell": 16},
     }
+    # error of the timed fp32 run against the same shot run in fp64 on the GPU (outside the timing)
+    w64 = workload(nsteps=max(nsteps, 4000), dtype=torch.float64)
+    tr64, st64 = solver.run(w64["grid"], w64["medium"], nsteps, src=w64["src"], sig=w64["sig"], rec=w64["rec"])
+    torch.cuda.synchronize()
+    t64 = tr64[:, args.warmup:nsteps].double()
+    f32 = g.nodes(state[1])[0].double()
+    f64 = w64["grid"].nodes(st64[1])[0].double()
+    out["fp64_gpu_oracle"] = {
+        "traces_rel_l2": float(((traces.double() - t64).norm() / t64.norm()).item()),
+        "field_rel_l2": float(((f32 - f64).norm() / f64.norm()).item()),
+        "run": "the same shot in float64 on the GPU (float64 instantiation of the same kernels)"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out))
