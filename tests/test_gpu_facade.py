@@ -119,18 +119,17 @@ def test_density_indicator_misfit_gradient_fp64(order):
         assert fd == pytest.approx(float(grad[cell]), rel=1e-5, abs=1e-6 * float(grad.abs().max()))
 
 
-def test_plate_fwi_flow_with_density_indicator():
-    """projects/18_fwi/plate.py + 2D_fwi.py at a reduced size: a void measured at order 8 with a
-    density indicator, resampled onto the order-4 inversion grid, then one indicator gradient of
-    all shots; the steepest-descent step lowers the misfit."""
+def _plate_problem(Nx=(96, 96), n_elements=8):
+    """projects/18_fwi/plate.py at a reduced size: a void measured at order 8 with a density
+    indicator, resampled onto the order-4 inversion grid (every other element fires)."""
     import math
 
     from neuralmech_amd import geometry, signals, utils, wave
     from neuralmech_amd.scalar import ScalarWave
 
-    Nx, L, T = (96, 96), (1.0, 1.0), 1.6
+    L, T = (1.0, 1.0), 1.6
     dx = tuple(l / (n - 3) for l, n in zip(L, Nx))
-    elements = utils.line((0.1, L[1]), (0.9, L[1]), 8)
+    elements = utils.line((0.1, L[1]), (0.9, L[1]), n_elements)
 
     def simulation(order):
         N = math.ceil(T / (0.9 * wave.stable_dt(dx, 1.0, order))) + 1
@@ -145,9 +144,75 @@ def test_plate_fwi_flow_with_density_indicator():
     truth = np.where(void, 1e-4, 1.0)
     observed = utils.measure(sim_obs, src_obs[::2], truth, sens_obs)
     observed = [utils.resample(r, t_obs[1], t[1], sim.N) for r in observed]
-    gamma = torch.ones(Nx, dtype=torch.float64, device="cuda")
-    J0, g = utils.misfit_gradient(sim, src[::2], gamma, sens, observed)
+    return sim, src[::2], sens, observed
+
+
+def test_plate_fwi_flow_with_density_indicator():
+    """projects/18_fwi/2D_fwi.py at a reduced size: one indicator gradient of all shots; the
+    steepest-descent step lowers the misfit."""
+    from neuralmech_amd import utils
+
+    sim, src, sens, observed = _plate_problem()
+    gamma = torch.ones(sim.Nx, dtype=torch.float64, device="cuda")
+    J0, g = utils.misfit_gradient(sim, src, gamma, sens, observed)
     assert J0 > 0 and torch.isfinite(g).all() and float(g.abs().max()) > 0
     step = 0.05 / float(g.abs().max())
-    J1 = utils.misfit(sim, src[::2], (gamma - step * g).clamp(1e-3, 1.0), sens, observed)
+    J1 = utils.misfit(sim, src, (gamma - step * g).clamp(1e-3, 1.0), sens, observed)
     assert J1 < J0
+
+
+def test_penalized_lbfgs_flow():
+    """projects/9_structural_optimization/fwi_penalization.py:125-153 at a reduced size: L-BFGS on
+    misfit + total variation with the box projection; every accepted step lowers the penalised
+    objective (Armijo) and the iterate stays in [gamma_min, 1]."""
+    from neuralmech_amd import utils
+    from neuralmech_amd.optimization import Lbfgs
+    from neuralmech_amd.regularization import TotalVariation
+
+    sim, src, sens, observed = _plate_problem()
+    penalty = TotalVariation(3e-3, eps=1e-2)
+    gamma = torch.ones(sim.Nx_padded, dtype=torch.float32, device="cuda")
+    opt = Lbfgs(k=4, first_step=0.05, armijo=1e-4, shrink=0.5, min_alpha=1e-3)
+    objective = lambda g: float(utils.misfit(sim, src, g, sens, observed)) + float(penalty(g))
+    history = []
+    for _ in range(4):
+        cost, gradient = utils.misfit_gradient(sim, src, gamma, sens, observed)
+        gradient = gradient + penalty.grad(gamma)
+        total = float(cost) + float(penalty(gamma))
+        history.append(total)
+        gamma, f_new, alpha = opt.search(gamma, gradient, total, objective, lambda g: g.clamp(1e-3, 1.0))
+        assert alpha > 0 and f_new < total
+        assert float(gamma.min()) >= 1e-3 and float(gamma.max()) <= 1.0 and gamma.dtype == torch.float32
+    assert history[-1] < history[0]
+
+
+def test_neural_and_projection_ansatz_descend():
+    """projects/16_neuralopt/neuralfwi.py:99-124 at a reduced size: the misfit gradient chained
+    through the Generator (field.backward) and through the Heaviside projection gives a descent
+    direction for the network weights and for the projected design variables."""
+    from neuralmech_amd import utils
+    from neuralmech_amd.nn import Generator
+    from neuralmech_amd.regularization import Projection
+
+    gmin = 1e-3
+    sim, src, sens, observed = _plate_problem()
+    model = Generator([8, 8, 8, 4, 1], sim.Nx_padded, kernel=5, output_bias=3.0).to("cuda")
+    field = model()
+    gamma = gmin + (1.0 - gmin) * field.detach()[0, 0]
+    J0, gradient = utils.misfit_gradient(sim, src, gamma, sens, observed)
+    field.backward(torch.as_tensor((1.0 - gmin) * gradient).to(field.dtype)[None, None])
+    grads = [p.grad.detach().clone() for p in model.parameters()]
+    assert all(torch.isfinite(g).all() for g in grads) and sum(float(g.abs().sum()) for g in grads) > 0
+    norm = max(float(g.abs().max()) for g in grads)
+    with torch.no_grad():
+        for p, g in zip(model.parameters(), grads):
+            p -= (0.02 / norm) * g
+        gamma1 = gmin + (1.0 - gmin) * model()[0, 0]
+    assert utils.misfit(sim, src, gamma1, sens, observed) < J0
+
+    proj = Projection(3.0, 0.5)
+    x = torch.full(sim.Nx_padded, 0.9, dtype=torch.float32, device="cuda")
+    J0, gradient = utils.misfit_gradient(sim, src, gmin + (1.0 - gmin) * proj(x), sens, observed)
+    gx = proj.grad(x, (1.0 - gmin) * gradient)
+    x1 = (x - (0.05 / float(gx.abs().max())) * gx).clamp(0.0, 1.0)
+    assert utils.misfit(sim, src, gmin + (1.0 - gmin) * proj(x1), sens, observed) < J0
