@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from oracle import slice_ref
-from tests.wave_cases import config1_setup, gpu_objects, make_case, oracle_forward, rel_l2
+from tests.wave_cases import ricker_256_order2_setup, gpu_objects, make_case, oracle_forward, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -54,11 +54,11 @@ def test_forward_bitexact_vs_c_oracle(shape, order, varc, sponge, dtype):
     assert np.array_equal(tr_g[0], tr_o)
 
 
-def test_config1_vs_reference_slice_form_fp32():
+def test_ricker_256_order2_1000_steps_vs_slice_form_fp32():
     """256^2, order 2, 1000 steps, fp32: wavefield and 64 traces within 1e-5 relative L2 of the
     slice-form restatement of the reference step, and bit-exact with the C oracle."""
-    case = config1_setup()
-    print("\n[config1] 256x256 order 2 1000 steps fp32", flush=True)
+    case = ricker_256_order2_setup()
+    print("\n[ricker 256x256] order 2 1000 steps fp32", flush=True)
     tr_g, up_g, uc_g = _run_gpu(case, torch.float32)
     kc32 = np.full(case["shape"], case["kconst"], np.float32)
     tr_s, _, uc_s = slice_ref.forward(kc32, 2, case["h"], case["nsteps"], None, case["src"],

@@ -75,7 +75,7 @@ def rel_l2(a, b) -> float:
     return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
 
 
-def config1_setup():
+def ricker_256_order2_setup():
     """256 x 256 nodes, dx = 1, c = 1, c dt / dx = 0.5, order 2, Ricker f0 = 0.05/dt at the centre,
     64 receivers on row 20 (the reference's own CPU-runnable case)."""
     n = 256
