@@ -115,7 +115,7 @@ __device__ __forceinline__ void gather_points(const StepArgs<T>& a, uint32_t blk
 
 struct LaunchCfg {
     int ri;    // rows (2D) or planes (3D) marched per wave / block
-    int tile;  // 3D: tile variant
+    int tile;  // variant: bit 0 = lane-exchange halo (2D), bit 1 = streaming access (WFD_VARIANT_STREAM)
 };
 
 template <typename T>
@@ -129,7 +129,7 @@ template <typename T>
 hipError_t launch_step2d(const StepArgs<T>& a, bool vrow, int order, bool varc, bool sponge, bool img,
                          int variant, int ri, int nshots, hipStream_t s);
 template <typename T>
-hipError_t launch_step3d(const StepArgs<T>& a, int order, bool varc, bool sponge, bool img, int rk,
+hipError_t launch_step3d(const StepArgs<T>& a, int order, bool varc, bool sponge, bool img, int rk, bool nt,
                          int nshots, hipStream_t s);
 
 template <typename T>

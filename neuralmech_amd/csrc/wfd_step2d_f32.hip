@@ -1,6 +1,9 @@
-// float instantiation of the 1D / 2D step kernels (split by dtype so the two compile in parallel).
+// float instantiation of the 1D / 2D step kernels, plain access (split by dtype and access mode so
+// the instantiations compile in parallel).
 #include "wfd_step2d_impl.h"
 
 namespace wfd {
+template hipError_t launch_step2d_mode<float, false>(const StepArgs<float>&, bool, int, bool, bool, bool, int, int,
+                                                     int, hipStream_t);
 template hipError_t launch_step2d<float>(const StepArgs<float>&, bool, int, bool, bool, bool, int, int, int, hipStream_t);
 }  // namespace wfd

@@ -45,6 +45,27 @@ __device__ __forceinline__ C4<double> ld4<double>(const double* p)
     return {{a.x, a.y, b.x, b.y}};
 }
 
+// Streaming access to the fields a step touches once (u_prev, kc, u_next): with NT the loads and
+// stores carry the nontemporal hint, which keeps them from displacing u_cur (read by several
+// waves) from the caches.  That pays off when the step's fields do not fit the Infinity Cache and
+// costs when they do (the hinted accesses then go to HBM), so it is a launch option.
+typedef float wfd_v4f __attribute__((ext_vector_type(4)));
+typedef double wfd_v2d __attribute__((ext_vector_type(2)));
+template <bool NT, typename T>
+__device__ __forceinline__ C4<T> sld4(const T* p)
+{
+    if constexpr (NT && sizeof(T) == 4) {
+        const wfd_v4f v = __builtin_nontemporal_load(reinterpret_cast<const wfd_v4f*>(p));
+        return {{v.x, v.y, v.z, v.w}};
+    } else if constexpr (NT) {
+        const wfd_v2d a = __builtin_nontemporal_load(reinterpret_cast<const wfd_v2d*>(p));
+        const wfd_v2d b = __builtin_nontemporal_load(reinterpret_cast<const wfd_v2d*>(p + 2));
+        return {{a.x, a.y, b.x, b.y}};
+    } else {
+        return ld4<T>(p);
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void st4(T* p, const T (&v)[4]);
 template <>
@@ -57,6 +78,20 @@ __device__ __forceinline__ void st4<double>(double* p, const double (&v)[4])
 {
     *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
     *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void sst4(T* p, const T (&v)[4])
+{
+    if constexpr (NT && sizeof(T) == 4) {
+        const wfd_v4f w = {v[0], v[1], v[2], v[3]};
+        __builtin_nontemporal_store(w, reinterpret_cast<wfd_v4f*>(p));
+    } else if constexpr (NT) {
+        const wfd_v2d a = {v[0], v[1]}, b = {v[2], v[3]};
+        __builtin_nontemporal_store(a, reinterpret_cast<wfd_v2d*>(p));
+        __builtin_nontemporal_store(b, reinterpret_cast<wfd_v2d*>(p + 2));
+    } else {
+        st4<T>(p, v);
+    }
 }
 
 template <typename T>
@@ -111,8 +146,10 @@ __device__ __forceinline__ T lap_elem(const T wc, const T* wr, const T* wcol, co
 
 // One stencil tile of both variants: column block tx, row block ty of shot `shot`.  XL selects
 // the lane-exchange halo; VROW = false is the 1D case (a single row, no vertical stencil).  PF is
-// the number of rows whose per-row operands are loaded together (see the row loop).
-template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, int PF = 1>
+// the number of rows whose per-row operands are loaded together (see the row loop); NT selects
+// streaming access to u_prev, kc and u_next.
+template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, int PF = 1,
+          bool NT = false>
 __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, uint32_t ty, int shot, int64_t step)
 {
     constexpr int KR = VROW ? K : 0;
@@ -180,8 +217,8 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
             const int64_t r = r0 + q0 + i;
             const bool ld = own && r < a.row_hi;
             const int64_t o = r * a.ld + c;
-            Pv[i] = ld ? ld4(up + o) : zero4<T>();
-            if constexpr (VARC) Kv[i] = ld ? ld4(coef + o) : zero4<T>();
+            Pv[i] = ld ? sld4<NT>(up + o) : zero4<T>();
+            if constexpr (VARC) Kv[i] = ld ? sld4<NT>(coef + o) : zero4<T>();
             if constexpr (IMG) Gv[i] = ld ? ld4(a.g + sb + o) : zero4<T>();
         }
 #pragma unroll
@@ -238,7 +275,7 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
 #pragma unroll
             for (int e = 0; e < 4; e++)
                 if (!valid[e]) val[e] = T(0);
-            if (own) st4(un + o, val);
+            if (own) sst4<NT>(un + o, val);
             if constexpr (IMG) {
                 T gv[4];
 #pragma unroll
@@ -264,7 +301,7 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
 }
 
 // A step launch: gather blocks copy receiver values, the other blocks update one tile each.
-template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL>
+template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, bool NT>
 __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
 {
     const StepBlock bp = step_block(a.ngb, a.gx, a.gy);
@@ -273,26 +310,27 @@ __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
         gather_points(a, bp.g, step);
         return;
     }
-    step2d_tile<T, K, VROW, VARC, SPONGE, IMG, RI, XL>(a, bp.x, bp.y, (int)bp.z, step);
+    step2d_tile<T, K, VROW, VARC, SPONGE, IMG, RI, XL, 1, NT>(a, bp.x, bp.y, (int)bp.z, step);
 }
 
-template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI>
+template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool NT>
 __global__ __launch_bounds__(256) void step2d_kernel(const StepArgs<T> a)
 {
-    step2d_body<T, K, VROW, VARC, SPONGE, IMG, RI, false>(a);
+    step2d_body<T, K, VROW, VARC, SPONGE, IMG, RI, false, NT>(a);
 }
 
 constexpr int XL_COLS = 248;
 
-template <typename T, int K, bool VARC, bool SPONGE, bool IMG, int RI>
+template <typename T, int K, bool VARC, bool SPONGE, bool IMG, int RI, bool NT>
 __global__ __launch_bounds__(256) void step2d_xl_kernel(const StepArgs<T> a)
 {
-    step2d_body<T, K, true, VARC, SPONGE, IMG, RI, true>(a);
+    step2d_body<T, K, true, VARC, SPONGE, IMG, RI, true, NT>(a);
 }
 
 // ------------------------------------------------------------------------------ dispatch
-template <typename T, int K, bool VARC, bool SPONGE, bool IMG>
-static hipError_t launch2d_k(const StepArgs<T>& a0, bool vrow, int variant, int ri, int nshots, hipStream_t s)
+// variant bit 0: lane-exchange halo (XL); bit 1: streaming access (NT)
+template <typename T, int K, bool VARC, bool SPONGE, bool IMG, bool NT>
+static hipError_t launch2d_kn(const StepArgs<T>& a0, bool vrow, int variant, int ri, int nshots, hipStream_t s)
 {
     const dim3 block(256);
     const int64_t rows = a0.row_hi - a0.row_lo;
@@ -305,64 +343,77 @@ static hipError_t launch2d_k(const StepArgs<T>& a0, bool vrow, int variant, int 
     };
     if (!vrow) {
         const dim3 g = grid(256, 1);
-        hipLaunchKernelGGL((step2d_kernel<T, K, false, VARC, SPONGE, IMG, 1>), g, block, 0, s, a);
+        hipLaunchKernelGGL((step2d_kernel<T, K, false, VARC, SPONGE, IMG, 1, NT>), g, block, 0, s, a);
         return hipGetLastError();
     }
-    if (variant == 1) {
+    if (variant & 1) {
         if constexpr (sizeof(T) == 4) {
             switch (ri) {
-                case 1: { const dim3 g = grid(XL_COLS, 1); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 1>), g, block, 0, s, a); break; }
-                case 2: { const dim3 g = grid(XL_COLS, 2); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 2>), g, block, 0, s, a); break; }
-                case 8: { const dim3 g = grid(XL_COLS, 8); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 8>), g, block, 0, s, a); break; }
-                default: { const dim3 g = grid(XL_COLS, 4); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4>), g, block, 0, s, a); break; }
+                case 1: { const dim3 g = grid(XL_COLS, 1); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 1, NT>), g, block, 0, s, a); break; }
+                case 2: { const dim3 g = grid(XL_COLS, 2); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 2, NT>), g, block, 0, s, a); break; }
+                case 8: { const dim3 g = grid(XL_COLS, 8); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 8, NT>), g, block, 0, s, a); break; }
+                default: { const dim3 g = grid(XL_COLS, 4); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4, NT>), g, block, 0, s, a); break; }
             }
         } else {
             const dim3 g = grid(XL_COLS, 4);
-            hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4>), g, block, 0, s, a);
+            hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4, NT>), g, block, 0, s, a);
         }
         return hipGetLastError();
     }
     if constexpr (sizeof(T) == 4) {
         switch (ri) {
-            case 1: { const dim3 g = grid(256, 1); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 1>), g, block, 0, s, a); break; }
-            case 2: { const dim3 g = grid(256, 2); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 2>), g, block, 0, s, a); break; }
-            case 8: { const dim3 g = grid(256, 8); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 8>), g, block, 0, s, a); break; }
-            default: { const dim3 g = grid(256, 4); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4>), g, block, 0, s, a); break; }
+            case 1: { const dim3 g = grid(256, 1); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 1, NT>), g, block, 0, s, a); break; }
+            case 2: { const dim3 g = grid(256, 2); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 2, NT>), g, block, 0, s, a); break; }
+            case 8: { const dim3 g = grid(256, 8); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 8, NT>), g, block, 0, s, a); break; }
+            default: { const dim3 g = grid(256, 4); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4, NT>), g, block, 0, s, a); break; }
         }
     } else {
         const dim3 g = grid(256, 4);
-        hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4>), g, block, 0, s, a);
+        hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4, NT>), g, block, 0, s, a);
     }
     return hipGetLastError();
 }
 
-template <typename T, int K>
+template <typename T, int K, bool NT>
 static hipError_t launch2d_flags(const StepArgs<T>& a, bool vrow, bool varc, bool sponge, bool img, int variant,
                                  int ri, int nshots, hipStream_t s)
 {
     const int f = (varc ? 4 : 0) | (sponge ? 2 : 0) | (img ? 1 : 0);
     switch (f) {
-        case 0: return launch2d_k<T, K, false, false, false>(a, vrow, variant, ri, nshots, s);
-        case 1: return launch2d_k<T, K, false, false, true>(a, vrow, variant, ri, nshots, s);
-        case 2: return launch2d_k<T, K, false, true, false>(a, vrow, variant, ri, nshots, s);
-        case 3: return launch2d_k<T, K, false, true, true>(a, vrow, variant, ri, nshots, s);
-        case 4: return launch2d_k<T, K, true, false, false>(a, vrow, variant, ri, nshots, s);
-        case 5: return launch2d_k<T, K, true, false, true>(a, vrow, variant, ri, nshots, s);
-        case 6: return launch2d_k<T, K, true, true, false>(a, vrow, variant, ri, nshots, s);
-        default: return launch2d_k<T, K, true, true, true>(a, vrow, variant, ri, nshots, s);
+        case 0: return launch2d_kn<T, K, false, false, false, NT>(a, vrow, variant, ri, nshots, s);
+        case 1: return launch2d_kn<T, K, false, false, true, NT>(a, vrow, variant, ri, nshots, s);
+        case 2: return launch2d_kn<T, K, false, true, false, NT>(a, vrow, variant, ri, nshots, s);
+        case 3: return launch2d_kn<T, K, false, true, true, NT>(a, vrow, variant, ri, nshots, s);
+        case 4: return launch2d_kn<T, K, true, false, false, NT>(a, vrow, variant, ri, nshots, s);
+        case 5: return launch2d_kn<T, K, true, false, true, NT>(a, vrow, variant, ri, nshots, s);
+        case 6: return launch2d_kn<T, K, true, true, false, NT>(a, vrow, variant, ri, nshots, s);
+        default: return launch2d_kn<T, K, true, true, true, NT>(a, vrow, variant, ri, nshots, s);
     }
 }
+
+// The kernels of one access mode (NT) for every order and flag set; the float instantiations of
+// the two modes live in separate files so that they compile in parallel.
+template <typename T, bool NT>
+hipError_t launch_step2d_mode(const StepArgs<T>& a, bool vrow, int order, bool varc, bool sponge, bool img,
+                              int variant, int ri, int nshots, hipStream_t s)
+{
+    switch (order) {
+        case 2: return launch2d_flags<T, 1, NT>(a, vrow, varc, sponge, img, variant, ri, nshots, s);
+        case 4: return launch2d_flags<T, 2, NT>(a, vrow, varc, sponge, img, variant, ri, nshots, s);
+        case 6: return launch2d_flags<T, 3, NT>(a, vrow, varc, sponge, img, variant, ri, nshots, s);
+        default: return launch2d_flags<T, 4, NT>(a, vrow, varc, sponge, img, variant, ri, nshots, s);
+    }
+}
+
+extern template hipError_t launch_step2d_mode<float, true>(const StepArgs<float>&, bool, int, bool, bool, bool, int,
+                                                           int, int, hipStream_t);
 
 template <typename T>
 hipError_t launch_step2d(const StepArgs<T>& a, bool vrow, int order, bool varc, bool sponge, bool img, int variant,
                          int ri, int nshots, hipStream_t s)
 {
-    switch (order) {
-        case 2: return launch2d_flags<T, 1>(a, vrow, varc, sponge, img, variant, ri, nshots, s);
-        case 4: return launch2d_flags<T, 2>(a, vrow, varc, sponge, img, variant, ri, nshots, s);
-        case 6: return launch2d_flags<T, 3>(a, vrow, varc, sponge, img, variant, ri, nshots, s);
-        default: return launch2d_flags<T, 4>(a, vrow, varc, sponge, img, variant, ri, nshots, s);
-    }
+    if (variant & 2) return launch_step2d_mode<T, true>(a, vrow, order, varc, sponge, img, variant, ri, nshots, s);
+    return launch_step2d_mode<T, false>(a, vrow, order, varc, sponge, img, variant, ri, nshots, s);
 }
 
 }  // namespace wfd

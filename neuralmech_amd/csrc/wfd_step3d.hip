@@ -46,6 +46,41 @@ __device__ __forceinline__ void gst4<double>(double* p, const double (&v)[4])
     *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], v[3]);
 }
 
+// Streaming access to the fields a step touches once (u_prev, kc, u_next): with NT the loads and
+// stores carry the nontemporal hint so they do not displace the planes of u_cur that neighbouring
+// tiles still read.  It pays off when the fields exceed the Infinity Cache and costs when they fit,
+// so it is a launch option.
+typedef float s3_v4f __attribute__((ext_vector_type(4)));
+typedef double s3_v2d __attribute__((ext_vector_type(2)));
+template <bool NT, typename T>
+__device__ __forceinline__ V4<T> gld4s(const T* p)
+{
+    if constexpr (NT && sizeof(T) == 4) {
+        const s3_v4f v = __builtin_nontemporal_load(reinterpret_cast<const s3_v4f*>(p));
+        return {{v.x, v.y, v.z, v.w}};
+    } else if constexpr (NT) {
+        const s3_v2d a = __builtin_nontemporal_load(reinterpret_cast<const s3_v2d*>(p));
+        const s3_v2d b = __builtin_nontemporal_load(reinterpret_cast<const s3_v2d*>(p + 2));
+        return {{a.x, a.y, b.x, b.y}};
+    } else {
+        return gld4<T>(p);
+    }
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void gst4s(T* p, const T (&v)[4])
+{
+    if constexpr (NT && sizeof(T) == 4) {
+        const s3_v4f w = {v[0], v[1], v[2], v[3]};
+        __builtin_nontemporal_store(w, reinterpret_cast<s3_v4f*>(p));
+    } else if constexpr (NT) {
+        const s3_v2d a = {v[0], v[1]}, b = {v[2], v[3]};
+        __builtin_nontemporal_store(a, reinterpret_cast<s3_v2d*>(p));
+        __builtin_nontemporal_store(b, reinterpret_cast<s3_v2d*>(p + 2));
+    } else {
+        gst4<T>(p, v);
+    }
+}
+
 __device__ __forceinline__ float fma3(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma3(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
@@ -98,7 +133,7 @@ __device__ __forceinline__ T lap3(const T wc, const T* wp, const T* wr, const T*
     return lap;
 }
 
-template <typename T, int K, bool VARC, bool SPONGE, bool IMG>
+template <typename T, int K, bool VARC, bool SPONGE, bool IMG, bool NT>
 __global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk, int nkb)
 {
     __shared__ T tile[IMG ? 2 : 1][2][T3_LH][T3_LW];
@@ -169,10 +204,10 @@ __global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk
         if constexpr (IMG) stage_plane<T>(tile[IMG ? 1 : 0][buf], uf, wf[K], k, i0, j0, ty, tx, a.nr, ld, ps);
         __syncthreads();
         const int64_t o = k * ps + rowoff;
-        const V4<T> P = out ? gld4(up + o) : zero;
+        const V4<T> P = out ? gld4s<NT>(up + o) : zero;
         T kc[4];
         if constexpr (VARC) {
-            const V4<T> kv = out ? gld4(coef + o) : zero;
+            const V4<T> kv = out ? gld4s<NT>(coef + o) : zero;
 #pragma unroll
             for (int e = 0; e < 4; e++) kc[e] = kv.e[e];
         } else {
@@ -225,7 +260,7 @@ __global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk
 #pragma unroll
         for (int e = 0; e < 4; e++)
             if (!valid[e]) val[e] = T(0);
-        if (out) gst4(un + o, val);
+        if (out) gst4s<NT>(un + o, val);
         if constexpr (IMG) {
             T gv[4];
 #pragma unroll
@@ -240,8 +275,8 @@ __global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk
     }
 }
 
-template <typename T, int K, bool VARC, bool SPONGE, bool IMG>
-static hipError_t launch3d_k(const StepArgs<T>& a, int rk, int nshots, hipStream_t s)
+template <typename T, int K, bool VARC, bool SPONGE, bool IMG, bool NT>
+static hipError_t launch3d_kn(const StepArgs<T>& a, int rk, int nshots, hipStream_t s)
 {
     StepArgs<T> b = a;
     b.gx = (uint32_t)((a.nc + T3_TJ - 1) / T3_TJ);
@@ -249,42 +284,49 @@ static hipError_t launch3d_k(const StepArgs<T>& a, int rk, int nshots, hipStream
     const int64_t nkb = (a.np - 2 + rk - 1) / rk;
     // 1D launch: b.ngb gather blocks, then gx * gy * (nkb * nshots) tiles
     const int64_t nblocks = step_grid((uint32_t)((int64_t)b.gx * b.gy * nkb * nshots), b.ngb);
-    hipLaunchKernelGGL((step3d_kernel<T, K, VARC, SPONGE, IMG>), dim3((unsigned)nblocks), dim3(256), 0, s, b, rk,
+    hipLaunchKernelGGL((step3d_kernel<T, K, VARC, SPONGE, IMG, NT>), dim3((unsigned)nblocks), dim3(256), 0, s, b, rk,
                        (int)nkb);
     return hipGetLastError();
 }
 
+template <typename T, int K, bool VARC, bool SPONGE, bool IMG>
+static hipError_t launch3d_k(const StepArgs<T>& a, int rk, int nshots, hipStream_t s, bool nt)
+{
+    if (nt) return launch3d_kn<T, K, VARC, SPONGE, IMG, true>(a, rk, nshots, s);
+    return launch3d_kn<T, K, VARC, SPONGE, IMG, false>(a, rk, nshots, s);
+}
+
 template <typename T, int K>
-static hipError_t launch3d_flags(const StepArgs<T>& a, bool varc, bool sponge, bool img, int rk, int nshots,
+static hipError_t launch3d_flags(const StepArgs<T>& a, bool varc, bool sponge, bool img, int rk, bool nt, int nshots,
                                  hipStream_t s)
 {
     const int f = (varc ? 4 : 0) | (sponge ? 2 : 0) | (img ? 1 : 0);
     switch (f) {
-        case 0: return launch3d_k<T, K, false, false, false>(a, rk, nshots, s);
-        case 1: return launch3d_k<T, K, false, false, true>(a, rk, nshots, s);
-        case 2: return launch3d_k<T, K, false, true, false>(a, rk, nshots, s);
-        case 3: return launch3d_k<T, K, false, true, true>(a, rk, nshots, s);
-        case 4: return launch3d_k<T, K, true, false, false>(a, rk, nshots, s);
-        case 5: return launch3d_k<T, K, true, false, true>(a, rk, nshots, s);
-        case 6: return launch3d_k<T, K, true, true, false>(a, rk, nshots, s);
-        default: return launch3d_k<T, K, true, true, true>(a, rk, nshots, s);
+        case 0: return launch3d_k<T, K, false, false, false>(a, rk, nshots, s, nt);
+        case 1: return launch3d_k<T, K, false, false, true>(a, rk, nshots, s, nt);
+        case 2: return launch3d_k<T, K, false, true, false>(a, rk, nshots, s, nt);
+        case 3: return launch3d_k<T, K, false, true, true>(a, rk, nshots, s, nt);
+        case 4: return launch3d_k<T, K, true, false, false>(a, rk, nshots, s, nt);
+        case 5: return launch3d_k<T, K, true, false, true>(a, rk, nshots, s, nt);
+        case 6: return launch3d_k<T, K, true, true, false>(a, rk, nshots, s, nt);
+        default: return launch3d_k<T, K, true, true, true>(a, rk, nshots, s, nt);
     }
 }
 
 template <typename T>
-hipError_t launch_step3d(const StepArgs<T>& a, int order, bool varc, bool sponge, bool img, int rk, int nshots,
+hipError_t launch_step3d(const StepArgs<T>& a, int order, bool varc, bool sponge, bool img, int rk, bool nt, int nshots,
                          hipStream_t s)
 {
     switch (order) {
-        case 2: return launch3d_flags<T, 1>(a, varc, sponge, img, rk, nshots, s);
-        case 4: return launch3d_flags<T, 2>(a, varc, sponge, img, rk, nshots, s);
-        case 6: return launch3d_flags<T, 3>(a, varc, sponge, img, rk, nshots, s);
-        default: return launch3d_flags<T, 4>(a, varc, sponge, img, rk, nshots, s);
+        case 2: return launch3d_flags<T, 1>(a, varc, sponge, img, rk, nt, nshots, s);
+        case 4: return launch3d_flags<T, 2>(a, varc, sponge, img, rk, nt, nshots, s);
+        case 6: return launch3d_flags<T, 3>(a, varc, sponge, img, rk, nt, nshots, s);
+        default: return launch3d_flags<T, 4>(a, varc, sponge, img, rk, nt, nshots, s);
     }
 }
 
-template hipError_t launch_step3d<float>(const StepArgs<float>&, int, bool, bool, bool, int, int, hipStream_t);
-template hipError_t launch_step3d<double>(const StepArgs<double>&, int, bool, bool, bool, int, int, hipStream_t);
+template hipError_t launch_step3d<float>(const StepArgs<float>&, int, bool, bool, bool, int, bool, int, hipStream_t);
+template hipError_t launch_step3d<double>(const StepArgs<double>&, int, bool, bool, bool, int, bool, int, hipStream_t);
 
 // ------------------------------------------------------------------------- small kernels
 // Gather-only launch: copy a field at the receiver nodes into one trace row.

@@ -201,3 +201,38 @@ def test_acoustic_entry_points_validate_arguments(case):
                              ctypes.c_void_p(0x8000), None)
     assert rc == _lib.WFD_EINVAL, _lib.last_error()
     assert _lib.last_error()
+
+
+def test_launch_config_rules_need_no_gpu():
+    """Launch shapes are host state: the streaming bit (WFD_VARIANT_STREAM = 2) is accepted on both
+    2D halo variants and the 3D kernel, anything else is rejected, and an untuned grid streams
+    exactly when its four step fields exceed 512 MiB."""
+    lib = _lib.load()
+    v, r = ctypes.c_int32(), ctypes.c_int32()
+
+    def cfg(g):
+        assert lib.wfd_get_config(ctypes.byref(g), ctypes.byref(v), ctypes.byref(r)) == 0
+        return v.value, r.value
+
+    _, small = _grid((2048, 2048), (10.0, 10.0))
+    _, big = _grid((8192, 8192), (10.0, 10.0))
+    _, big64 = _grid((6000, 6000), (10.0, 10.0), dtype=_lib.WFD_F64)
+    _, batch = _grid((2048, 2048), (10.0, 10.0), nshots=16)
+    _, fwi = _grid((1024, 1024), (10.0, 10.0), nshots=16)
+    _, cube = _grid((512, 512, 512), (10.0, 10.0, 10.0))
+    _, small3 = _grid((128, 128, 128), (10.0, 10.0, 10.0))
+    assert cfg(small)[0] & 2 == 0 and cfg(big)[0] & 2 == 2 and cfg(big64)[0] & 2 == 2
+    assert cfg(batch)[0] & 2 == 2  # 16 shots of 2048^2: 16 x 4 fields x 17 MB
+    assert cfg(fwi)[0] & 2 == 0  # 16 shots of 1024^2: 279 MB, in good part cache-resident
+    assert cfg(cube)[0] == 2 and cfg(small3)[0] == 0
+    for variant in (0, 1, 2, 3):
+        assert lib.wfd_set_config(ctypes.byref(small), variant, 4) == 0
+        assert cfg(small) == (variant, 4)
+    for variant in (-1, 4, 5):
+        assert lib.wfd_set_config(ctypes.byref(small), variant, 4) == _lib.WFD_EINVAL
+    assert lib.wfd_set_config(ctypes.byref(cube), 2, 16) == 0 and cfg(cube) == (2, 16)
+    assert lib.wfd_set_config(ctypes.byref(cube), 1, 16) == _lib.WFD_EINVAL
+    assert lib.wfd_set_config(ctypes.byref(cube), 3, 16) == _lib.WFD_EINVAL
+    for g in (small, cube):
+        assert lib.wfd_set_config(ctypes.byref(g), 0, 0) == 0  # forget: back to the default
+    assert cfg(small)[0] & 2 == 0 and cfg(cube) == (2, 32)

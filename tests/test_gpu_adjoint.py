@@ -138,7 +138,7 @@ def test_batched_gradient_is_sum_of_shot_gradients():
     assert rel_l2(total.cpu().numpy(), acc.cpu().numpy()) < 1e-13
 
 
-@pytest.mark.parametrize("variant,rows", [(0, 2), (1, 1), (1, 8)])
+@pytest.mark.parametrize("variant,rows", [(0, 2), (1, 1), (1, 8), (2, 2), (3, 8)])
 def test_adjoint_launch_shapes_bitexact(variant, rows):
     case = make_case((45, 300), 8, seed=29, nsteps=30, nrec=5)
     from neuralmech_amd import solver

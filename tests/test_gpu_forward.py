@@ -164,7 +164,8 @@ def test_medium_check_reports_nonfinite():
     assert e.value.code == _lib.WFD_ENONFINITE
 
 
-SHAPES_2D = [(0, 1), (0, 2), (0, 4), (0, 8), (1, 1), (1, 2), (1, 4), (1, 8)]
+# (variant, rows): variant bit 0 = lane-exchange halo, bit 1 = streaming access to u_prev, kc, u_next
+SHAPES_2D = [(v, r) for v in (0, 1, 2, 3) for r in (1, 2, 4, 8)]
 
 
 @pytest.mark.parametrize("variant,rows", SHAPES_2D)
@@ -187,19 +188,39 @@ def test_every_2d_launch_shape_bitexact(variant, rows):
         assert np.array_equal(tr[0].cpu().numpy(), tr_o)
 
 
-@pytest.mark.parametrize("rows", [4, 8, 16, 64])
-def test_every_3d_launch_shape_bitexact(rows):
+@pytest.mark.parametrize("variant,rows", [(0, 4), (0, 8), (0, 16), (0, 64), (2, 4), (2, 32)])
+def test_every_3d_launch_shape_bitexact(variant, rows):
     from neuralmech_amd import solver
 
     case = make_case((30, 21, 70), 8, seed=31, nsteps=12)
     g, med, src, rec = gpu_objects(case, torch.float32)
-    g.set_launch(0, rows)
+    g.set_launch(variant, rows)
     try:
         tr, (_, uc) = solver.run(g, med, 12, src=src, sig=case["sig"], rec=rec)
         torch.cuda.synchronize()
     finally:
         g.set_launch(0, 0)
     tr_o, _, uc_o = oracle_forward(case, np.float32)
+    assert np.array_equal(g.nodes(uc)[0].cpu().numpy(), uc_o)
+    assert np.array_equal(tr[0].cpu().numpy(), tr_o)
+
+
+@pytest.mark.parametrize("shape,variant,rows", [((61, 530), 3, 4), ((61, 530), 2, 4), ((30, 21, 70), 2, 8)])
+def test_streaming_access_fp64_bitexact(shape, variant, rows):
+    """The streaming variants in float64 (two 16-byte hinted accesses per 4-node chunk), in-place
+    leapfrog (u_next aliases u_prev), against the C oracle bit for bit."""
+    from neuralmech_amd import solver
+
+    print(f"\n[launch] fp64 streaming {shape} variant={variant} rows={rows}", flush=True)
+    case = make_case(shape, 8, seed=43, nsteps=15)
+    g, med, src, rec = gpu_objects(case, torch.float64)
+    g.set_launch(variant, rows)
+    try:
+        tr, (_, uc) = solver.run(g, med, 15, src=src, sig=case["sig"], rec=rec)
+        torch.cuda.synchronize()
+    finally:
+        g.set_launch(0, 0)
+    tr_o, _, uc_o = oracle_forward(case, np.float64)
     assert np.array_equal(g.nodes(uc)[0].cpu().numpy(), uc_o)
     assert np.array_equal(tr[0].cpu().numpy(), tr_o)
 
