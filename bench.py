@@ -286,7 +286,40 @@ def copy_main(args):
         out[label] = {"GB_per_s": round(2 * nbytes * reps / s / 1e9, 1), "bytes_read_per_copy": nbytes,
                       "bytes_written_per_copy": nbytes, "copies": reps + 1}
         del a, b
-    print(json.dumps({"metric": "device copy bandwidth", "unit": "GB/s", "results": out}))
+    # hand-written streaming kernels (wfd_stream_bench): copy, the stencil's 3-read / 1-write mix
+    # (plain and with the nontemporal hint), read-only; 1 GiB per array (HBM) and 32 MiB (cache)
+    import ctypes
+
+    from neuralmech_amd import _lib
+
+    lib = _lib.load()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    kernels = {}
+    for label, nbytes in (("1GiB", 1 << 30), ("32MiB", 32 << 20)):
+        n = nbytes // 4
+        arr = [torch.ones(n, dtype=torch.float32, device="cuda") for _ in range(4)]
+        reps = 10 if nbytes >= (1 << 30) else 200
+        row = {}
+        for mode, name, moved in ((0, "copy", 2), (1, "read3_write1", 4), (2, "read3_write1_nontemporal", 4),
+                                  (3, "read", 1)):
+            def launch():
+                _lib.check(lib.wfd_stream_bench(arr[0].data_ptr(), arr[1].data_ptr(), arr[2].data_ptr(),
+                                                arr[3].data_ptr(), n, mode, st))
+            launch()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                launch()
+            e1.record()
+            torch.cuda.synchronize()
+            row[name] = round(moved * nbytes * reps / (e0.elapsed_time(e1) / 1e3) / 1e9, 1)
+        if not torch.equal(arr[3][: n // 4096], torch.full((n // 4096,), 4096.0, device="cuda")):
+            raise RuntimeError("wfd_stream_bench: wrong block sums")
+        kernels[label] = row
+        del arr
+    print(json.dumps({"metric": "device copy bandwidth", "unit": "GB/s", "results": out,
+                      "stream_kernels": kernels}))
 
 
 def volume_main(args):

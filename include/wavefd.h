@@ -391,6 +391,15 @@ int wfd_tfm(const void* h, const float* xe, const float* ye, int32_t E, int32_t 
  * instructions, like every kernel here. */
 int wfd_fma_peak(float* out, int32_t blocks, int32_t n, wfd_stream_t stream);
 
+/* HBM streaming ceilings for the access mixes of the step kernels, the hand-written counterpart
+ * of the reference harness's library copy (2D_wave_roofline.py:46-57): n floats per array (a
+ * multiple of 4, 16-byte aligned), one 16-byte chunk per lane.  mode 0: o = a (copy); 1: o = a +
+ * b + c (three reads, one write: the stencil's mix); 2: mode 1 with the nontemporal hint on every
+ * access; 3: read a only, block sums of 4096 floats into o[0 .. ceil(n / 4096)).  Time it with
+ * events on `stream`. */
+int wfd_stream_bench(const float* a, const float* b, const float* c, float* o, int64_t n, int32_t mode,
+                     wfd_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

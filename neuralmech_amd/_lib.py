@@ -144,6 +144,7 @@ SIGNATURES = {
     "wfd_batch_shot_info": (ctypes.c_int, [_P, ctypes.c_int32, _I32P, _I32P, _I32P, _I64P, _I64P, _I64P]),
     "wfd_batch_shot_ops": (ctypes.c_int, [_I64, _I64, ctypes.c_int32, ctypes.c_int32, _I32P, _I64, _I64P, _I32P]),
     "wfd_fma_peak": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _P]),
+    "wfd_stream_bench": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_int64, ctypes.c_int32, _P]),
     "wfd_tfm": (ctypes.c_int, [_P, _P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_double, _P, _P, _I64, _P, ctypes.c_int32, _P]),
     "wfd_ac_forward": (ctypes.c_int, [_GP, _P, _P, ctypes.POINTER(AcMedium), _PP, _PP, _PP, _I64, _I64, _P]),

@@ -155,3 +155,36 @@ def test_roofline_and_scaling_harness_small():
     assert all(math.isfinite(k["gflops"]) and k["gflops"] > 0 for k in r["kernels"])
     dofs, t = harness.scaling("padded", max_resolution=40, steps=40)
     assert dofs and len(dofs) == len(t) and all(x > 0 for x in t)
+
+
+def test_stream_bench_kernels_compute_what_they_state():
+    """wfd_stream_bench (the roofline harness's streaming ceilings): copy, a + b + c with and
+    without the nontemporal hint, and the block sums of the read-only kernel; argument checks."""
+    import ctypes
+
+    import torch
+
+    from neuralmech_amd import _lib
+
+    lib = _lib.load()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    n = 4096 * 3 + 8  # a partial last block in every mode
+    g = torch.Generator().manual_seed(3)
+    a, b, c = (torch.rand(n, generator=g).cuda() for _ in range(3))
+    o = torch.zeros(n, device="cuda")
+    _lib.check(lib.wfd_stream_bench(a.data_ptr(), 0, 0, o.data_ptr(), n, 0, st))
+    assert torch.equal(o, a)
+    for mode in (1, 2):
+        o.zero_()
+        _lib.check(lib.wfd_stream_bench(a.data_ptr(), b.data_ptr(), c.data_ptr(), o.data_ptr(), n, mode, st))
+        assert torch.equal(o, a + b + c)
+    o.zero_()
+    _lib.check(lib.wfd_stream_bench(a.data_ptr(), 0, 0, o.data_ptr(), n, 3, st))
+    torch.cuda.synchronize()
+    blocks = -(-n // 4096)
+    ref = torch.stack([a[k * 4096:(k + 1) * 4096].double().sum() for k in range(blocks)])
+    assert torch.allclose(o[:blocks].double(), ref, rtol=1e-5) and torch.all(o[blocks:] == 0)
+    assert lib.wfd_stream_bench(a.data_ptr(), 0, 0, o.data_ptr(), n, 1, st) == _lib.WFD_EINVAL
+    assert lib.wfd_stream_bench(a.data_ptr(), 0, 0, o.data_ptr(), n - 2, 0, st) == _lib.WFD_EINVAL
+    assert lib.wfd_stream_bench(a.data_ptr() + 4, 0, 0, o.data_ptr(), 8, 0, st) == _lib.WFD_EINVAL
+    assert lib.wfd_stream_bench(a.data_ptr(), 0, 0, o.data_ptr(), n, 4, st) == _lib.WFD_EINVAL
