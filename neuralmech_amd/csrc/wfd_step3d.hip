@@ -84,7 +84,8 @@ __device__ __forceinline__ void gst4s(T* p, const T (&v)[4])
 __device__ __forceinline__ float fma3(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma3(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-constexpr int T3_TI = 16;  // tile rows
+constexpr int T3_TI = 16;  // tile rows: a block has 16 lanes per row (4-node chunks), 4 rows per wave
+constexpr int T3_THREADS = 16 * T3_TI;
 constexpr int T3_TJ = 64;  // tile columns
 constexpr int T3_LH = T3_TI + 2 * HALO;
 constexpr int T3_LW = T3_TJ + 2 * HALO;
@@ -107,8 +108,8 @@ __device__ __forceinline__ void stage_plane(T (*t)[T3_LW], const T* u, const V4<
         if (row <= nr + 3) v = gld4(u + k * ps + row * ld + j0 + hc * 4);
 #pragma unroll
         for (int e = 0; e < 4; e++) t[lr][HALO + hc * 4 + e] = v.e[e];
-    } else if (tid < 160) {
-        const int s = tid - 128, side = s >> 4, rr = s & 15;
+    } else if (tid < 128 + 2 * T3_TI) {
+        const int s = tid - 128, side = s / T3_TI, rr = s % T3_TI;
         const int64_t row = i0 + rr;
         V4<T> v = {{T(0), T(0), T(0), T(0)}};
         if (row <= nr + 3) v = gld4(u + k * ps + row * ld + (side ? j0 + T3_TJ : j0 - 4));
@@ -134,7 +135,7 @@ __device__ __forceinline__ T lap3(const T wc, const T* wp, const T* wr, const T*
 }
 
 template <typename T, int K, bool VARC, bool SPONGE, bool IMG, bool NT>
-__global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk, int nkb)
+__global__ __launch_bounds__(T3_THREADS) void step3d_kernel(const StepArgs<T> a, int rk, int nkb)
 {
     __shared__ T tile[IMG ? 2 : 1][2][T3_LH][T3_LW];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
@@ -143,7 +144,7 @@ __global__ __launch_bounds__(256) void step3d_kernel(const StepArgs<T> a, int rk
     const StepBlock bp = step_block(a.ngb, a.gx, a.gy);
     const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
     if (bp.gather) {
-        gather_points(a, bp.g, step);
+        if (threadIdx.x < 256) gather_points(a, bp.g, step);  // 256 points per gather block
         return;
     }
     const int shot = bp.z / nkb, kb = bp.z % nkb;
@@ -284,7 +285,7 @@ static hipError_t launch3d_kn(const StepArgs<T>& a, int rk, int nshots, hipStrea
     const int64_t nkb = (a.np - 2 + rk - 1) / rk;
     // 1D launch: b.ngb gather blocks, then gx * gy * (nkb * nshots) tiles
     const int64_t nblocks = step_grid((uint32_t)((int64_t)b.gx * b.gy * nkb * nshots), b.ngb);
-    hipLaunchKernelGGL((step3d_kernel<T, K, VARC, SPONGE, IMG, NT>), dim3((unsigned)nblocks), dim3(256), 0, s, b, rk,
+    hipLaunchKernelGGL((step3d_kernel<T, K, VARC, SPONGE, IMG, NT>), dim3((unsigned)nblocks), dim3(T3_THREADS), 0, s, b, rk,
                        (int)nkb);
     return hipGetLastError();
 }

@@ -28,7 +28,8 @@ def _plan_grad(case, tdt, seg=0, obs=None, nshots=1):
 
 
 @pytest.mark.parametrize("dtype", ["f32", "f64"])
-@pytest.mark.parametrize("shape,order", [((31, 43), 4), ((40, 36), 8), ((14, 17, 23), 8), ((60,), 2)])
+@pytest.mark.parametrize("shape,order", [((31, 43), 4), ((40, 36), 8), ((14, 17, 23), 8), ((9, 70, 75), 4),
+                                         ((60,), 2)])
 def test_imaging_sum_bitexact_vs_oracle(shape, order, dtype):
     print(f"\n[adjoint] shape={shape} order={order} {dtype}", flush=True)
     npdt, tdt = (np.float32, torch.float32) if dtype == "f32" else (np.float64, torch.float64)

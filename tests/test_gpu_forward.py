@@ -192,17 +192,20 @@ def test_every_2d_launch_shape_bitexact(variant, rows):
 def test_every_3d_launch_shape_bitexact(variant, rows):
     from neuralmech_amd import solver
 
-    case = make_case((30, 21, 70), 8, seed=31, nsteps=12)
-    g, med, src, rec = gpu_objects(case, torch.float32)
-    g.set_launch(variant, rows)
-    try:
-        tr, (_, uc) = solver.run(g, med, 12, src=src, sig=case["sig"], rec=rec)
-        torch.cuda.synchronize()
-    finally:
-        g.set_launch(0, 0)
-    tr_o, _, uc_o = oracle_forward(case, np.float32)
-    assert np.array_equal(g.nodes(uc)[0].cpu().numpy(), uc_o)
-    assert np.array_equal(tr[0].cpu().numpy(), tr_o)
+    print(f"\n[launch] 3D variant={variant} rows={rows}", flush=True)
+    # the second grid spans several tile rows and columns (tile boundaries inside the grid)
+    for shape, seed in (((30, 21, 70), 31), ((12, 75, 140), 37)):
+        case = make_case(shape, 8, seed=seed, nsteps=12)
+        g, med, src, rec = gpu_objects(case, torch.float32)
+        g.set_launch(variant, rows)
+        try:
+            tr, (_, uc) = solver.run(g, med, 12, src=src, sig=case["sig"], rec=rec)
+            torch.cuda.synchronize()
+        finally:
+            g.set_launch(0, 0)
+        tr_o, _, uc_o = oracle_forward(case, np.float32)
+        assert np.array_equal(g.nodes(uc)[0].cpu().numpy(), uc_o), shape
+        assert np.array_equal(tr[0].cpu().numpy(), tr_o), shape
 
 
 @pytest.mark.parametrize("shape,variant,rows", [((61, 530), 3, 4), ((61, 530), 2, 4), ((30, 21, 70), 2, 8)])
