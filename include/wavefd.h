@@ -378,7 +378,9 @@ int wfd_ac_backward(const wfd_grid* g, void* u_a, void* u_b, void* mu_a, void* m
  * with linear interpolation in time.  reciprocal = 1: pairs r >= e, r > e weighted 2, samples
  * with k >= Nt-1 skipped (the tfm RawKernel, projects/18_fwi/tfm.py:12-36); reciprocal = 0: all
  * E^2 pairs, k clamped to Nt-2 and skipped there (the array form, tfm.py:39-53).  fp32
- * arithmetic; 1 <= E <= 64; t0 >= 0.  All arrays on the device. */
+ * arithmetic; 1 <= E <= 64; t0 >= 0.  All arrays on the device.  Images of fewer than 2^20
+ * pixels split each pixel's pairs over 2-16 lanes, which changes the fp32 summation order (not
+ * the terms); larger images sum the pairs in emitter-major order. */
 int wfd_tfm(const void* h, const float* xe, const float* ye, int32_t E, int32_t Nt, double dt,
             double t0, double c, const float* xp, const float* yp, int64_t P, float* image,
             int32_t reciprocal, wfd_stream_t stream);

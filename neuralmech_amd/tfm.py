@@ -3,8 +3,9 @@ signal it images (``cupyx.scipy.signal.hilbert`` at ``projects/18_fwi/2D_tfm.py:
 
 * :func:`analytic_signal` -- ``x + i H[x]`` along an axis, by the FFT (the one-sided spectrum
   doubled), as ``scipy.signal.hilbert`` computes it; torch on the device.
-* :func:`tfm_kernel` -- ``tfm_kernel(h, elements, t0, dt, c, x, y)`` (``tfm.py:56-67``): one lane per
-  pixel, element pairs r >= e with the pair r > e weighted 2 (reciprocity), fp32.
+* :func:`tfm_kernel` -- ``tfm_kernel(h, elements, t0, dt, c, x, y)`` (``tfm.py:56-67``): element
+  pairs r >= e with the pair r > e weighted 2 (reciprocity), fp32; one lane per pixel for images
+  of 2^20 pixels and more, a group of up to 16 lanes per pixel below that.
 * :func:`tfm` -- ``tfm(h, elements, t0, dt, c, x, y)`` (``tfm.py:39-53``): every pair, the samples
   past the last interval dropped; runs the same kernel in its all-pairs form.
 

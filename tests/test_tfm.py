@@ -104,6 +104,26 @@ def test_gpu_tfm_matches_restatement(E, form):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("side", [64, 200, 520, 1030])
+def test_gpu_tfm_lane_groups(side):
+    """Images from 4 K to 1 M pixels (16, 16, 4 and 1 lanes per pixel): both forms against the
+    restatement.  The record (500 samples) holds every pulse: where a trace is still nonzero at
+    the end of the record, the forms' end-of-record skip makes pixels whose delay rounds across
+    the last sample differ between fp32 and the restatement's float64 delays."""
+    from neuralmech_amd import tfm
+
+    h, el, t0, dt, c = _capture(E=12, Nt=500)
+    x, y = np.meshgrid(np.linspace(0, 1, side), np.linspace(0, 1, side + 3), indexing="ij")
+    args32 = (h.astype(np.complex64), el.astype(np.float32), np.float32(t0), np.float32(dt), np.float32(c),
+              x.astype(np.float32), y.astype(np.float32))
+    for got, ref in ((tfm.tfm_kernel(h, el, t0, dt, c, x, y), tfm_ref.tfm_reciprocal(*args32)),
+                     (tfm.tfm(h, el, t0, dt, c, x, y), tfm_ref.tfm_all_pairs(*args32))):
+        got = got.cpu().numpy()
+        assert got.shape == x.shape
+        assert np.abs(got - ref).max() / ref.max() < 2e-5
+
+
+@pytest.mark.gpu
 def test_gpu_analytic_signal_fp32():
     from neuralmech_amd.tfm import analytic_signal
 
