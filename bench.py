@@ -740,7 +740,9 @@ def main():
                    "launch": {"variant": g.launch()[0], "rows": g.launch()[1], "tuned": tuned}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "step2d_kernel<float, K=4, VARC, SPONGE>", "bytes_per_cell": 16},
+                     "kernel": (("step2d_xl_kernel" if g.launch()[0] & 1 else "step2d_kernel")
+                                + f"<float, K=4, VARC, SPONGE, rows {g.launch()[1]}"
+                                + (", streaming>" if g.launch()[0] & 2 else ">")), "bytes_per_cell": 16},
     }
     # error of the timed fp32 run against the same shot run in fp64 on the GPU (outside the timing)
     w64 = workload(nsteps=max(nsteps, 4000), dtype=torch.float64)
