@@ -366,11 +366,13 @@ def backward(grid: WaveGrid, med: AcMedium, nsteps: int, state, src: Optional[Po
     return torch.stack([ext_view(g, G[a], med.K) for a in range(nf)]), mu_src
 
 
-def material_gradient(med: AcMedium, G: torch.Tensor, mu_src, sig, src: Optional[PointSet]):
+def material_gradient(med: AcMedium, G: torch.Tensor, mu_src, sig, src: Optional[PointSet],
+                      sum_shots: bool = False):
     """``(dJ/dmass, dJ/dstiff)`` per shot ``(nshots, n0, n1)`` float64 from the sweep's sums ``G``
     ``(2K+1, nshots, n0+K-1, n1+K-1)``: ``dJ/dk = -(G[0] + sum_n mu^{n+1} sig^n) / k``,
     ``dJ/dcn_m = G[m]``, ``dJ/dcw_m = G[K+m]``, carried to mass and stiffness through the
-    coefficient map; zero off the undamped interior."""
+    coefficient map; zero off the undamped interior.  ``sum_shots``: the sum over shots
+    ``(1, n0, n1)``, by one pass through the (linear) chain rule on the shot-summed sums."""
     g = med.grid
     K = med.K
     n0, n1 = g.shape
@@ -381,6 +383,9 @@ def material_gradient(med: AcMedium, G: torch.Tensor, mu_src, sig, src: Optional
         for s, f in enumerate(src.nodes_flat):
             if len(f):
                 Gm[s].reshape(-1).index_add_(0, torch.as_tensor(f, device=Gm.device), contrib[s, : len(f)])
+    if sum_shots:
+        Gm = Gm.sum(0, keepdim=True)
+        G = G.sum(1, keepdim=True)
     und = torch.as_tensor(med.undamped, device=Gm.device)
     dm, ds = [], []
     with torch.enable_grad():  # also inside an autograd.Function's backward
@@ -388,7 +393,7 @@ def material_gradient(med: AcMedium, G: torch.Tensor, mu_src, sig, src: Optional
         st = med.stiff.detach().clone().requires_grad_(True)
         k, cn, cw = face_operator(m, st, g.spacing, g.dt, med.closure, g.order)
         kd = k.detach()
-        for s in range(g.nshots):
+        for s in range(Gm.shape[0]):
             dk = torch.where(und, -Gm[s] / torch.where(und, kd, torch.ones_like(kd)), torch.zeros_like(kd))
             gn = torch.stack([G[a][s] for a in range(1, K + 1)])
             gw = torch.stack([G[a][s] for a in range(K + 1, 2 * K + 1)])
@@ -412,9 +417,9 @@ class _AcTraces(torch.autograd.Function):
     def backward(ctx, grad_traces):
         grid, med, src, sig, rec, strip, store, state, nsteps = ctx.keep
         G, mu_src = backward(grid, med, nsteps, state, src, sig, rec, grad_traces.contiguous(), strip, store)
-        dm, ds = material_gradient(med, G, mu_src, sig, src)
+        dm, ds = material_gradient(med, G, mu_src, sig, src, sum_shots=True)
         ctx.keep = None
-        return dm.sum(0), ds.sum(0), None, None, None, None, None, None
+        return dm[0], ds[0], None, None, None, None, None, None
 
 
 def acoustic_traces(mass, stiff, grid: WaveGrid, src: Optional[PointSet], sig, rec: PointSet,
@@ -602,9 +607,9 @@ def shots_misfit(sim, sources, gamma, sensors, observed, gradient: bool = True):
     if not gradient:
         return cost
     G, mu_src = backward(grid, med, sim.N, state, src, sig, rec, r, strip, store)
-    dm, ds = material_gradient(med, G, mu_src, sig, src)
+    dm, ds = material_gradient(med, G, mu_src, sig, src, sum_shots=True)
     d_mass, d_stiff = sim.parametrization_jacobian(gamma)
-    return cost, d_mass * dm.sum(0) + d_stiff * ds.sum(0)
+    return cost, d_mass * dm[0] + d_stiff * ds[0]
 
 
 def simulate_acoustic(sim, source=None, indicator=None, sensors=None, record_every=None):

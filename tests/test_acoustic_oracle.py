@@ -445,3 +445,24 @@ def test_high_contrast_operator_is_negative_semidefinite_and_stable(order):
     bcn, bcw = _endpoint_bonds(torch.as_tensor(g), h, order)
     Ab = _operator_matrix(bcn, bcw, shape)
     assert np.linalg.eigvalsh(0.5 * (Ab + Ab.T)).max() > 1e-3 * np.abs(ev).max()
+
+
+@pytest.mark.parametrize("order", [2, 4])
+def test_material_gradient_shot_sum_is_the_sum_of_shots(order):
+    """The chain rule is linear in the sweep's sums: one pass over the shot-summed sums equals the
+    sum of the per-shot passes."""
+    from neuralmech_amd.solver import WaveGrid
+
+    K = order // 2
+    shape = (23, 27)
+    grid = WaveGrid(shape, (0.1, 0.12), 0.01, order, torch.float64, nshots=3, device="cpu")
+    rng = np.random.default_rng(order)
+    m = rng.uniform(0.5, 2.0, shape)
+    s = rng.uniform(0.5, 2.0, shape)
+    med = ac.AcMedium(grid, m, s, "neumann", None)
+    G = torch.as_tensor(rng.standard_normal((2 * K + 1, 3, shape[0] + K - 1, shape[1] + K - 1)))
+    dm, ds = ac.material_gradient(med, G, None, None, None)
+    dm1, ds1 = ac.material_gradient(med, G, None, None, None, sum_shots=True)
+    assert dm1.shape == (1,) + shape and ds1.shape == (1,) + shape
+    for a, b in ((dm.sum(0), dm1[0]), (ds.sum(0), ds1[0])):
+        assert float((a - b).abs().max()) <= 1e-12 * float(a.abs().max())
