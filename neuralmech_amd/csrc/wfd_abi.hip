@@ -543,7 +543,8 @@ int wfd_forward2(const wfd_grid* g, void* u_a, void* u_b, void* u_c, void* u_d, 
     if (rc) return rc;
     if (g->ndim != 2) return fail(WFD_EINVAL, "wfd_forward2: 2D grids only");
     if (nsteps < 0) return fail(WFD_EINVAL, "nsteps must be >= 0");
-    if (rows != 0 && rows != 8 && rows != 16 && rows != 32) return fail(WFD_EINVAL, "wfd_forward2: rows must be 0, 8, 16 or 32");
+    if (rows != 0 && rows != 2 && rows != 4 && rows != 8 && rows != 16 && rows != 32)
+        return fail(WFD_EINVAL, "wfd_forward2: rows must be 0, 2, 4, 8, 16 or 32");
     if (!u_c || !u_d) return fail(WFD_EINVAL, "wfd_forward2: NULL work field");
     const void* f[4] = {u_a, u_b, u_c, u_d};
     for (int i = 0; i < 4; i++)

@@ -169,6 +169,187 @@ __device__ __forceinline__ void t2_tile(const StepArgs<T>& a, uint32_t tx, uint3
     }
 }
 
+// Two steps per launch with the intermediate level in LDS (rows per wave RI = 2 or 4).
+//
+// A block of 4 waves owns 4 RI output rows x 248 columns (62 chunks; lanes 1..62 of 64, lane 0's
+// chunk 4 columns left of the first output column).  Step 1 covers the 4 RI + 2K rows from K above
+// to K below the block's rows on all 64 lanes: wave w marches rows w S .. w S + S - 1 of that band
+// (S = ceil((4 RI + 2K) / 4)) exactly like a single step (window of 2K + 1 rows of u^n in
+// registers, column neighbours loaded from the adjacent chunks), writes u^{n+1} into LDS and, on
+// its own rows and lanes, to global memory.  After one barrier, wave w evaluates step 2 on its RI
+// rows from the LDS band (its window rolls over LDS rows, column neighbours by lane exchange).
+// Step-1 values are zero outside the interior, exactly as a single step leaves them, so step 2 is
+// exact on lanes 1..62.  Per output node the launch reads u^{n-1}, kc (1 + 2K / (4 RI) times) and
+// u^n (once from the cache-resident level; the halo rows are cache hits) and writes u^{n+1} and
+// u^{n+2}.
+template <typename T, int K, bool VARC, bool SPONGE, int RI>
+__device__ __forceinline__ void t2l_tile(const StepArgs<T>& a, uint32_t tx, uint32_t ty, int shot, int64_t step,
+                                         C4<T> (*band)[64])
+{
+    constexpr int W = 2 * K + 1;
+    constexpr int NR1 = 4 * RI + 2 * K;  // step-1 rows of the block
+    constexpr int S = (NR1 + 3) / 4;     // step-1 rows per wave
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) & 3);
+    const int64_t base = (int64_t)tx * 248 - 4;  // column of lane 0's chunk
+    const int64_t c = base + lane * 4;
+    const int64_t R0 = a.row_lo + (int64_t)ty * 4 * RI;  // the block's first output row
+    const int64_t nr = a.nr, nc = a.nc, ld = a.ld;
+    const bool load_ok = c < ld;  // no wrap of a lane's chunk into the next row
+    const bool own = lane >= 1 && lane <= 62 && c < nc;
+
+    const int64_t sb = (int64_t)shot * a.shot_stride;
+    const T* A = a.up + sb;
+    const T* B = a.uc + sb;
+    T* C = a.un + sb;
+    T* D = a.un2 + sb;
+    const T* coef = VARC ? a.coef + (int64_t)shot * a.coef_shot_stride : nullptr;
+
+    bool valid[4];
+    T bc[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        valid[e] = (c + e >= 1) && (c + e <= nc - 2);
+        bc[e] = T(0);
+        if constexpr (SPONGE) bc[e] = (c + e >= 0 && c + e < nc) ? a.beta_c[c + e] : T(0);
+    }
+    const int32_t* rp = a.src.count ? a.src.row_ptr + (int64_t)shot * (a.src.nrows + 1) : nullptr;
+    auto clamp_row = [&](int64_t r) { return r < 0 ? (int64_t)0 : (r > nr ? nr : r); };
+    auto node_row = [&](int64_t r) { return r >= 0 && r < nr && load_ok; };
+
+    // ---- step 1: rows R0 - K + q, q = q0 .. q1 - 1 of the band
+    const int q0 = wave * S, q1 = min(q0 + S, NR1);
+    if (q0 < q1) {
+        const int64_t rf = R0 - K + q0;  // first step-1 row of this wave
+        const bool src1 = rp && rp[clamp_row(rf)] != rp[clamp_row(R0 - K + q1)];
+        C4<T> win[W];
+#pragma unroll
+        for (int t = 1; t < W; t++) {
+            const int64_t r = rf - K + (t - 1);
+            win[t] = node_row(r) ? ld4(B + r * ld + c) : zero4<T>();
+        }
+        for (int q = q0; q < q1; q++) {
+            const int64_t r1 = R0 - K + q;
+#pragma unroll
+            for (int t = 0; t + 1 < W; t++) win[t] = win[t + 1];
+            win[W - 1] = node_row(r1 + K) ? ld4(B + (r1 + K) * ld + c) : zero4<T>();
+            const bool row1 = r1 >= 1 && r1 <= nr - 2;
+            C4<T> out = zero4<T>();
+            if (row1) {
+                const int64_t o = r1 * ld + c;
+                const C4<T> L = load_ok ? ld4(B + o - 4) : zero4<T>();
+                const C4<T> R = load_ok ? ld4(B + o + 4) : zero4<T>();
+                const C4<T> P = load_ok ? ld4(A + o) : zero4<T>();
+                C4<T> kv = zero4<T>();
+                if constexpr (VARC) kv = load_ok ? ld4(coef + o) : zero4<T>();
+                T kc[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) kc[e] = VARC ? kv.e[e] : a.kconst;
+                T b[4] = {T(0), T(0), T(0), T(0)};
+                bool damp = false;
+                if constexpr (SPONGE) {
+                    const T br = a.beta_r[r1];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        b[e] = br + bc[e];
+                        damp |= b[e] != T(0);
+                    }
+                }
+                T val[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const T lap = lap_elem<T, K, true>(a.wc, a.wr, a.wcol, win, 0, L, R, e);
+                    const T t = damp ? T(2) * win[K].e[e] - (T(1) - b[e]) * P.e[e] : T(2) * win[K].e[e] - P.e[e];
+                    val[e] = fmaT(kc[e], lap, t);
+                }
+                if (src1) wave_inject(a.src, shot, r1, base, 0, 63, step, lane, kc, val);
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const T x = damp ? val[e] / (T(1) + b[e]) : val[e];
+                    out.e[e] = valid[e] ? x : T(0);
+                }
+                if (own && r1 >= R0 && r1 < R0 + 4 * RI && r1 < a.row_hi) {
+                    const T v[4] = {out.e[0], out.e[1], out.e[2], out.e[3]};
+                    st4(C + o, v);
+                }
+            }
+            band[q][lane] = out;
+        }
+    }
+    __syncthreads();
+
+    // ---- step 2: rows R0 + wave RI .. + RI - 1, band rows q = r2 - (R0 - K)
+    const int64_t r2f = R0 + (int64_t)wave * RI;
+    if (r2f >= a.row_hi) return;  // whole wave; no barrier follows
+    const int qf = wave * RI;  // band row of r2f - K
+    const int64_t r2l = r2f + RI < a.row_hi ? r2f + RI : a.row_hi;
+    const bool src2 = rp && rp[clamp_row(r2f)] != rp[clamp_row(r2l)];
+    C4<T> sw[W];
+#pragma unroll
+    for (int t = 1; t < W; t++) sw[t] = band[qf + t - 1][lane];
+#pragma unroll
+    for (int i = 0; i < RI; i++) {
+        const int64_t r2 = r2f + i;
+        if (r2 >= a.row_hi) break;  // whole wave
+#pragma unroll
+        for (int t = 0; t + 1 < W; t++) sw[t] = sw[t + 1];
+        sw[W - 1] = band[qf + i + 2 * K][lane];
+        const int64_t o = r2 * ld + c;
+        const C4<T> L2 = from_lane_left(sw[K]), R2 = from_lane_right(sw[K]);
+        const C4<T> P2 = load_ok ? ld4(B + o) : zero4<T>();  // u_prev of step 2 is u^n
+        C4<T> kv2 = zero4<T>();
+        if constexpr (VARC) kv2 = load_ok ? ld4(coef + o) : zero4<T>();
+        T kc2[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) kc2[e] = VARC ? kv2.e[e] : a.kconst;
+        T b2[4] = {T(0), T(0), T(0), T(0)};
+        bool damp2 = false;
+        if constexpr (SPONGE) {
+            const T br = a.beta_r[r2];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                b2[e] = br + bc[e];
+                damp2 |= b2[e] != T(0);
+            }
+        }
+        T val2[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const T lap = lap_elem<T, K, true>(a.wc, a.wr, a.wcol, sw, 0, L2, R2, e);
+            const T t = damp2 ? T(2) * sw[K].e[e] - (T(1) - b2[e]) * P2.e[e] : T(2) * sw[K].e[e] - P2.e[e];
+            val2[e] = fmaT(kc2[e], lap, t);
+        }
+        if (src2) wave_inject(a.src, shot, r2, base, 1, 62, step + 1, lane, kc2, val2);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const T x = damp2 ? val2[e] / (T(1) + b2[e]) : val2[e];
+            val2[e] = valid[e] ? x : T(0);
+        }
+        if (own) st4(D + o, val2);
+    }
+}
+
+template <typename T, int K, bool VARC, bool SPONGE, int RI>
+__global__ __launch_bounds__(256) void step2d_t2l_kernel(const StepArgs<T> a)
+{
+    __shared__ C4<T> band[4 * RI + 2 * K][64];
+    const StepBlock bp = step_block(a.ngb, a.gx, a.gy);
+    const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
+    if (bp.gather) {  // the gather blocks never reach the tile barrier
+        const uint32_t half = a.ngb / 2;
+        if (bp.g < half) {
+            gather_points(a, bp.g, step);
+        } else {
+            StepArgs<T> b = a;
+            b.uc = a.up;
+            b.gather_off = a.gather_off - 1;
+            gather_points(b, bp.g - half, step);
+        }
+        return;
+    }
+    t2l_tile<T, K, VARC, SPONGE, RI>(a, bp.x, bp.y, (int)bp.z, step, band);
+}
+
 template <typename T, int K, bool VARC, bool SPONGE, int RI>
 __global__ __launch_bounds__(256) void step2d_t2_kernel(const StepArgs<T> a)
 {
@@ -200,6 +381,17 @@ static hipError_t launch_t2_k(const StepArgs<T>& a0, int rows, int nshots, hipSt
         a.gy = (uint32_t)((nrows + 4 * ri - 1) / (4 * ri));
         return dim3(step_grid((uint32_t)((int64_t)a.gx * a.gy * nshots), a.ngb));
     };
+    if (rows == 2 || rows == 4) {  // LDS band: 4 * rows output rows x 248 columns per block
+        a.gx = (uint32_t)((a.nc + 247) / 248);
+        const int r = rows;
+        a.gy = (uint32_t)((nrows + 4 * r - 1) / (4 * r));
+        const dim3 g(step_grid((uint32_t)((int64_t)a.gx * a.gy * nshots), a.ngb));
+        if (rows == 2)
+            hipLaunchKernelGGL((step2d_t2l_kernel<T, K, VARC, SPONGE, 2>), g, dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((step2d_t2l_kernel<T, K, VARC, SPONGE, 4>), g, dim3(256), 0, s, a);
+        return hipGetLastError();
+    }
     switch (rows) {
         case 8: hipLaunchKernelGGL((step2d_t2_kernel<T, K, VARC, SPONGE, 8>), grid(8), dim3(256), 0, s, a); break;
         case 32: hipLaunchKernelGGL((step2d_t2_kernel<T, K, VARC, SPONGE, 32>), grid(32), dim3(256), 0, s, a); break;
