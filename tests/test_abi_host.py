@@ -236,3 +236,28 @@ def test_launch_config_rules_need_no_gpu():
     for g in (small, cube):
         assert lib.wfd_set_config(ctypes.byref(g), 0, 0) == 0  # forget: back to the default
     assert cfg(small)[0] & 2 == 0 and cfg(cube) == (2, 32)
+
+
+def test_measurement_and_two_step_arguments_need_no_gpu():
+    """wfd_stream_bench and wfd_forward2 reject bad arguments before any launch."""
+    lib = _lib.load()
+    buf = (ctypes.c_float * 64)()
+    p = ctypes.addressof(buf)
+    p16 = (p + 15) // 16 * 16
+    assert lib.wfd_stream_bench(p16, 0, 0, p16, 8, 7, None) == _lib.WFD_EINVAL  # mode
+    assert lib.wfd_stream_bench(p16, 0, 0, p16, 8, 1, None) == _lib.WFD_EINVAL  # b, c missing
+    assert lib.wfd_stream_bench(p16, 0, 0, p16, 6, 0, None) == _lib.WFD_EINVAL  # n % 4
+    assert lib.wfd_stream_bench(p16 + 4, 0, 0, p16, 8, 0, None) == _lib.WFD_EINVAL  # alignment
+    assert lib.wfd_stream_bench(0, 0, 0, p16, 8, 0, None) == _lib.WFD_EINVAL
+    assert lib.wfd_stream_bench(p16, 0, 0, p16, 0, 0, None) == _lib.WFD_OK  # nothing to do
+    rc, g = _grid((20, 30), (1.0, 1.0))
+    med = _lib.Medium()
+    med.kconst = 0.1
+    f = [p16 + 64 * i for i in range(4)]
+    for rows in (1, 3, 5, 64):
+        assert lib.wfd_forward2(ctypes.byref(g), f[0], f[1], f[2], f[3], ctypes.byref(med), None, None, 0, 4, rows,
+                                None) == _lib.WFD_EINVAL
+        assert "rows" in _lib.last_error()
+    _, g3 = _grid((9, 11, 13), (1.0, 1.0, 1.0))
+    assert lib.wfd_forward2(ctypes.byref(g3), f[0], f[1], f[2], f[3], ctypes.byref(med), None, None, 0, 4, 2,
+                            None) == _lib.WFD_EINVAL
