@@ -48,7 +48,7 @@ def cpu_baseline(w, budget_s=15.0):
 
     from oracle import c_oracle
 
-    threads = os.cpu_count() or 1
+    threads = host_threads()
     kc = ((w["v"] * w["dt"]) ** 2).astype(np.float32)
     rec = [(45, int(j)) for j in np.round(np.linspace(1, w["n"] - 2, 512)).astype(int)]
     kw = dict(beta_profiles=w["beta"], src_nodes=[(50, w["n"] // 2)], rec_nodes=rec, nthreads=threads)
@@ -65,7 +65,37 @@ def cpu_baseline(w, budget_s=15.0):
     cells = (w["n"] - 2) ** 2 * steps
     return {"value": cells / el / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
             "sample": f"C oracle, {steps} of {w['nsteps']} steps of the same 2048^2 order-8 shot "
-                      f"(sponge, source, 512 receivers), {threads} OpenMP threads, {el:.1f} s"}
+                      f"(sponge, source, 512 receivers), {threads} OpenMP threads, {el:.1f} s",
+            "host": host_info(), "dtype": "f32"}
+
+
+def host_threads() -> int:
+    """Host threads for the CPU baseline: the CPUs this process may run on, at most OMP_NUM_THREADS
+    when that is set (the share of a machine's cores given to this job)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(omp))) if omp.isdigit() and int(omp) > 0 else max(1, n)
+
+
+def host_info() -> dict:
+    """The host the CPU baseline ran on: CPU model, os.cpu_count(), usable CPUs, torch threads."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    try:
+        import torch
+
+        tthreads = torch.get_num_threads()
+    except ImportError:
+        tthreads = None
+    return {"cpu_model": model, "os_cpu_count": os.cpu_count(), "usable_cpus": usable, "torch_threads": tthreads}
 
 
 def fwi_main(args):
@@ -611,7 +641,7 @@ def tfm_main(args):
         "scaling_rows": rows,
         "cpu_baseline": {"value": round(t_cpu * 1e3, 1), "unit": "ms/image", "cores": 1, "kind": "port",
                          "sample": "NumPy array form of tfm.py:39-53 (all pairs), 256^2 pixels, float64",
-                         "gpu_array_form_max_rel_diff": gpu_rel_diff},
+                         "gpu_array_form_max_rel_diff": gpu_rel_diff, "host": host_info()},
     }
     if args.csv_dir:
         from neuralmech_amd import harness
