@@ -74,6 +74,19 @@ def test_ricker_256_order2_1000_steps_vs_slice_form_fp32():
     assert np.array_equal(uc_g[0], uc_o) and np.array_equal(tr_g[0], tr_o)
 
 
+def test_fp64_gpu_matches_golden_fixture():
+    """The 256^2 order-2 Ricker shot in float64 on the GPU against the committed fixture of the
+    float64 slice-form restatement (tests/golden/ricker256_order2.npz)."""
+    from pathlib import Path
+
+    gold = np.load(Path(__file__).parent / "golden" / "ricker256_order2.npz")
+    case = ricker_256_order2_setup()
+    tr_g, _, uc_g = _run_gpu(case, torch.float64)
+    assert rel_l2(tr_g[0][::10], gold["traces_every_10"]) < 1e-12
+    assert rel_l2(uc_g[0].sum(1), gold["row_sums"]) < 1e-12 and rel_l2(uc_g[0].sum(0), gold["col_sums"]) < 1e-12
+    assert abs(np.linalg.norm(uc_g[0]) / float(gold["field_l2"]) - 1) < 1e-12
+
+
 def test_fp64_gpu_matches_reference_slice_form():
     case = make_case((64, 80), 8, seed=5, nsteps=200)
     tr_g, _, uc_g = _run_gpu(case, torch.float64)

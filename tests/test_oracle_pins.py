@@ -326,3 +326,22 @@ def test_pins_catch_a_dropped_sponge_term(monkeypatch):
                                  src_sig=case["sig"], rec_nodes=case["rec"])
     assert rel_l2(t_ok, t_c) < 1e-12
     assert rel_l2(t_bad, t_c) > 1e-3
+
+
+def test_golden_ricker256_order2_fixture():
+    """The committed fixture (tests/golden/make_fixtures.py): the slice-form restatement reproduces
+    it bit for bit, and the C restatement (another implementation of the same update) agrees to
+    float64 rounding."""
+    from pathlib import Path
+
+    from tests.golden.make_fixtures import compute
+    from tests.wave_cases import oracle_forward, ricker_256_order2_setup
+
+    gold = np.load(Path(__file__).parent / "golden" / "ricker256_order2.npz")
+    now = compute()
+    for k in ("traces_every_10", "row_sums", "col_sums", "field_l2"):
+        assert np.array_equal(now[k], gold[k]), k
+    tr, _, uc = oracle_forward(ricker_256_order2_setup(), np.float64)
+    assert rel_l2(tr[::10], gold["traces_every_10"]) < 1e-12
+    assert rel_l2(uc.sum(1), gold["row_sums"]) < 1e-12 and rel_l2(uc.sum(0), gold["col_sums"]) < 1e-12
+    assert abs(np.linalg.norm(uc) / float(gold["field_l2"]) - 1) < 1e-12
