@@ -143,8 +143,8 @@ int wfd_forward(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med,
  * and writes u^{n+1}, u^{n+2} (20 bytes per node per two steps instead of 32), recomputing a
  * 2K-row / 8-column halo of step 1; bit-identical to single steps.  u_c, u_d are work fields of
  * the same size (all four distinct).  On return u_a = u^{N-1}, u_b = u^N.  rows = output rows
- * per wave: 8, 16, 32 (0 = 16) march both levels in registers; 2, 4 keep the intermediate level
- * of a 4-wave block's rows in LDS.  Replaces temporal_step (step2D_wave.cu:244-304) and its 4-array
+ * per wave: 2, 4, 8 (0 = 8) keep the intermediate level of a 4-wave block's rows in LDS; 16, 32
+ * march both levels in registers.  Replaces temporal_step (step2D_wave.cu:244-304) and its 4-array
  * rotation (2D_wave_cuda_temporal.py:110-112) for the general update. */
 int wfd_forward2(const wfd_grid* g, void* u_a, void* u_b, void* u_c, void* u_d,
                  const wfd_medium* med, const wfd_points* src, const wfd_points* rec,

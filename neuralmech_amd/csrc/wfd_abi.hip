@@ -568,7 +568,7 @@ int wfd_forward2(const wfd_grid* g, void* u_a, void* u_b, void* u_c, void* u_d, 
         // u^{n+1} -> X, u^{n+2} -> Y; the launch records u^{n-1}, u^n into rows n-2, n-1
         Launch L{g, P, C, X, med, src, rec, step0 + k, nullptr, nullptr, nullptr, false, default_cfg(g), -1};
         L.un2 = Y;
-        L.t2rows = rows ? rows : 16;
+        L.t2rows = rows ? rows : 8;
         rc = do_launch(L, st);
         if (rc) return rc;
         std::swap(P, X);

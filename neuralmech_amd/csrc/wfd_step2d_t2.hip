@@ -169,7 +169,7 @@ __device__ __forceinline__ void t2_tile(const StepArgs<T>& a, uint32_t tx, uint3
     }
 }
 
-// Two steps per launch with the intermediate level in LDS (rows per wave RI = 2 or 4).
+// Two steps per launch with the intermediate level in LDS (rows per wave RI = 2, 4 or 8).
 //
 // A block of 4 waves owns 4 RI output rows x 248 columns (62 chunks; lanes 1..62 of 64, lane 0's
 // chunk 4 columns left of the first output column).  Step 1 covers the 4 RI + 2K rows from K above
@@ -381,19 +381,20 @@ static hipError_t launch_t2_k(const StepArgs<T>& a0, int rows, int nshots, hipSt
         a.gy = (uint32_t)((nrows + 4 * ri - 1) / (4 * ri));
         return dim3(step_grid((uint32_t)((int64_t)a.gx * a.gy * nshots), a.ngb));
     };
-    if (rows == 2 || rows == 4) {  // LDS band: 4 * rows output rows x 248 columns per block
+    if (rows == 2 || rows == 4 || rows == 8) {  // LDS band: 4 * rows output rows x 248 columns per block
         a.gx = (uint32_t)((a.nc + 247) / 248);
         const int r = rows;
         a.gy = (uint32_t)((nrows + 4 * r - 1) / (4 * r));
         const dim3 g(step_grid((uint32_t)((int64_t)a.gx * a.gy * nshots), a.ngb));
         if (rows == 2)
             hipLaunchKernelGGL((step2d_t2l_kernel<T, K, VARC, SPONGE, 2>), g, dim3(256), 0, s, a);
-        else
+        else if (rows == 4)
             hipLaunchKernelGGL((step2d_t2l_kernel<T, K, VARC, SPONGE, 4>), g, dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((step2d_t2l_kernel<T, K, VARC, SPONGE, 8>), g, dim3(256), 0, s, a);
         return hipGetLastError();
     }
-    switch (rows) {
-        case 8: hipLaunchKernelGGL((step2d_t2_kernel<T, K, VARC, SPONGE, 8>), grid(8), dim3(256), 0, s, a); break;
+    switch (rows) {  // register march: 16 or 32 output rows per wave
         case 32: hipLaunchKernelGGL((step2d_t2_kernel<T, K, VARC, SPONGE, 32>), grid(32), dim3(256), 0, s, a); break;
         default: hipLaunchKernelGGL((step2d_t2_kernel<T, K, VARC, SPONGE, 16>), grid(16), dim3(256), 0, s, a); break;
     }

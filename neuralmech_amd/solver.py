@@ -301,8 +301,8 @@ def run(grid: WaveGrid, medium: Medium, nsteps: int, src: Optional[PointSet] = N
     ``traces[s, n, r] = u^{step0+n+1}`` at receiver ``r`` of shot ``s``; ``state`` is
     ``(u_prev, u_cur)`` layout tensors (rest state when omitted, updated in place otherwise).
     ``graph_chunk`` > 0 replays a captured hipGraph of that many steps; ``two_step`` > 0 (2D)
-    runs two steps per launch with that many output rows per wave (2 or 4: the intermediate level
-    in LDS; 8, 16 or 32: in registers), using two scratch fields; the results are the same bits.
+    runs two steps per launch with that many output rows per wave (2, 4 or 8: the intermediate
+    level in LDS; 16 or 32: in registers), using two scratch fields; the results are the same bits.
     """
     lib = _lib.load()
     if state is None:

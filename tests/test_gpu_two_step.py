@@ -40,7 +40,7 @@ def test_two_step_matches_oracle_fp32_and_fp64():
     for dtype, npd in [(torch.float32, np.float32), (torch.float64, np.float64)]:
         case = make_case((90, 260), 8, seed=3, nsrc=2, nrec=7, nsteps=24)
         print("dtype", dtype, flush=True)
-        for rows in (16, 4):
+        for rows in (16, 8, 4):
             tr, up, uc = _run(case, dtype, two_step=rows)
             tr_o, up_o, uc_o = oracle_forward(case, npd)
             assert np.array_equal(uc[0].cpu().numpy(), uc_o) and np.array_equal(up[0].cpu().numpy(), up_o)
