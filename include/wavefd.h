@@ -372,6 +372,13 @@ int wfd_ac_backward(const wfd_grid* g, void* u_a, void* u_b, void* mu_a, void* m
                     const wfd_points* strip, const wfd_points* mu_src, int64_t nsteps,
                     void* g_work, double* G, wfd_stream_t stream);
 
+/* Rows per wave of the wfd_ac_* kernels: 0 (default) chooses per launch (the forward halves 8
+ * rows until about 16 K waves are in flight; the adjoint aims at 32 K waves while its 8 + 4K
+ * fields fit in 192 MiB and at 8 K otherwise); 1, 2, 4 or 8 forces that count for every later
+ * launch in the process.  Results do not depend on it (the same expression at every node); it
+ * exists so that every launch shape can be tested and timed.  No counterpart in the reference. */
+int wfd_ac_set_rows(int32_t rows);
+
 /* ------------------------------------------------------------- total focusing method */
 /* Delay and sum of a full matrix capture onto P pixels: h = analytic traces, complex64 as
  * (re, im) float pairs, [E][E][Nt] (emitter, receiver, sample); element e at (xe[e], ye[e]),

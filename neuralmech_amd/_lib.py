@@ -150,6 +150,7 @@ SIGNATURES = {
     "wfd_ac_forward": (ctypes.c_int, [_GP, _P, _P, ctypes.POINTER(AcMedium), _PP, _PP, _PP, _I64, _I64, _P]),
     "wfd_ac_backward": (ctypes.c_int, [_GP, _P, _P, _P, _P, ctypes.POINTER(AcMedium), _PP, _PP, _PP, _PP, _I64,
                                        _P, _P, _P]),
+    "wfd_ac_set_rows": (ctypes.c_int, [ctypes.c_int32]),
     "wfd_ref_step2d": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, _P]),

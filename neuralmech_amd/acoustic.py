@@ -298,6 +298,13 @@ def _strip(und: np.ndarray, K: int) -> np.ndarray:
 
 
 # ------------------------------------------------------------------------- runs
+def set_rows(rows: int = 0) -> None:
+    """Rows per wave of the variable mass / stiffness kernels for every later launch in this
+    process: 0 lets each launch choose, 1 / 2 / 4 / 8 force that count (results are the same
+    bits whatever it is)."""
+    check(_lib.load().wfd_ac_set_rows(int(rows)))
+
+
 def forward(grid: WaveGrid, med: AcMedium, nsteps: int, src: Optional[PointSet] = None, sig=None,
             rec: Optional[PointSet] = None, strip: Optional[PointSet] = None, state=None, step0: int = 0):
     """Advance ``nsteps`` steps: ``(traces, store, (p_prev, p_cur))``.

@@ -19,6 +19,8 @@
 // the forward recorded.  Then dJ/dk = -(Gm + sum_n mu^{n+1} sig^n) / k and dJ/dc = F.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cstring>
 
 #include "wfd_internal.h"
@@ -63,6 +65,9 @@ struct AcArgs {
     int32_t nshots;
     int64_t step;
 };
+
+// rows per wave forced by wfd_ac_set_rows (0: chosen per launch)
+std::atomic<int> g_ac_rows{0};
 
 template <typename T>
 __device__ __forceinline__ T ld0(const T* f, int64_t o, bool in)
@@ -315,6 +320,8 @@ hipError_t ac_launch_k(AcArgs<T> a, bool tiles_on, hipStream_t s)
     const uint64_t target = !BWD ? 16384u : (bwd_bytes <= ((uint64_t)192 << 20) ? 32768u : 8192u);
     int ri = 8;
     while (ri > 1 && waves(ri) < target) ri /= 2;
+    const int forced = g_ac_rows.load(std::memory_order_relaxed);
+    if (forced) ri = forced;
     a.gy = (uint32_t)((a.n0 - 2 + 4 * ri - 1) / (4 * ri));
     const uint32_t tiles = tiles_on ? a.gx * a.gy * (uint32_t)a.nshots : 0u;
     const uint32_t blocks = step_grid(tiles, a.nga + a.ngb + a.nsc);
@@ -551,4 +558,12 @@ extern "C" int wfd_ac_backward(const wfd_grid* g, void* u_a, void* u_b, void* mu
     return g->dtype == WFD_F64
                ? ac_backward_t<double>(g, u_a, u_b, mu_a, mu_b, med, src, adj, strip, mu_src, nsteps, g_work, G, s)
                : ac_backward_t<float>(g, u_a, u_b, mu_a, mu_b, med, src, adj, strip, mu_src, nsteps, g_work, G, s);
+}
+
+extern "C" int wfd_ac_set_rows(int32_t rows)
+{
+    if (rows != 0 && rows != 1 && rows != 2 && rows != 4 && rows != 8)
+        return set_error(WFD_EINVAL, "wfd_ac_set_rows: rows must be 0, 1, 2, 4 or 8");
+    g_ac_rows.store(rows, std::memory_order_relaxed);
+    return WFD_OK;
 }

@@ -261,3 +261,11 @@ def test_measurement_and_two_step_arguments_need_no_gpu():
     _, g3 = _grid((9, 11, 13), (1.0, 1.0, 1.0))
     assert lib.wfd_forward2(ctypes.byref(g3), f[0], f[1], f[2], f[3], ctypes.byref(med), None, None, 0, 4, 2,
                             None) == _lib.WFD_EINVAL
+
+
+def test_acoustic_rows_override_validation():
+    lib = _lib.load()
+    for bad in (-1, 3, 5, 16):
+        assert lib.wfd_ac_set_rows(bad) == _lib.WFD_EINVAL
+    for ok in (1, 2, 4, 8, 0):
+        assert lib.wfd_ac_set_rows(ok) == _lib.WFD_OK

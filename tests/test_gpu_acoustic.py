@@ -229,9 +229,10 @@ def test_autograd_function_matches_material_gradient():
 
 
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
-def test_bitexact_on_launches_with_eight_rows_per_wave(dtype):
-    """16 shots of 258 x 520 nodes: enough waves that each wave marches 8 rows (small launches
-    march 4); shots 0 and 15 against the restatement, forward and backward, order 4."""
+def test_bitexact_on_every_rows_per_wave(dtype):
+    """16 shots of 258 x 520 nodes, forward and backward, order 4, with each wave marching 1, 2, 4
+    and 8 rows (forced; the automatic choice takes 2 forward and 1 backward here, 4 / 8 on large
+    grids); shots 0 and 15 against the restatement every time."""
     from neuralmech_amd import acoustic as ac
     from neuralmech_amd.solver import PointSet, WaveGrid
 
@@ -250,18 +251,24 @@ def test_bitexact_on_launches_with_eight_rows_per_wave(dtype):
     sn = med.strip_nodes()
     strip = PointSet(grid, [tuple(int(c) for c in p) for p in sn])
     sg = torch.as_tensor(sig)
-    tr, store, state = ac.forward(grid, med, N, src, sg, rec, strip)
-    dJ = (tr - 0.5 * tr).to(tr.dtype)
-    G, mu_src = ac.backward(grid, med, N, state, src, sg, rec, dJ, strip, store)
-    torch.cuda.synchronize()
     npdt = np.float32 if dtype == "float32" else np.float64
     k, cn, cw = (f.cpu().numpy().astype(npdt) for f in med.bonds())
     prof = tuple(med.profiles)
     otr, ostore, last = ar.forward_np(k, cn, cw, N, prof, src_nodes, sig, rec_nodes, sn, dtype=npdt)
     adj = np.zeros((N + 1, len(rec_nodes)), npdt)
-    adj[1:] = -dJ[0].cpu().numpy()
+    adj[1:] = -0.5 * otr
     OG, omu = ar.backward_np(k, cn, cw, N, last, prof, src_nodes, sig, rec_nodes, adj, sn, ostore, dtype=npdt)
-    for shot in (0, 15):
-        assert np.array_equal(tr[shot].cpu().numpy(), otr)
-        assert np.array_equal(G[:, shot].cpu().numpy(), OG)
-        assert np.array_equal(mu_src[shot].cpu().numpy(), omu)
+    try:
+        for rows in (1, 2, 4, 8, 0):
+            print(f"\n[acoustic] 16 shots 258x520 {dtype} rows={rows}", flush=True)
+            ac.set_rows(rows)
+            tr, store, state = ac.forward(grid, med, N, src, sg, rec, strip)
+            dJ = torch.as_tensor(0.5 * otr, device=tr.device).expand_as(tr).contiguous()
+            G, mu_src = ac.backward(grid, med, N, state, src, sg, rec, dJ, strip, store)
+            torch.cuda.synchronize()
+            for shot in (0, 15):
+                assert np.array_equal(tr[shot].cpu().numpy(), otr), rows
+                assert np.array_equal(G[:, shot].cpu().numpy(), OG), rows
+                assert np.array_equal(mu_src[shot].cpu().numpy(), omu), rows
+    finally:
+        ac.set_rows(0)
