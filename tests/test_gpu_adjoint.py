@@ -155,3 +155,24 @@ def test_adjoint_launch_shapes_bitexact(variant, rows):
     finally:
         g.set_launch(0, 0)
     assert torch.equal(tr, base[1]) and torch.equal(grad, base[3])
+
+
+@pytest.mark.parametrize("variant,rows", [(0, 4), (2, 4), (2, 16)])
+def test_adjoint_3d_launch_shapes_bitexact(variant, rows):
+    """3D gradient plan (forward, recompute, adjoint with imaging) under forced launch shapes,
+    streaming access included, against the default shape: same traces and gradient bits."""
+    from neuralmech_amd import solver
+
+    case = make_case((12, 40, 75), 8, seed=31, nsteps=24, nrec=5)
+    base = _plan_grad(case, torch.float32, seg=5)
+    g, med, src, rec = gpu_objects(case, torch.float32)
+    g.set_launch(variant, rows)
+    try:
+        plan = solver.GradientPlan(g, 24, seg=5)
+        tr = plan.forward(med, src, case["sig"], rec)
+        grad = plan.backward(med, src, rec, tr)
+        torch.cuda.synchronize()
+    finally:
+        g.set_launch(0, 0)
+    assert torch.equal(tr, base[1]) and torch.equal(grad, base[3])
+
