@@ -63,11 +63,13 @@ class ShotBatch:
         self.shots = list(shots)
         self._keep = []
         arr = (_lib.BatchShot * len(shots))()
-        self.grids, self.media, self.srcs, self.recs = [], [], [], []
+        self.grids, self.media, self.srcs, self.recs, self.vel = [], [], [], [], []
         self.sig, self.traces, self.obs, self.adj, self.mu_src, self.G = [], [], [], [], [], []
         for k, sh in enumerate(shots):
             g = WaveGrid(sh.shape, sh.spacing, sh.dt, sh.order, dtype, nshots=1, device=self.device)
-            med = Medium(g, velocity=torch.as_tensor(sh.velocity, dtype=torch.float64), beta_profiles=sh.beta_profiles)
+            v64 = torch.as_tensor(sh.velocity, dtype=torch.float64)
+            med = Medium(g, velocity=v64, beta_profiles=sh.beta_profiles)
+            v64 = v64.to(self.device)  # kept on the device for dJ/dv = dJ/dkc 2 v dt^2
             src = PointSet(g, [tuple(int(c) for c in p) for p in sh.src_nodes])
             rec = PointSet(g, [tuple(int(c) for c in p) for p in sh.rec_nodes])
             N = int(sh.nsteps)
@@ -94,6 +96,7 @@ class ShotBatch:
             c.G = G.data_ptr()
             self.grids.append(g)
             self.media.append(med)
+            self.vel.append(v64)
             self.srcs.append(src)
             self.recs.append(rec)
             self.sig.append(sig)
@@ -168,8 +171,7 @@ class ShotBatch:
             mu = self.mu_src[k][:, :, : src.count] if src.count else None
             kc = self.media[k].kc_nodes()[0]
             gk = kc_gradient(g, self.imaging_sum(k).clone(), mu, self.sig[k], src if src.count else None, kc)
-            v = torch.as_tensor(self.shots[k].velocity, dtype=torch.float64, device=self.device)
-            out.append((J, gk * 2.0 * v * g.dt ** 2))
+            out.append((J, gk * 2.0 * self.vel[k] * g.dt ** 2))
         return out
 
 
