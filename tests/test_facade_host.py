@@ -142,3 +142,32 @@ def test_fwi_driver_helpers():
     y = coords[1][m]
     assert y.min() >= 0.1 - 0.05 and y.max() <= 0.5 + 0.05
     assert torch.equal(utils.threshold(torch.tensor([0.2, 0.5, 0.9])), torch.tensor([0.0, 1.0, 1.0]))
+
+
+def test_gradient_plan_cache_leases(monkeypatch):
+    """The kept gradient plan is reused only when no forward holds it; a second outstanding
+    forward gets a plan of its own; releasing (or dropping) a lease frees the kept plan."""
+    from neuralmech_amd import autograd, solver
+
+    made = []
+
+    class FakePlan:
+        def __init__(self, grid, nsteps, seg=0, max_bytes=0):
+            made.append(self)
+
+    monkeypatch.setattr(autograd, "GradientPlan", FakePlan)
+    autograd.clear_plans()
+    g = solver.WaveGrid((20, 30), (1.0, 1.0), 0.1, 4, torch.float32, device="cpu")
+    p1 = autograd._plan_for(g, 50, 0, 0)
+    lease1 = autograd._Lease(p1)
+    p2 = autograd._plan_for(g, 50, 0, 0)
+    assert p2 is not p1 and len(made) == 2  # kept plan leased: a fresh one
+    lease1.release()
+    assert autograd._plan_for(g, 50, 0, 0) is p1 and len(made) == 2
+    lease = autograd._Lease(p1)
+    del lease  # a dropped graph releases its lease
+    assert autograd._plan_for(g, 50, 0, 0) is p1
+    g2 = solver.WaveGrid((20, 31), (1.0, 1.0), 0.1, 4, torch.float32, device="cpu")
+    p3 = autograd._plan_for(g2, 50, 0, 0)
+    assert p3 is not p1 and autograd._plan_for(g, 50, 0, 0) is not p1  # one plan kept at a time
+    autograd.clear_plans()

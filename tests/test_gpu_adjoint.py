@@ -176,3 +176,31 @@ def test_adjoint_3d_launch_shapes_bitexact(variant, rows):
         g.set_launch(0, 0)
     assert torch.equal(tr, base[1]) and torch.equal(grad, base[3])
 
+
+
+def test_two_outstanding_forwards_keep_their_own_plans():
+    """wave_traces keeps its gradient plan for the next call; two forwards alive at once (one
+    backward through both) must not share it: gradients equal one-at-a-time runs bit for bit, and
+    a later call reuses the kept plan with the same result."""
+    from neuralmech_amd import autograd
+
+    case = make_case((40, 60), 4, seed=5, nsteps=30, nrec=4)
+    g, med, src, rec = gpu_objects(case, torch.float64)
+    sig = torch.as_tensor(case["sig"], device="cuda")
+    v0 = torch.as_tensor(case["v"], dtype=torch.float64, device="cuda")
+
+    def grad_of(v):
+        w = v.clone().requires_grad_(True)
+        (autograd.wave_traces(w, g, src, sig, rec, case["prof"]) ** 2).sum().backward()
+        return w.grad
+
+    autograd.clear_plans()
+    a = v0.clone().requires_grad_(True)
+    b = (1.1 * v0).requires_grad_(True)
+    da = autograd.wave_traces(a, g, src, sig, rec, case["prof"])
+    db = autograd.wave_traces(b, g, src, sig, rec, case["prof"])
+    ((da ** 2).sum() + (db ** 2).sum()).backward()
+    ga, gb = grad_of(v0), grad_of(1.1 * v0)
+    assert torch.equal(a.grad, ga) and torch.equal(b.grad, gb)
+    assert torch.equal(grad_of(v0), ga)  # the kept plan again
+    autograd.clear_plans()
