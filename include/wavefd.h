@@ -5,7 +5,9 @@
  * inside wfd_plan and wfd_batch objects.  Every launch is stream-ordered on the caller's hipStream_t (passed as
  * void*, NULL = the default stream) and the library never synchronises the host except where a
  * function says so.  Functions return WFD_OK (0) or a positive status; wfd_last_error() gives a
- * message (thread-local).
+ * message (thread-local).  Calls are re-entrant across threads and streams; the only process-wide
+ * state is the launch-shape table (wfd_tune / wfd_set_config, guarded by a mutex) and the rows
+ * override of wfd_ac_set_rows (atomic), neither of which changes results.
  *
  * Field layout ("halo layout").  A field of a grid with nodes n[0..ndim-1] (slowest axis first,
  * boundary ring included) is stored as
