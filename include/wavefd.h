@@ -2,12 +2,16 @@
  *
  * Plain pointers and sizes only.  Every device pointer is a HIP device allocation owned by the
  * caller (typically a torch tensor's data_ptr()); the library allocates device memory only
- * inside wfd_plan and wfd_batch objects.  Every launch is stream-ordered on the caller's hipStream_t (passed as
- * void*, NULL = the default stream) and the library never synchronises the host except where a
- * function says so.  Functions return WFD_OK (0) or a positive status; wfd_last_error() gives a
- * message (thread-local).  Calls are re-entrant across threads and streams; the only process-wide
- * state is the launch-shape table (wfd_tune / wfd_set_config, guarded by a mutex) and the rows
- * override of wfd_ac_set_rows (atomic), neither of which changes results.
+ * inside wfd_plan, wfd_batch and wfd_fgraph objects (wfd_forward_graph keeps its wfd_fgraph
+ * objects in a cache) and for the flag of the host-synchronising wfd_check_medium.  Every launch
+ * is stream-ordered on the caller's hipStream_t (passed as void*, NULL = the default stream) and
+ * the library never synchronises the host except where a function says so.  Functions return
+ * WFD_OK (0) or a positive status; wfd_last_error() gives a message (thread-local).  Calls are
+ * re-entrant across threads and streams; the process-wide state is the launch-shape table
+ * (wfd_tune / wfd_set_config, guarded by a mutex), the rows override of wfd_ac_set_rows (atomic)
+ * -- neither of which changes results -- and the graph cache of wfd_forward_graph (guarded by a
+ * mutex; one step counter per cached graph and stream).  Two calls that write the same fields
+ * race, as any two writers do.
  *
  * Field layout ("halo layout").  A field of a grid with nodes n[0..ndim-1] (slowest axis first,
  * boundary ring included) is stored as
@@ -152,9 +156,28 @@ int wfd_forward2(const wfd_grid* g, void* u_a, void* u_b, void* u_c, void* u_d,
                  const wfd_medium* med, const wfd_points* src, const wfd_points* rec,
                  int64_t step0, int64_t nsteps, int32_t rows, wfd_stream_t stream);
 
-/* Same as wfd_forward, replayed from a captured hipGraph of `chunk` steps (chunk even, the
- * reference's 20-step graph of 2D_wave_cuda_graph.py:73-79,94-95).  The step index travels in
- * device memory, so one graph serves every chunk; the remainder runs as plain launches. */
+/* A forward run of `chunk` steps (chunk even) on the fields (u_a, u_b) with this medium and these
+ * point lists, captured once into a hipGraph and replayed per chunk: the reference's captured
+ * 20-step graph, uploaded once and launched repeatedly (2D_wave_cuda_graph.py:73-79,91,94-95).
+ * The step index of every launch is read from a device counter the object owns, set before each
+ * run (by a kernel) and advanced by the graph's last node, so one graph serves every chunk of a
+ * run and objects never share a counter.  wfd_fgraph_run(step0, nsteps) equals
+ * wfd_forward(u_a, u_b, ..., step0, nsteps) bit for bit: nsteps / chunk replays, the remainder as
+ * plain launches; u_a, u_b rotate as there.  The buffers, the medium's fields and the point
+ * lists' arrays must stay allocated while the object exists (their descriptors are copied at
+ * creation).  Runs of one object must not overlap in time (they share its counter); objects on
+ * different streams run concurrently.  wfd_fgraph_destroy waits for the object's last run. */
+typedef struct wfd_fgraph wfd_fgraph;
+int wfd_fgraph_create(wfd_fgraph** fg, const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med,
+                      const wfd_points* src, const wfd_points* rec, int32_t chunk);
+int wfd_fgraph_run(wfd_fgraph* fg, int64_t step0, int64_t nsteps, wfd_stream_t stream);
+int wfd_fgraph_destroy(wfd_fgraph* fg);
+
+/* Same as wfd_forward, replayed from a captured hipGraph of `chunk` steps (chunk even).  Graphs
+ * are kept between calls in a process-wide cache (16 entries, least recently used dropped) keyed
+ * by the stream and everything a capture depends on, so repeated calls capture once and calls on
+ * different streams never share a step counter; runs shorter than one chunk are plain launches.
+ * Same contract as wfd_fgraph_run (replaces the loop of 2D_wave_cuda_graph.py:91-96). */
 int wfd_forward_graph(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med,
                       const wfd_points* src, const wfd_points* rec, int64_t step0,
                       int64_t nsteps, int32_t chunk, wfd_stream_t stream);
@@ -169,8 +192,16 @@ int wfd_adjoint_step(const wfd_grid* g, const void* mu_prev, const void* mu_cur,
 /* G[x] += sum over shots of g_partial[shot][x] (G float64, one field), then g_partial = 0. */
 int wfd_grad_flush(const wfd_grid* g, void* g_partial, double* G, wfd_stream_t stream);
 
-/* Write 1 to *flag (device int) if any coefficient / beta entry is NaN/Inf or kc <= 0.
- * wfd_check_medium is the host-synchronising variant: it returns WFD_ENONFINITE itself. */
+/* Medium check.  wfd_check_medium_async ORs into *flag (device int32, cleared by the caller):
+ * 1 if a coefficient or damping entry is NaN/Inf, a coefficient is <= 0 or a damping entry < 0;
+ * 2 if a coefficient exceeds the stability (CFL) limit of the grid's order and spacing,
+ * kc * sum_d S / h_d^2 <= 4 with S = |a_0| + 2 sum |a_k| (wfd_stable_dt's bound; relative slack
+ * 1e-6).  A constant coefficient (coef == NULL) is checked on the host: WFD_ENONFINITE or, above
+ * the limit, WFD_EINVAL, returned at once.  wfd_check_medium is the host-synchronising variant:
+ * it returns WFD_ENONFINITE for flag 1 and WFD_EINVAL for flag 2.  The stream-ordered entry
+ * points check a constant coefficient the same way (WFD_EINVAL above the limit); a coefficient
+ * field is checked only by these two functions, which the Python layer runs once per medium.
+ * The reference's driver fixes dt below this limit by construction (2D_wave_cupy.py:37-39). */
 int wfd_check_medium_async(const wfd_grid* g, const wfd_medium* med, int32_t* flag,
                            wfd_stream_t stream);
 int wfd_check_medium(const wfd_grid* g, const wfd_medium* med, wfd_stream_t stream);

@@ -120,6 +120,9 @@ SIGNATURES = {
     "wfd_step": (ctypes.c_int, [_GP, _P, _P, _P, _MP, _PP, _PP, _I64, _P]),
     "wfd_forward": (ctypes.c_int, [_GP, _P, _P, _MP, _PP, _PP, _I64, _I64, _P]),
     "wfd_forward_graph": (ctypes.c_int, [_GP, _P, _P, _MP, _PP, _PP, _I64, _I64, ctypes.c_int32, _P]),
+    "wfd_fgraph_create": (ctypes.c_int, [ctypes.POINTER(_P), _GP, _P, _P, _MP, _PP, _PP, ctypes.c_int32]),
+    "wfd_fgraph_run": (ctypes.c_int, [_P, _I64, _I64, _P]),
+    "wfd_fgraph_destroy": (ctypes.c_int, [_P]),
     "wfd_forward2": (ctypes.c_int, [_GP, _P, _P, _P, _P, _MP, _PP, _PP, _I64, _I64, ctypes.c_int32, _P]),
     "wfd_adjoint_step": (ctypes.c_int, [_GP, _P, _P, _P, _MP, _PP, _PP, _I64, _P, _P, _P]),
     "wfd_grad_flush": (ctypes.c_int, [_GP, _P, _P, _P]),

@@ -306,14 +306,18 @@ def set_rows(rows: int = 0) -> None:
 
 
 def forward(grid: WaveGrid, med: AcMedium, nsteps: int, src: Optional[PointSet] = None, sig=None,
-            rec: Optional[PointSet] = None, strip: Optional[PointSet] = None, state=None, step0: int = 0):
-    """Advance ``nsteps`` steps: ``(traces, store, (p_prev, p_cur))``.
+            rec: Optional[PointSet] = None, strip: Optional[PointSet] = None, state=None, step0=None):
+    """Advance ``nsteps`` steps: ``(traces, store, state)``.
 
     ``traces[s, n, r]`` = p^{step0+n+1} at receiver r; ``store[s, n, i]`` = p^{step0+n} at strip node
-    i for n = 0..nsteps.  ``state`` is updated in place (rest when omitted).
+    i for n = 0..nsteps.  ``state`` is a ``solver.WaveState`` (``prev = p^{n-1}``, ``cur = p^n``,
+    ``step = n``): rest at ``step0`` (default 0) when omitted, otherwise continued from
+    ``state.step`` and advanced in place, levels and step, so that chained calls equal one call.
     """
     lib = _lib.load()
-    ua, ub = (grid.zeros(), grid.zeros()) if state is None else state
+    state = solver._as_state(grid, state, step0)
+    step0 = state.step
+    ua, ub = state.prev, state.cur
     total = step0 + nsteps
     s_sig = solver._sig_tensor(grid, src, sig, total)
     traces = None
@@ -328,11 +332,12 @@ def forward(grid: WaveGrid, med: AcMedium, nsteps: int, src: Optional[PointSet] 
     check(lib.wfd_ac_forward(ctypes.byref(grid.c), _ptr(ua), _ptr(ub), ctypes.byref(med.c),
                              ctypes.byref(sp) if sp else None, ctypes.byref(rp) if rp else None,
                              ctypes.byref(tp) if tp else None, int(step0), int(nsteps), _stream_ptr(grid.device)))
-    if nsteps % 2:
-        ua, ub = ub, ua
+    if nsteps % 2:  # the newest level is now in the buffer that held p^{n-1}
+        state.prev, state.cur = state.cur, state.prev
+    state.step += int(nsteps)
     tr = None if traces is None else traces[:, step0:]
     st = None if store is None else store[:, step0:]
-    return tr, st, (ua, ub)
+    return tr, st, state
 
 
 def backward(grid: WaveGrid, med: AcMedium, nsteps: int, state, src: Optional[PointSet], sig, rec: PointSet,
@@ -624,12 +629,12 @@ def simulate_acoustic(sim, source=None, indicator=None, sensors=None, record_eve
     grid, med = _medium(sim, indicator)
     src, sig = _source_setup(grid, source, sim.N, _scale_of(sim))
     rec = PointSet(grid, _node_list(sensors)) if sensors is not None else None
-    state = (grid.zeros(), grid.zeros())
+    state = solver.WaveState.rest(grid)
     frames, traces, step = [], [], 0
     chunk = int(record_every) if record_every else sim.N
     while step < sim.N:
         n = min(chunk, sim.N - step)
-        tr, _, state = forward(grid, med, n, src, sig, rec, state=state, step0=step)
+        tr, _, state = forward(grid, med, n, src, sig, rec, state=state)
         if tr is not None:
             traces.append(tr[0])
         step += n

@@ -332,6 +332,8 @@ int do_launch(const Launch& L, hipStream_t s)
     return L.g->dtype == WFD_F64 ? launch_t<double>(L, s) : launch_t<float>(L, s);
 }
 
+int check_kconst(const wfd_grid* g, const wfd_medium* med);
+
 int check_step_args(const wfd_grid* g, const void* up, const void* uc, void* un, const wfd_medium* med)
 {
     int rc = check_grid(g);
@@ -342,8 +344,8 @@ int check_step_args(const wfd_grid* g, const void* up, const void* uc, void* un,
     if (!med) return fail(WFD_EINVAL, "medium is NULL");
     if (med->coef && med->coef_shot_stride != 0 && med->coef_shot_stride != g->elems)
         return fail(WFD_EINVAL, "coef_shot_stride must be 0 or elems");
-    if (!med->coef && !(med->kconst > 0.0 && std::isfinite(med->kconst)))
-        return fail(WFD_ENONFINITE, "constant coefficient must be finite and > 0");
+    rc = check_kconst(g, med);
+    if (rc) return rc;
     const uintptr_t align = 4 * esize(g);
     if (((uintptr_t)up | (uintptr_t)uc | (uintptr_t)un) % align ||
         (med->coef && (uintptr_t)med->coef % align))
@@ -351,24 +353,205 @@ int check_step_args(const wfd_grid* g, const void* up, const void* uc, void* un,
     return WFD_OK;
 }
 
-// one device counter per device for graph replay
-std::mutex g_ctr_mu;
-std::vector<int64_t*> g_ctr;
-
-int device_counter(int64_t** out)
+// Largest stable kc = v^2 dt^2 of the grid's order and spacing: the leapfrog (damped or not) is
+// stable for kc * sum_d S / h_d^2 <= 4, S = |a_0| + 2 sum |a_k| (the symbol's largest magnitude);
+// a relative slack of 1e-6 admits a dt computed at the limit and rounded to the grid dtype.
+double kc_limit(const wfd_grid* g)
 {
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
-    std::lock_guard<std::mutex> lk(g_ctr_mu);
-    if ((int)g_ctr.size() <= dev) g_ctr.resize(dev + 1, nullptr);
-    if (!g_ctr[dev]) HIP_TRY(hipMalloc(&g_ctr[dev], 64), "hipMalloc(step counter)");
-    *out = g_ctr[dev];
+    double a[5];
+    fd_coeffs(g->order, a);
+    double S = std::fabs(a[0]);
+    for (int k = 1; k <= g->order / 2; k++) S += 2.0 * std::fabs(a[k]);
+    double q = 0.0;
+    for (int d = 0; d < g->ndim; d++) q += S / (g->h[d] * g->h[d]);
+    return 4.0 / q * (1.0 + 1e-6);
+}
+
+int check_kconst(const wfd_grid* g, const wfd_medium* med)
+{
+    if (med->coef) return WFD_OK;
+    if (!(med->kconst > 0.0 && std::isfinite(med->kconst)))
+        return fail(WFD_ENONFINITE, "constant coefficient must be finite and > 0");
+    if (med->kconst > kc_limit(g))
+        return fail(WFD_EINVAL, "CFL violation: kc = %.6g exceeds the order-%d limit %.6g for this spacing",
+                    med->kconst, g->order, kc_limit(g));
     return WFD_OK;
 }
 
 }  // namespace
 
+// A forward run of `chunk` steps captured once into a hipGraph, with the step index of every
+// launch read from its own device counter (set before each run, advanced by the graph's last
+// node), so that one graph serves every chunk of a run and graphs on different streams never
+// share a counter.
+struct wfd_fgraph {
+    wfd_grid g;
+    wfd_medium med;
+    wfd_points src, rec;
+    bool has_src, has_rec;
+    void* u_a;
+    void* u_b;
+    int32_t chunk;
+    int64_t* ctr;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    hipEvent_t done;  // recorded after each run: destruction waits for it
+    bool ran;
+};
+
+namespace {
+
+void fgraph_free(wfd_fgraph* f)
+{
+    if (!f) return;
+    if (f->ran && f->done) (void)hipEventSynchronize(f->done);
+    if (f->exec) (void)hipGraphExecDestroy(f->exec);
+    if (f->graph) (void)hipGraphDestroy(f->graph);
+    if (f->done) (void)hipEventDestroy(f->done);
+    if (f->ctr) (void)hipFree(f->ctr);
+    delete f;
+}
+
+int fgraph_capture(wfd_fgraph* f)
+{
+    HIP_TRY(hipMalloc(&f->ctr, 64), "hipMalloc(step counter)");
+    HIP_TRY(hipEventCreateWithFlags(&f->done, hipEventDisableTiming), "hipEventCreate");
+    hipStream_t cap;
+    HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
+    const LaunchCfg cfg = default_cfg(&f->g);
+    int rc = WFD_OK;
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        void* p = f->u_a;
+        void* c = f->u_b;
+        for (int k = 0; k < f->chunk && rc == WFD_OK; k++) {
+            Launch L{&f->g, p, c, p, &f->med, f->has_src ? &f->src : nullptr, f->has_rec ? &f->rec : nullptr,
+                     k, f->ctr, nullptr, nullptr, false, cfg, -1};
+            rc = do_launch(L, cap);
+            std::swap(p, c);
+        }
+        if (rc == WFD_OK) {
+            e = launch_add_i64(f->ctr, f->chunk, cap);
+            if (e != hipSuccess) rc = hip_fail(e, "counter kernel");
+        }
+        hipError_t e2 = hipStreamEndCapture(cap, &f->graph);
+        if (rc == WFD_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
+    } else {
+        rc = hip_fail(e, "hipStreamBeginCapture");
+    }
+    if (rc == WFD_OK) {
+        e = hipGraphInstantiate(&f->exec, f->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+    }
+    (void)hipStreamDestroy(cap);
+    return rc;
+}
+
+int fgraph_make(wfd_fgraph** out, const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med,
+                const wfd_points* src, const wfd_points* rec, int32_t chunk)
+{
+    *out = nullptr;
+    int rc = check_step_args(g, u_a, u_b, u_a, med);
+    if (rc) return rc;
+    if (chunk < 2 || chunk % 2) return fail(WFD_EINVAL, "chunk must be even and >= 2 (got %d)", chunk);
+    wfd_fgraph* f = new wfd_fgraph{};
+    f->g = *g;
+    f->med = *med;
+    f->has_src = src && src->count > 0;
+    f->has_rec = rec && rec->count > 0;
+    if (f->has_src) f->src = *src;
+    if (f->has_rec) f->rec = *rec;
+    f->u_a = u_a;
+    f->u_b = u_b;
+    f->chunk = chunk;
+    rc = fgraph_capture(f);
+    if (rc) {
+        fgraph_free(f);
+        return rc;
+    }
+    *out = f;
+    return WFD_OK;
+}
+
+int fgraph_run(wfd_fgraph* f, int64_t step0, int64_t nsteps, hipStream_t st)
+{
+    if (nsteps < 0) return fail(WFD_EINVAL, "nsteps must be >= 0");
+    const wfd_points* src = f->has_src ? &f->src : nullptr;
+    const wfd_points* rec = f->has_rec ? &f->rec : nullptr;
+    const int64_t reps = nsteps / f->chunk;
+    if (reps > 0) {
+        // counter = step0 (a kernel, stream-ordered, so the graph may itself be captured), replays
+        hipError_t e = launch_set_i64(f->ctr, step0, st);
+        for (int64_t r = 0; e == hipSuccess && r < reps; r++) e = hipGraphLaunch(f->exec, st);
+        if (e == hipSuccess) e = hipEventRecord(f->done, st);
+        if (e != hipSuccess) return hip_fail(e, "graph replay");
+        f->ran = true;
+    }
+    const int64_t rem = nsteps - reps * f->chunk;
+    // after an even number of replayed steps the roles of u_a / u_b are unchanged; the
+    // remainder's first launch records the last replayed level, otherwise record it here
+    if (rem == 0 && reps > 0) return wfd::gather_launch(&f->g, f->u_b, rec, step0 + nsteps - 1, st);
+    int rc = wfd_forward(&f->g, f->u_a, f->u_b, &f->med, src, rec, step0 + reps * f->chunk, rem, st);
+    if (rc == WFD_OK && rem > 0) {
+        hipError_t e = hipEventRecord(f->done, st);
+        if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+        f->ran = true;
+    }
+    return rc;
+}
+
+// wfd_forward_graph keeps its graphs between calls, keyed by everything a capture bakes in
+// (stream, grid, buffers, medium, point descriptors, chunk, launch shape); each entry has its own
+// counter, so calls on different streams never share one.  At most kGraphCache entries; the
+// least recently used one is destroyed (after its last run has finished) to make room.
+struct CachedGraph {
+    hipStream_t stream;
+    std::string key;
+    wfd_fgraph* f;
+    uint64_t used;
+};
+constexpr size_t kGraphCache = 16;
+std::mutex g_graph_mu;
+std::vector<CachedGraph> g_graphs;
+uint64_t g_graph_tick = 0;
+
+template <typename T>
+void key_add(std::string& k, const T& v)
+{
+    k.append(reinterpret_cast<const char*>(&v), sizeof v);
+}
+
+std::string graph_key(const wfd_grid* g, const void* u_a, const void* u_b, const wfd_medium* med,
+                      const wfd_points* src, const wfd_points* rec, int32_t chunk)
+{
+    std::string k;
+    key_add(k, *g);
+    key_add(k, u_a);
+    key_add(k, u_b);
+    key_add(k, *med);
+    const wfd_points none{};
+    key_add(k, src && src->count ? *src : none);
+    key_add(k, rec && rec->count ? *rec : none);
+    key_add(k, chunk);
+    const LaunchCfg c = default_cfg(g);
+    key_add(k, c);
+    int dev = -1;
+    (void)hipGetDevice(&dev);
+    key_add(k, dev);
+    return k;
+}
+
+}  // namespace
+
 int wfd::set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
+int wfd::check_medium_args(const wfd_grid* g, const wfd_medium* med)
+{
+    if (!med) return fail(WFD_EINVAL, "medium is NULL");
+    if (med->coef && med->coef_shot_stride != 0 && med->coef_shot_stride != g->elems)
+        return fail(WFD_EINVAL, "coef_shot_stride must be 0 or elems");
+    return check_kconst(g, med);
+}
 
 int wfd::step_launch(const wfd_grid* g, const void* up, const void* uc, void* un, const wfd_medium* med,
                      const wfd_points* src, const wfd_points* rec, int64_t step, int64_t gather_off, const void* uf,
@@ -604,55 +787,48 @@ int wfd_forward_graph(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium*
     if (rc) return rc;
     if (nsteps < 0) return fail(WFD_EINVAL, "nsteps must be >= 0");
     if (chunk < 2 || chunk % 2) return fail(WFD_EINVAL, "chunk must be even and >= 2 (got %d)", chunk);
-    const int64_t reps = nsteps / chunk;
     hipStream_t st = (hipStream_t)stream;
-    if (reps > 0) {
-        int64_t* ctr = nullptr;
-        rc = device_counter(&ctr);
+    if (nsteps < chunk) return wfd_forward(g, u_a, u_b, med, src, rec, step0, nsteps, stream);
+    const std::string key = graph_key(g, u_a, u_b, med, src, rec, chunk);
+    std::lock_guard<std::mutex> lk(g_graph_mu);
+    wfd_fgraph* f = nullptr;
+    for (CachedGraph& c : g_graphs)
+        if (c.stream == st && c.key == key) {
+            c.used = ++g_graph_tick;
+            f = c.f;
+            break;
+        }
+    if (!f) {
+        if (g_graphs.size() >= kGraphCache) {
+            auto lru = std::min_element(g_graphs.begin(), g_graphs.end(),
+                                        [](const CachedGraph& a, const CachedGraph& b) { return a.used < b.used; });
+            fgraph_free(lru->f);
+            g_graphs.erase(lru);
+        }
+        rc = fgraph_make(&f, g, u_a, u_b, med, src, rec, chunk);
         if (rc) return rc;
-        const LaunchCfg cfg = default_cfg(g);
-        hipStream_t cap;
-        HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-        if (e == hipSuccess) {
-            void* p = u_a;
-            void* c = u_b;
-            for (int k = 0; k < chunk && rc == WFD_OK; k++) {
-                Launch L{g, p, c, p, med, src, rec, k, ctr, nullptr, nullptr, false, cfg, -1};
-                rc = do_launch(L, cap);
-                std::swap(p, c);
-            }
-            if (rc == WFD_OK) {
-                e = launch_add_i64(ctr, chunk, cap);
-                if (e != hipSuccess) rc = hip_fail(e, "counter kernel");
-            }
-            hipError_t e2 = hipStreamEndCapture(cap, &graph);
-            if (rc == WFD_OK && e2 != hipSuccess) rc = hip_fail(e2, "hipStreamEndCapture");
-        } else {
-            rc = hip_fail(e, "hipStreamBeginCapture");
-        }
-        if (rc == WFD_OK) {
-            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
-        }
-        if (rc == WFD_OK) {
-            // counter = step0 (a kernel, stream-ordered), then replay
-            e = launch_set_i64(ctr, step0, st);
-            for (int64_t r = 0; e == hipSuccess && r < reps; r++) e = hipGraphLaunch(exec, st);
-            if (e != hipSuccess) rc = hip_fail(e, "graph replay");
-        }
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        (void)hipStreamDestroy(cap);
-        if (rc) return rc;
+        g_graphs.push_back({st, key, f, ++g_graph_tick});
     }
-    const int64_t rem = nsteps - reps * chunk;
-    // after an even number of replayed steps the roles of u_a / u_b are unchanged; the
-    // remainder's first launch records the last replayed level, otherwise record it here
-    if (rem == 0 && reps > 0) return wfd::gather_launch(g, u_b, rec, step0 + nsteps - 1, st);
-    return wfd_forward(g, u_a, u_b, med, src, rec, step0 + reps * chunk, rem, stream);
+    return fgraph_run(f, step0, nsteps, st);
+}
+
+int wfd_fgraph_create(wfd_fgraph** fg, const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med,
+                      const wfd_points* src, const wfd_points* rec, int32_t chunk)
+{
+    if (!fg) return fail(WFD_EINVAL, "NULL graph handle");
+    return fgraph_make(fg, g, u_a, u_b, med, src, rec, chunk);
+}
+
+int wfd_fgraph_run(wfd_fgraph* fg, int64_t step0, int64_t nsteps, wfd_stream_t stream)
+{
+    if (!fg) return fail(WFD_EINVAL, "NULL graph");
+    return fgraph_run(fg, step0, nsteps, (hipStream_t)stream);
+}
+
+int wfd_fgraph_destroy(wfd_fgraph* fg)
+{
+    fgraph_free(fg);
+    return WFD_OK;
 }
 
 int wfd_adjoint_step(const wfd_grid* g, const void* mu_prev, const void* mu_cur, void* mu_next,
@@ -687,49 +863,69 @@ int wfd_check_medium_async(const wfd_grid* g, const wfd_medium* med, int32_t* fl
     int rc = check_grid(g);
     if (rc) return rc;
     if (!med || !flag) return fail(WFD_EINVAL, "NULL argument");
+    if (med->coef && med->coef_shot_stride != 0 && med->coef_shot_stride != g->elems)
+        return fail(WFD_EINVAL, "coef_shot_stride must be 0 or elems");
+    rc = check_kconst(g, med);
+    if (rc) return rc;
     int64_t np, nr, nc;
     extents(g, np, nr, nc);
     const int64_t nbeta = (g->ndim == 3 ? np + nr : g->ndim == 2 ? nr : 0) + nc;
     const int64_t ps = g->plane * g->ld;
+    const double kmax = kc_limit(g);
     hipError_t e = hipSuccess;
     const int nsets = med->coef && med->coef_shot_stride ? g->nshots : 1;
     for (int s = 0; s < nsets && e == hipSuccess; s++) {
         const int64_t so = (int64_t)s * med->coef_shot_stride + g->offset;
         if (g->dtype == WFD_F64)
             e = launch_check<double>(med->coef ? (const double*)med->coef + so : nullptr, np, nr, nc, g->ld, ps,
-                                     s == 0 ? (const double*)med->beta : nullptr, nbeta, flag, (hipStream_t)stream);
+                                     kmax, s == 0 ? (const double*)med->beta : nullptr, nbeta, flag,
+                                     (hipStream_t)stream);
         else
             e = launch_check<float>(med->coef ? (const float*)med->coef + so : nullptr, np, nr, nc, g->ld, ps,
-                                    s == 0 ? (const float*)med->beta : nullptr, nbeta, flag, (hipStream_t)stream);
+                                    (float)kmax, s == 0 ? (const float*)med->beta : nullptr, nbeta, flag,
+                                    (hipStream_t)stream);
     }
     if (e != hipSuccess) return hip_fail(e, "check kernel");
     return WFD_OK;
 }
+
+}  // extern "C"
+
+namespace {
+
+// The host-synchronising medium check on a caller-provided device flag (cleared here by a kernel).
+int check_medium_on(const wfd_grid* g, const wfd_medium* med, int32_t* flag, hipStream_t st)
+{
+    hipError_t e = launch_zero(flag, sizeof(int32_t), st);
+    if (e != hipSuccess) return hip_fail(e, "flag clear");
+    int rc = wfd_check_medium_async(g, med, flag, st);
+    if (rc) return rc;
+    int32_t host = 0;
+    HIP_TRY(hipMemcpyAsync(&host, flag, sizeof host, hipMemcpyDeviceToHost, st), "flag copy");
+    HIP_TRY(hipStreamSynchronize(st), "medium check");
+    if (host & 1) return fail(WFD_ENONFINITE, "medium holds a NaN/Inf, a non-positive coefficient or negative damping");
+    if (host & 2)
+        return fail(WFD_EINVAL, "CFL violation: a coefficient kc = v^2 dt^2 exceeds the order-%d limit %.6g for this spacing",
+                    g->order, kc_limit(g));
+    return WFD_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 int wfd_check_medium(const wfd_grid* g, const wfd_medium* med, wfd_stream_t stream)
 {
     int rc = check_grid(g);
     if (rc) return rc;
     if (!med) return fail(WFD_EINVAL, "NULL medium");
-    if (!med->coef && !(med->kconst > 0.0 && std::isfinite(med->kconst)))
-        return fail(WFD_ENONFINITE, "constant coefficient must be finite and > 0");
+    rc = check_kconst(g, med);
+    if (rc) return rc;
     int32_t* flag = nullptr;
     HIP_TRY(hipMalloc(&flag, sizeof(int32_t)), "hipMalloc(flag)");
-    hipStream_t st = (hipStream_t)stream;
-    int32_t host = 0;
-    hipError_t e = hipMemcpyAsync(flag, &host, sizeof host, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) {
-        rc = wfd_check_medium_async(g, med, flag, stream);
-        if (rc == WFD_OK) {
-            e = hipMemcpyAsync(&host, flag, sizeof host, hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
-        }
-    }
+    rc = check_medium_on(g, med, flag, (hipStream_t)stream);
     (void)hipFree(flag);
-    if (rc) return rc;
-    if (e != hipSuccess) return hip_fail(e, "medium check");
-    if (host) return fail(WFD_ENONFINITE, "medium holds a NaN/Inf, a non-positive coefficient or negative damping");
-    return WFD_OK;
+    return rc;
 }
 
 // ------------------------------------------------------------------------- launch tuning

@@ -828,6 +828,12 @@ int wfd_batch_create(wfd_batch** out, int32_t nshots, const wfd_batch_shot* shot
     for (int s = 0; s < nshots; s++) {
         int rc = check_shot(shots[s], s, dtype);
         if (rc) return rc;
+        // values of the velocity field and the damping (NaN/Inf, kc <= 0, CFL), host-synchronising
+        rc = wfd_check_medium(&shots[s].grid, &shots[s].med, nullptr);
+        if (rc) {
+            const std::string msg = wfd_last_error();
+            return bfail(rc, "wfd_batch_create: shot %d: %s", s, msg.c_str());
+        }
     }
     wfd_batch* b = new (std::nothrow) wfd_batch;
     if (!b) return bfail(WFD_ENOMEM, "wfd_batch_create: host allocation failed");

@@ -122,7 +122,7 @@ template <typename T>
 hipError_t launch_flush(T* g, double* G, int64_t elems, int nshots, hipStream_t s);
 
 template <typename T>
-hipError_t launch_check(const T* origin, int64_t np, int64_t nr, int64_t nc, int64_t ld, int64_t ps,
+hipError_t launch_check(const T* origin, int64_t np, int64_t nr, int64_t nc, int64_t ld, int64_t ps, T kmax,
                         const T* beta, int64_t nbeta, int32_t* flag, hipStream_t s);
 
 template <typename T>
@@ -147,6 +147,10 @@ hipError_t launch_zero(void* p, int64_t bytes, hipStream_t s);
 
 // record an error message for wfd_last_error and return `code`
 int set_error(int code, const char* msg);
+
+// Host checks of a medium descriptor for a stream-ordered call: non-NULL, coefficient stride 0 or
+// elems, a constant coefficient finite, > 0 and within the CFL limit of the grid.
+int check_medium_args(const wfd_grid* g, const wfd_medium* med);
 
 // One step launch (stencil tiles + gather blocks copying u_cur at `rec` into trace row
 // step + gather_off), and a gather-only launch of `field` into trace row `row`.  Used by the

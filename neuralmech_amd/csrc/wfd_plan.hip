@@ -162,6 +162,8 @@ int wfd_plan_forward(wfd_plan* p, const wfd_medium* med, const wfd_points* src, 
                      void* u_last, wfd_stream_t stream)
 {
     if (!p) return plan_fail(WFD_EINVAL, "wfd_plan_forward: NULL plan");
+    int rc = wfd::check_medium_args(&p->g, med);
+    if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     const int64_t N = p->nsteps, C = p->seg;
     hipError_t e = wfd::launch_zero(p->ck_prev[0], (int64_t)p->batch_bytes, st);
@@ -181,12 +183,12 @@ int wfd_plan_forward(wfd_plan* p, const wfd_medium* med, const wfd_points* src, 
         else
             D = Cu == p->work[0] ? p->work[1] : p->work[0];
         // the launch also records u^n (its current level) into trace row n-1
-        int rc = wfd::step_launch(&p->g, P, Cu, D, med, src, rec, n, -1, nullptr, nullptr, false, st);
+        rc = wfd::step_launch(&p->g, P, Cu, D, med, src, rec, n, -1, nullptr, nullptr, false, st);
         if (rc) return rc;
         P = Cu;
         Cu = D;
     }
-    int rc = wfd::gather_launch(&p->g, Cu, rec, N - 1, st);
+    rc = wfd::gather_launch(&p->g, Cu, rec, N - 1, st);
     if (rc) return rc;
     if (u_last) {
         e = hipMemcpyAsync(u_last, P, p->batch_bytes, hipMemcpyDeviceToDevice, st);
@@ -203,6 +205,8 @@ int wfd_plan_backward(wfd_plan* p, const wfd_medium* med, const wfd_points* src,
 {
     if (!p || !G) return plan_fail(WFD_EINVAL, "wfd_plan_backward: NULL argument");
     if (!p->forward_done) return plan_fail(WFD_EINVAL, "wfd_plan_backward: run wfd_plan_forward first");
+    int rc0 = wfd::check_medium_args(&p->g, med);
+    if (rc0) return rc0;
     hipStream_t st = (hipStream_t)stream;
     const int64_t N = p->nsteps, C = p->seg;
     const size_t gbytes = (size_t)p->g.elems * sizeof(double);

@@ -371,10 +371,11 @@ hipError_t launch_flush(T* g, double* G, int64_t elems, int nshots, hipStream_t 
 template hipError_t launch_flush<float>(float*, double*, int64_t, int, hipStream_t);
 template hipError_t launch_flush<double>(double*, double*, int64_t, int, hipStream_t);
 
-// flag = 1 if a medium coefficient at an interior node is NaN/Inf or not > 0
+// flag |= 1 if a medium coefficient at an interior node is NaN/Inf or not > 0, flag |= 2 if one
+// exceeds kmax (the CFL limit; kmax <= 0 checks nothing)
 template <typename T>
 __global__ void check_coef_kernel(const T* origin, int64_t np, int64_t nr, int64_t nc, int64_t ld, int64_t ps,
-                                  int32_t* flag)
+                                  T kmax, int32_t* flag)
 {
     const int64_t n = np * nr * nc;
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
@@ -383,29 +384,32 @@ __global__ void check_coef_kernel(const T* origin, int64_t np, int64_t nr, int64
                            (np == 1 || (k >= 1 && k <= np - 2));
         if (!inner) continue;
         const T y = origin[k * ps + i * ld + j];
-        if (!(y > T(0)) || y - y != T(0)) *flag = 1;
+        if (!(y > T(0)) || y - y != T(0))
+            atomicOr(flag, 1);
+        else if (kmax > T(0) && y > kmax)
+            atomicOr(flag, 2);
     }
 }
 
-// flag = 1 if any of the n damping entries is NaN/Inf or negative
+// flag |= 1 if any of the n damping entries is NaN/Inf or negative
 template <typename T>
 __global__ void check_beta_kernel(const T* v, int64_t n, int32_t* flag)
 {
     for (int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (int64_t)gridDim.x * blockDim.x) {
         const T y = v[x];
-        if (!(y >= T(0)) || y - y != T(0)) *flag = 1;
+        if (!(y >= T(0)) || y - y != T(0)) atomicOr(flag, 1);
     }
 }
 
 template <typename T>
-hipError_t launch_check(const T* origin, int64_t np, int64_t nr, int64_t nc, int64_t ld, int64_t ps,
+hipError_t launch_check(const T* origin, int64_t np, int64_t nr, int64_t nc, int64_t ld, int64_t ps, T kmax,
                         const T* beta, int64_t nbeta, int32_t* flag, hipStream_t s)
 {
     if (origin) {
         const int64_t n = np * nr * nc;
         const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
         hipLaunchKernelGGL((check_coef_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, s, origin, np, nr, nc, ld,
-                           ps, flag);
+                           ps, kmax, flag);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -416,9 +420,9 @@ hipError_t launch_check(const T* origin, int64_t np, int64_t nr, int64_t nc, int
     }
     return hipSuccess;
 }
-template hipError_t launch_check<float>(const float*, int64_t, int64_t, int64_t, int64_t, int64_t, const float*,
-                                        int64_t, int32_t*, hipStream_t);
-template hipError_t launch_check<double>(const double*, int64_t, int64_t, int64_t, int64_t, int64_t,
+template hipError_t launch_check<float>(const float*, int64_t, int64_t, int64_t, int64_t, int64_t, float,
+                                        const float*, int64_t, int32_t*, hipStream_t);
+template hipError_t launch_check<double>(const double*, int64_t, int64_t, int64_t, int64_t, int64_t, double,
                                          const double*, int64_t, int32_t*, hipStream_t);
 
 __global__ void add_i64_kernel(int64_t* p, int64_t v)

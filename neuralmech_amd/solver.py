@@ -226,10 +226,13 @@ class Medium:
     """``kc = v^2 dt^2`` per node (or a constant) and optional sponge profiles.
 
     ``velocity`` is a node array of the grid shape (shared by all shots) or ``(nshots, *shape)``;
-    ``beta_profiles`` one damping profile per axis (slowest first), ``beta = d dt / 2``.
+    ``beta_profiles`` one damping profile per axis (slowest first), ``beta = d dt / 2``.  On a GPU
+    the values are checked once here (``check``): NaN/Inf or ``kc <= 0`` raise ``WaveError``
+    (status WFD_ENONFINITE), a ``kc`` above the CFL limit of the grid's order and spacing raises
+    it with status WFD_EINVAL.
     """
 
-    def __init__(self, grid: WaveGrid, velocity=None, kc=None, kconst=None, beta_profiles=None):
+    def __init__(self, grid: WaveGrid, velocity=None, kc=None, kconst=None, beta_profiles=None, check=True):
         self.grid = grid
         self.coef = None
         self.kconst = 0.0
@@ -264,6 +267,8 @@ class Medium:
         self.c.kconst = self.kconst
         self.c.coef_shot_stride = self.coef_shot_stride
         self.c.beta = None if self.beta is None else self.beta.data_ptr()
+        if check and grid.device.type == "cuda":
+            self.validate()
 
     def kc_nodes(self) -> torch.Tensor:
         if self.coef is None:
@@ -271,7 +276,8 @@ class Medium:
         return self.grid.nodes(self.coef)
 
     def validate(self):
-        """Raise ``WaveError`` (status WFD_ENONFINITE) on NaN/Inf or non-positive coefficients."""
+        """Raise ``WaveError`` on NaN/Inf or non-positive coefficients (status WFD_ENONFINITE) or on a
+        coefficient above the CFL limit (WFD_EINVAL); host-synchronising."""
         check(_lib.load().wfd_check_medium(ctypes.byref(self.grid.c), ctypes.byref(self.c),
                                            _stream_ptr(self.grid.device)))
 
@@ -294,45 +300,177 @@ def _sig_tensor(grid: WaveGrid, pts: Optional[PointSet], sig, nsteps: int):
     return s.contiguous()
 
 
-def run(grid: WaveGrid, medium: Medium, nsteps: int, src: Optional[PointSet] = None, sig=None,
-        rec: Optional[PointSet] = None, state=None, step0: int = 0, graph_chunk: int = 0, two_step: int = 0):
-    """Advance ``nsteps`` steps; returns ``(traces, (u_prev, u_cur))``.
+class WaveState:
+    """The two newest time levels of a run and the step they stand at: ``prev = u^{n-1}``,
+    ``cur = u^n`` (layout tensors ``(nshots, elems)``), ``step = n``.
 
-    ``traces[s, n, r] = u^{step0+n+1}`` at receiver ``r`` of shot ``s``; ``state`` is
-    ``(u_prev, u_cur)`` layout tensors (rest state when omitted, updated in place otherwise).
-    ``graph_chunk`` > 0 replays a captured hipGraph of that many steps; ``two_step`` > 0 (2D)
-    runs two steps per launch with that many output rows per wave (2, 4 or 8: the intermediate
-    level in LDS; 16 or 32: in registers), using two scratch fields; the results are the same bits.
+    :func:`run` and :class:`ForwardRun` advance it in place -- the levels and the step -- so the
+    same object passed again continues the run whatever the parity of the step counts (the
+    library rotates two buffers in place, ``2D_wave_cuda_padding.py:83-85``, so after an odd
+    count the newest level sits in the other buffer).  It reads like the pair ``(prev, cur)``.
     """
-    lib = _lib.load()
+
+    __slots__ = ("prev", "cur", "step")
+
+    def __init__(self, prev: torch.Tensor, cur: torch.Tensor, step: int = 0):
+        if prev is cur or prev.data_ptr() == cur.data_ptr():
+            raise ValueError("prev and cur must be distinct fields")
+        self.prev, self.cur, self.step = prev, cur, int(step)
+
+    @classmethod
+    def rest(cls, grid: WaveGrid, step: int = 0) -> "WaveState":
+        """Zero fields (the medium at rest) standing at ``step``."""
+        return cls(grid.zeros(), grid.zeros(), step)
+
+    def clone(self) -> "WaveState":
+        return WaveState(self.prev.clone(), self.cur.clone(), self.step)
+
+    def __iter__(self):
+        return iter((self.prev, self.cur))
+
+    def __getitem__(self, i):
+        return (self.prev, self.cur)[i]
+
+    def __len__(self):
+        return 2
+
+
+def _as_state(grid: WaveGrid, state, step0) -> WaveState:
     if state is None:
-        ua, ub = grid.zeros(), grid.zeros()
-    else:
-        ua, ub = state
-    total = step0 + nsteps
-    s_sig = _sig_tensor(grid, src, sig, total)
-    traces = None
-    if rec is not None and rec.count:
-        traces = torch.zeros(grid.nshots, total, rec.count, dtype=grid.dtype, device=grid.device)
-    sp = src.struct(s_sig) if src is not None else None
-    rp = rec.struct(traces) if rec is not None else None
-    args = (ctypes.byref(grid.c), _ptr(ua), _ptr(ub), ctypes.byref(medium.c),
-            ctypes.byref(sp) if sp else None, ctypes.byref(rp) if rp else None, step0, nsteps)
-    if two_step:
-        wc, wd = grid.zeros(), grid.zeros()
-        check(lib.wfd_forward2(ctypes.byref(grid.c), _ptr(ua), _ptr(ub), _ptr(wc), _ptr(wd), ctypes.byref(medium.c),
-                               ctypes.byref(sp) if sp else None, ctypes.byref(rp) if rp else None, step0, nsteps,
-                               int(two_step), _stream_ptr(grid.device)))
-        out = None if traces is None else traces[:, step0:]
-        return out, (ua, ub)  # (u^{N-1}, u^N) in place
-    if graph_chunk:
-        check(lib.wfd_forward_graph(*args, int(graph_chunk), _stream_ptr(grid.device)))
-    else:
-        check(lib.wfd_forward(*args, _stream_ptr(grid.device)))
-    if nsteps % 2:
-        ua, ub = ub, ua
-    out = None if traces is None else traces[:, step0:]
-    return out, (ua, ub)
+        return WaveState.rest(grid, 0 if step0 is None else step0)
+    if not isinstance(state, WaveState):
+        raise TypeError("state must be a solver.WaveState: a plain (u_prev, u_cur) pair cannot record "
+                        "which buffer holds the newest level after an odd number of steps")
+    if step0 is not None and int(step0) != state.step:
+        raise ValueError(f"step0 = {step0} but the state stands at step {state.step}")
+    return state
+
+
+class ForwardRun:
+    """A forward run staged once: the signal, a trace buffer with a row for every step up to
+    ``total`` and the C descriptors are built at construction, so :meth:`advance` only launches
+    (``wfd_forward``; a replay of a ``wfd_fgraph`` captured once per buffer orientation with
+    ``graph_chunk``; ``wfd_forward2`` with ``two_step``).
+
+    ``traces[s, n, r] = u^{n+1}`` at receiver ``r`` of shot ``s`` for the absolute step ``n``;
+    ``state`` (a :class:`WaveState`, rest at step 0 when omitted) is advanced in place.
+    ``graph_chunk`` > 0 (even) replays a hipGraph of that many steps -- the reference's captured
+    20-step graph, ``2D_wave_cuda_graph.py:73-79,91-96`` -- and runs the remainder as plain
+    launches; ``two_step`` > 0 (2D) runs two steps per launch with that many output rows per wave
+    (2, 4 or 8: the intermediate level in LDS; 16 or 32: in registers) on two work fields.  Every
+    form gives the same bits.
+    """
+
+    def __init__(self, grid: WaveGrid, medium: Medium, total: int, src: Optional[PointSet] = None, sig=None,
+                 rec: Optional[PointSet] = None, state: Optional[WaveState] = None, graph_chunk: int = 0,
+                 two_step: int = 0):
+        self.grid, self.medium, self.total = grid, medium, int(total)
+        self.state = _as_state(grid, state, None)
+        if self.state.step > self.total:
+            raise ValueError(f"the state stands at step {self.state.step}, past total = {self.total}")
+        self.graph_chunk, self.two_step = int(graph_chunk), int(two_step)
+        if self.graph_chunk and (self.graph_chunk < 2 or self.graph_chunk % 2):
+            raise ValueError("graph_chunk must be even and >= 2")
+        self.src, self.rec = src, rec
+        self.sig = _sig_tensor(grid, src, sig, self.total)
+        self.traces = None
+        if rec is not None and rec.count:
+            self.traces = torch.zeros(grid.nshots, self.total, rec.count, dtype=grid.dtype, device=grid.device)
+        self._sp = src.struct(self.sig) if src is not None else None
+        self._rp = rec.struct(self.traces) if rec is not None else None
+        self._work = (grid.zeros(), grid.zeros()) if self.two_step else None
+        self._graphs = {}
+
+    def _pts(self):
+        return (ctypes.byref(self._sp) if self._sp else None, ctypes.byref(self._rp) if self._rp else None)
+
+    def prepare(self) -> None:
+        """Capture the graph for the current buffer orientation now (outside any timing)."""
+        if self.graph_chunk:
+            self._graph()
+
+    def _graph(self):
+        key = (self.state.prev.data_ptr(), self.state.cur.data_ptr())
+        gr = self._graphs.get(key)
+        if gr is None:
+            gr = ForwardGraph(self.grid, self.state.prev, self.state.cur, self.medium, self._sp, self._rp,
+                              self.graph_chunk)
+            self._graphs[key] = gr
+        return gr
+
+    def advance(self, nsteps: int) -> WaveState:
+        """Run ``nsteps`` more steps on the stream; the state moves to ``step + nsteps``."""
+        nsteps = int(nsteps)
+        st, g = self.state, self.grid
+        if nsteps < 0 or st.step + nsteps > self.total:
+            raise ValueError(f"{nsteps} steps from step {st.step} pass total = {self.total}")
+        if nsteps == 0:
+            return st
+        lib = _lib.load()
+        sp, rp = self._pts()
+        stream = _stream_ptr(g.device)
+        if self.two_step:
+            wc, wd = self._work
+            check(lib.wfd_forward2(ctypes.byref(g.c), _ptr(st.prev), _ptr(st.cur), _ptr(wc), _ptr(wd),
+                                   ctypes.byref(self.medium.c), sp, rp, st.step, nsteps, self.two_step, stream))
+            st.step += nsteps  # (u^{N-1}, u^N) stay in (prev, cur)
+            return st
+        if self.graph_chunk:
+            self._graph().run(st.step, nsteps)
+        else:
+            check(lib.wfd_forward(ctypes.byref(g.c), _ptr(st.prev), _ptr(st.cur), ctypes.byref(self.medium.c),
+                                  sp, rp, st.step, nsteps, stream))
+        if nsteps % 2:  # the newest level is now in the buffer that held u^{n-1}
+            st.prev, st.cur = st.cur, st.prev
+        st.step += nsteps
+        return st
+
+
+class ForwardGraph:
+    """``wfd_fgraph``: ``chunk`` steps of the forward run on the fields ``(u_a, u_b)`` captured once
+    into a hipGraph with its own device step counter (the reference's capture-once, replay-many
+    graph, ``2D_wave_cuda_graph.py:73-79,91-96``).  The buffers, medium and point descriptors it
+    names must stay alive while it exists."""
+
+    def __init__(self, grid: WaveGrid, u_a, u_b, medium: Medium, src_struct, rec_struct, chunk: int):
+        self.grid = grid
+        self._keep = (u_a, u_b, medium, src_struct, rec_struct)
+        self._p = ctypes.c_void_p()
+        check(_lib.load().wfd_fgraph_create(ctypes.byref(self._p), ctypes.byref(grid.c), _ptr(u_a), _ptr(u_b),
+                                            ctypes.byref(medium.c),
+                                            ctypes.byref(src_struct) if src_struct else None,
+                                            ctypes.byref(rec_struct) if rec_struct else None, int(chunk)))
+
+    def run(self, step0: int, nsteps: int) -> None:
+        check(_lib.load().wfd_fgraph_run(self._p, int(step0), int(nsteps), _stream_ptr(self.grid.device)))
+
+    def __del__(self):
+        p = getattr(self, "_p", None)
+        if p is not None and p.value:
+            try:
+                _lib.load().wfd_fgraph_destroy(p)
+            except Exception:
+                pass
+            self._p = None
+
+
+def run(grid: WaveGrid, medium: Medium, nsteps: int, src: Optional[PointSet] = None, sig=None,
+        rec: Optional[PointSet] = None, state: Optional[WaveState] = None, step0: Optional[int] = None,
+        graph_chunk: int = 0, two_step: int = 0):
+    """Advance ``nsteps`` steps; returns ``(traces, state)``.
+
+    ``traces[s, n, r] = u^{step0+n+1}`` at receiver ``r`` of shot ``s``.  ``state`` is a
+    :class:`WaveState`: omitted, the run starts at rest at ``step0`` (default 0); given, it
+    continues from ``state.step`` and is advanced in place (``step0``, if passed, must equal it),
+    so ``run(N1)`` followed by ``run(N2, state=...)`` gives the bits of ``run(N1 + N2)``.
+    ``graph_chunk`` and ``two_step`` select the launch form as in :class:`ForwardRun`.
+    """
+    st = _as_state(grid, state, step0)
+    s0 = st.step
+    fr = ForwardRun(grid, medium, s0 + int(nsteps), src, sig, rec, st, graph_chunk, two_step)
+    fr.advance(nsteps)
+    out = None if fr.traces is None else fr.traces[:, s0:]
+    return out, st
 
 
 def kc_gradient(grid: WaveGrid, Gn: torch.Tensor, mu_src, sig, src: Optional[PointSet], kc) -> torch.Tensor:

@@ -63,23 +63,18 @@ def simulate(sim, source=None, indicator=None, sensors=None, record_every: Optio
         return simulate_acoustic(sim, source, indicator, sensors, record_every)
     sources = [source] if source is not None else []
     g, med, src, sig, rec = _setup(sim, sources, sensors, 1)
-    state = (g.zeros(), g.zeros())
+    fr = solver.ForwardRun(g, med, sim.N, src=src, sig=sig, rec=rec)
     frames = []
-    traces = []
-    step = 0
     chunk = int(record_every) if record_every else sim.N
-    while step < sim.N:
-        n = min(chunk, sim.N - step)
-        tr, state = solver.run(g, med, n, src=src, sig=sig, rec=rec, state=state, step0=step)
-        if tr is not None:
-            traces.append(tr[0])
-        step += n
+    while fr.state.step < sim.N:
+        fr.advance(min(chunk, sim.N - fr.state.step))
         if record_every:
-            frames.append(g.nodes(state[1])[0].detach().cpu().numpy().copy())
-    u = g.nodes(state[1])[0].clone()
+            frames.append(g.nodes(fr.state.cur)[0].detach().cpu().numpy().copy())
+    u = g.nodes(fr.state.cur)[0].clone()
     out = [u]
     if sensors is not None:
-        out.append(torch.cat(traces, 0) if traces else torch.zeros(sim.N, 0, dtype=sim.dtype, device=sim.device))
+        out.append(fr.traces[0] if fr.traces is not None
+                   else torch.zeros(sim.N, 0, dtype=sim.dtype, device=sim.device))
     if record_every:
         out.append(frames)
     return out[0] if len(out) == 1 else tuple(out)

@@ -269,3 +269,62 @@ def test_acoustic_rows_override_validation():
         assert lib.wfd_ac_set_rows(bad) == _lib.WFD_EINVAL
     for ok in (1, 2, 4, 8, 0):
         assert lib.wfd_ac_set_rows(ok) == _lib.WFD_OK
+
+
+@pytest.mark.parametrize("ndim,order", [(1, 2), (2, 2), (2, 8), (3, 4), (3, 8)])
+def test_cfl_limit_of_constant_coefficient_needs_no_gpu(ndim, order):
+    """A constant kc above 4 / sum_d(S / h_d^2) (the bound wfd_stable_dt states) is rejected with
+    WFD_EINVAL by every stream-ordered entry point before any launch; at the limit it passes the
+    host check.  The limit is the closed form of the stencil symbol, checked here independently
+    from the exact rational weights."""
+    from fractions import Fraction
+    from math import factorial
+
+    lib = _lib.load()
+    h = [0.7, 1.3, 2.1][:ndim]
+    rc, g = _grid([12] * ndim, h, order)
+    assert rc == 0
+    K = order // 2
+    a = [Fraction(2 * (-1) ** (k + 1) * factorial(K) ** 2, k * k * factorial(K - k) * factorial(K + k))
+         for k in range(1, K + 1)]
+    S = 2 * abs(sum(a)) + 2 * sum(abs(x) for x in a)  # |a_0| + 2 sum |a_k|, a_0 = -2 sum a_k
+    limit = 4.0 / sum(float(S) / (x * x) for x in h)
+    # the same limit as wfd_stable_dt: kc = (vmax dt)^2
+    hh = (ctypes.c_double * 3)(*h, *([1.0] * (3 - ndim)))
+    assert abs(lib.wfd_stable_dt(ndim, hh, 1.0, order) ** 2 / limit - 1) < 1e-12
+    buf = (ctypes.c_float * 256)()
+    p = (ctypes.addressof(buf) + 15) // 16 * 16
+    med = _lib.Medium()
+    flag = ctypes.c_int32()
+    for kc, want in ((limit * 1.001, _lib.WFD_EINVAL), (limit * 1.0000001, None)):
+        med.kconst = kc
+        rc = lib.wfd_check_medium_async(ctypes.byref(g), ctypes.byref(med), ctypes.byref(flag), None)
+        if want is None:
+            continue  # inside the slack: the host check passes (the call goes on to the device)
+        assert rc == want and "CFL" in _lib.last_error()
+        assert lib.wfd_step(ctypes.byref(g), p, p + 256, p, ctypes.byref(med), None, None, 0, None) == want
+        assert lib.wfd_forward(ctypes.byref(g), p, p + 256, ctypes.byref(med), None, None, 0, 4, None) == want
+        assert lib.wfd_forward_graph(ctypes.byref(g), p, p + 256, ctypes.byref(med), None, None, 0, 40, 20,
+                                     None) == want
+        fg = ctypes.c_void_p()
+        assert lib.wfd_fgraph_create(ctypes.byref(fg), ctypes.byref(g), p, p + 256, ctypes.byref(med), None, None,
+                                     20) == want
+        assert not fg.value
+        assert lib.wfd_check_medium(ctypes.byref(g), ctypes.byref(med), None) == want
+
+
+def test_graph_object_argument_validation_needs_no_gpu():
+    lib = _lib.load()
+    rc, g = _grid((20, 30), (1.0, 1.0))
+    med = _lib.Medium()
+    med.kconst = 0.1
+    buf = (ctypes.c_float * 128)()
+    p = (ctypes.addressof(buf) + 15) // 16 * 16
+    fg = ctypes.c_void_p()
+    for chunk in (0, 1, 3, 21):
+        assert lib.wfd_fgraph_create(ctypes.byref(fg), ctypes.byref(g), p, p + 256, ctypes.byref(med), None, None,
+                                     chunk) == _lib.WFD_EINVAL
+        assert "chunk" in _lib.last_error() and not fg.value
+    assert lib.wfd_fgraph_create(None, ctypes.byref(g), p, p + 256, ctypes.byref(med), None, None, 20) == _lib.WFD_EINVAL
+    assert lib.wfd_fgraph_run(None, 0, 10, None) == _lib.WFD_EINVAL
+    assert lib.wfd_fgraph_destroy(None) == _lib.WFD_OK

@@ -1,0 +1,184 @@
+"""GPU tests of resuming a run and of graph replay: run(N1) then run(N2) from the returned state
+equals run(N1 + N2) bit for bit (plain launches, captured graphs, two steps per launch; odd and
+even N1), graphs are captured once and replayed, concurrent graph replays on two streams give the
+single-stream bits, and media are checked for NaN / non-positive values and the CFL limit."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.wave_cases import gpu_objects, make_case, oracle_forward
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _case():
+    return make_case((61, 83), 8, seed=21, nsteps=50)
+
+
+def _gpu(case):
+    g, med, src, rec = gpu_objects(case, torch.float32)
+    return g, med, src, rec, torch.as_tensor(case["sig"])
+
+
+@pytest.mark.parametrize("form", ["plain", "graph", "two_step"])
+@pytest.mark.parametrize("n1", [5, 6, 21, 20])
+def test_resume_equals_uninterrupted_run(form, n1):
+    from neuralmech_amd import solver
+
+    print(f"\n[resume] form={form} n1={n1}", flush=True)
+    case = _case()
+    kw = dict(graph_chunk=4 if form == "graph" else 0, two_step=8 if form == "two_step" else 0)
+    g, med, src, rec, sig = _gpu(case)
+    tr_all, st_all = solver.run(g, med, 50, src=src, sig=sig, rec=rec, **kw)
+    tr1, st = solver.run(g, med, n1, src=src, sig=sig, rec=rec, **kw)
+    assert st.step == n1
+    tr2, st2 = solver.run(g, med, 50 - n1, src=src, sig=sig, rec=rec, state=st, **kw)
+    torch.cuda.synchronize()
+    assert st2 is st and st.step == 50
+    assert torch.equal(torch.cat([tr1, tr2], 1), tr_all)
+    assert torch.equal(st.cur, st_all.cur) and torch.equal(st.prev, st_all.prev)
+    tr_o, up_o, uc_o = oracle_forward(case, np.float32)
+    assert np.array_equal(tr_all[0].cpu().numpy(), tr_o)
+    assert np.array_equal(g.nodes(st.cur)[0].cpu().numpy(), uc_o)
+    assert np.array_equal(g.nodes(st.prev)[0].cpu().numpy(), up_o)
+
+
+def test_staged_run_in_chunks_of_every_parity():
+    """One ForwardRun advanced by 7, 4, 1, 20, 18 steps (with a graph of 6) equals the oracle."""
+    from neuralmech_amd import solver
+
+    case = _case()
+    g, med, src, rec, sig = _gpu(case)
+    fr = solver.ForwardRun(g, med, 50, src=src, sig=sig, rec=rec, graph_chunk=6)
+    for n in (7, 4, 1, 20, 18):
+        fr.advance(n)
+    torch.cuda.synchronize()
+    assert fr.state.step == 50 and len(fr._graphs) == 2  # one capture per buffer orientation
+    tr_o, up_o, uc_o = oracle_forward(case, np.float32)
+    assert np.array_equal(fr.traces[0].cpu().numpy(), tr_o)
+    assert np.array_equal(g.nodes(fr.state.cur)[0].cpu().numpy(), uc_o)
+    with pytest.raises(ValueError):
+        fr.advance(1)  # past the staged total
+
+
+def test_acoustic_forward_resume():
+    from neuralmech_amd import acoustic as ac
+    from neuralmech_amd import solver
+
+    case = make_case((48, 57), 4, seed=4, nsteps=30, sponge=False)
+    grid = solver.WaveGrid(case["shape"], case["h"], case["dt"], 4, torch.float32)
+    rng = np.random.default_rng(1)
+    m = torch.as_tensor(rng.uniform(1.0, 2.0, case["shape"]))
+    s = torch.as_tensor(rng.uniform(1.0, 2.0, case["shape"]))
+    med = ac.AcMedium(grid, m, s, "neumann", None)
+    src = solver.PointSet(grid, case["src"])
+    rec = solver.PointSet(grid, case["rec"])
+    sig = torch.as_tensor(case["sig"])[None]
+    tr_all, _, st_all = ac.forward(grid, med, 30, src, sig, rec)
+    tr1, _, st = ac.forward(grid, med, 11, src, sig, rec)
+    tr2, _, st = ac.forward(grid, med, 19, src, sig, rec, state=st)
+    torch.cuda.synchronize()
+    assert st.step == 30
+    assert torch.equal(torch.cat([tr1, tr2], 1), tr_all) and torch.equal(st.cur, st_all.cur)
+
+
+def _structs(g, med, src, rec, sig, traces):
+    sp = src.struct(sig)
+    rp = rec.struct(traces)
+    return sp, rp
+
+
+@pytest.mark.parametrize("api", ["object", "cached"])
+def test_concurrent_graph_replays_on_two_streams(api):
+    """Two shots replayed from graphs on two streams at once (their launches interleave on the
+    device) give the bits of the same shots run one after the other on one stream; with the
+    cached wfd_forward_graph the second call on a stream replays the graph captured by the first."""
+    from neuralmech_amd import _lib, solver
+
+    print(f"\n[two streams] api={api}", flush=True)
+    lib = _lib.load()
+    cases = [make_case((200, 230), 8, seed=s, nsteps=200) for s in (31, 32)]
+    objs = [_gpu(c) for c in cases]
+
+    def staged(o):
+        g, med, src, rec, sig = o
+        st = solver.WaveState.rest(g)
+        s = solver._sig_tensor(g, src, sig, 200)
+        tr = torch.zeros(1, 200, rec.count, device="cuda")
+        return st, s, tr, src.struct(s), rec.struct(tr)
+
+    def launch(o, stg, nsteps, step0, stream):
+        g, med, *_ = o
+        st, _, _, sp, rp = stg
+        h = ctypes.c_void_p(stream.cuda_stream)
+        if api == "cached":
+            _lib.check(lib.wfd_forward_graph(ctypes.byref(g.c), st.prev.data_ptr(), st.cur.data_ptr(),
+                                             ctypes.byref(med.c), ctypes.byref(sp), ctypes.byref(rp), step0, nsteps,
+                                             20, h))
+        else:
+            fg = stg_graph[id(stg)]
+            _lib.check(lib.wfd_fgraph_run(fg, step0, nsteps, h))
+
+    stg_graph = {}
+    runs = []
+    for _ in range(2):  # reference: one stream, one after the other
+        runs.append([staged(o) for o in objs])
+    for stg_set in runs:
+        for o, stg in zip(objs, stg_set):
+            if api == "object":
+                fg = ctypes.c_void_p()
+                st, _, _, sp, rp = stg
+                _lib.check(lib.wfd_fgraph_create(ctypes.byref(fg), ctypes.byref(o[0].c), st.prev.data_ptr(),
+                                                 st.cur.data_ptr(), ctypes.byref(o[1].c), ctypes.byref(sp),
+                                                 ctypes.byref(rp), 20))
+                stg_graph[id(stg)] = fg
+    s0 = torch.cuda.current_stream()
+    for o, stg in zip(objs, runs[0]):
+        launch(o, stg, 100, 0, s0)
+        launch(o, stg, 100, 100, s0)  # second call: same graph, later steps
+    torch.cuda.synchronize()
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    for k in range(2):  # interleaved enqueues on two streams
+        launch(objs[0], runs[1][0], 100, 100 * k, sa)
+        launch(objs[1], runs[1][1], 100, 100 * k, sb)
+    torch.cuda.synchronize()
+    for i in range(2):
+        ref, got = runs[0][i], runs[1][i]
+        assert torch.equal(ref[2], got[2]), f"shot {i}: traces differ between one stream and two"
+        assert torch.equal(ref[0].cur, got[0].cur) and torch.equal(ref[0].prev, got[0].prev)
+        tr_o, _, uc_o = oracle_forward(cases[i], np.float32)
+        assert np.array_equal(ref[2][0].cpu().numpy(), tr_o)
+    for fg in stg_graph.values():
+        lib.wfd_fgraph_destroy(fg)
+
+
+def test_medium_check_rejects_unstable_and_nonfinite_fields():
+    from neuralmech_amd import _lib, solver
+
+    case = make_case((40, 50), 8, seed=2, nsteps=10, safety=0.6)
+    g = solver.WaveGrid(case["shape"], case["h"], case["dt"], 8, torch.float32)
+    kc = torch.as_tensor(case["kc"])
+    solver.Medium(g, kc=kc)  # 0.6 of the stable step: accepted
+    limit = 4.0 / sum((2048 / 315) / (h * h) for h in case["h"])  # S = |a_0| + 2 sum |a_k| = 2048/315
+    bad = kc.clone()
+    bad[20, 25] = limit * 1.01
+    with pytest.raises(_lib.WaveError) as e:
+        solver.Medium(g, kc=bad)
+    assert e.value.code == _lib.WFD_EINVAL and "CFL" in str(e.value)
+    bad[20, 25] = float("nan")
+    with pytest.raises(_lib.WaveError) as e:
+        solver.Medium(g, kc=bad)
+    assert e.value.code == _lib.WFD_ENONFINITE
+    bad = kc.clone()
+    bad[3, 4] = -1.0
+    with pytest.raises(_lib.WaveError) as e:
+        solver.Medium(g, kc=bad)
+    assert e.value.code == _lib.WFD_ENONFINITE
+    # a value above the limit on the boundary ring is never used and passes
+    ring = kc.clone()
+    ring[0, 10] = limit * 10
+    solver.Medium(g, kc=ring)
