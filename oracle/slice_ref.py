@@ -279,8 +279,11 @@ def scaled_step_reference(q0, s0, q1, s1, q2, s2, dt, dx, dy, wavespeed, N, M):
     """``scaled_step`` (``step2D_wave.cu:128-183``): decode, fp32 slice-form update on the interior
     (boundary columns and padding of interior rows encode zero), re-encode each 32-node block
     with its largest magnitude m: ``q = rint(u * (32767/m))``, ``s = m / 32767``, with ``q = 0``
-    where ``32767/m`` is not finite (the reference divides by zero there).  Rows 0 and N-1 are
-    left untouched."""
+    where ``32767/m`` is not finite.  The reference guards ``m > 0`` (``:179``), so an all-zero
+    block encodes as zero there too; it differs for ``0 < m < 32767 / FLT_MAX`` (about 9.6e-35),
+    where its ``32767/m`` overflows to inf and the saturated integer conversion stores the low
+    16 bits of INT_MAX (q = -1) -- here such a block stores q = 0, the value nearest to what it
+    holds.  Rows 0 and N-1 are left untouched."""
     u0 = scaled_decode(q0, s0)
     u1 = scaled_decode(q1, s1)
     u2 = np.zeros_like(u1)
@@ -307,7 +310,8 @@ def scaled_encode(u, M):
 
 
 def _block_inverse(blk):
-    """``32767 / m`` in fp32, zero where it is not finite (all-zero or near-denormal blocks)."""
+    """``32767 / m`` in fp32, zero where it is not finite (an all-zero block, or a block maximum
+    below 32767 / FLT_MAX, where the quotient overflows)."""
     with np.errstate(divide="ignore", over="ignore"):
         inv = (np.float32(32767.0) / blk.astype(np.float32)).astype(np.float32)
     inv[~np.isfinite(inv)] = 0

@@ -188,3 +188,50 @@ def test_stream_bench_kernels_compute_what_they_state():
     assert lib.wfd_stream_bench(a.data_ptr(), 0, 0, o.data_ptr(), n - 2, 0, st) == _lib.WFD_EINVAL
     assert lib.wfd_stream_bench(a.data_ptr() + 4, 0, 0, o.data_ptr(), 8, 0, st) == _lib.WFD_EINVAL
     assert lib.wfd_stream_bench(a.data_ptr(), 0, 0, o.data_ptr(), n, 4, st) == _lib.WFD_EINVAL
+
+
+def test_storage_kernels_satisfy_the_format_pins():
+    """One GPU step of the fp16 and int16 drop-ins checked against the formats' defining
+    properties (tests/test_storage_pins.py): nearest binary16 of the fp32 update; |q| = 32767 at
+    each block maximum, half-quantum decoding, nearest-integer q.  In rows 3..9 the int16 input is
+    u1 = c, u0 = 2c (c = 1e-33, exactly cancelling in -u0 + 2 u1), so rows 4..8 update to
+    coef * (-c / dy^2) = -7.4e-35 next to the boundary columns and to 0 inside: blocks whose maximum
+    lies below 32767 / FLT_MAX, which encode as q = 0."""
+    from neuralmech_amd import reference_steps as rs
+    from tests.test_storage_pins import _int16_pins, _is_nearest_f16
+
+    NX, NY, Mp = 40, 120, 128
+    dt, dx, dy = 0.3, 1.0, 1.1
+    rng = np.random.default_rng(9)
+    u1 = np.zeros((NX, Mp), np.float32)
+    u1[18:-1, 1:NY - 1] = rng.standard_normal((NX - 19, NY - 2))
+    u0 = 0.9 * u1
+    # fp16
+    h0, h1 = u0.astype(np.float16), u1.astype(np.float16)
+    g0, g1 = torch.as_tensor(h0).cuda(), torch.as_tensor(h1).cuda()
+    rs.mixed_step(g0, g1, g0, 1.0, dt, dx, dy, NX, NY, Mp)
+    a0, a1 = h0[:, :NY].astype(np.float32), h1[:, :NY].astype(np.float32)
+    x = a0.copy()
+    slice_ref.fd_step_reference(a0, a1, x, dt, dx, dy, 1.0)
+    out = g0.cpu().numpy()
+    assert _is_nearest_f16(out[1:NX - 1, 1:NY - 1], x[1:-1, 1:-1]).all()
+    # int16
+    c = np.float32(1e-33)
+    u1[3:10, 1:NY - 1] = c
+    u0[3:10, 1:NY - 1] = 2 * c
+    q0, s0 = slice_ref.scaled_encode(u0, Mp)
+    q1, s1 = slice_ref.scaled_encode(u1, Mp)
+    gq0, gs0 = torch.as_tensor(q0).cuda(), torch.as_tensor(s0).cuda()
+    gq1, gs1 = torch.as_tensor(q1).cuda(), torch.as_tensor(s1).cuda()
+    rs.scaled_step(gq0, gs0, gq1, gs1, gq0, gs0, 1.0, dt, dx, dy, NX, NY, Mp)
+    torch.cuda.synchronize()
+    q2, s2 = gq0.cpu().numpy(), gs0.cpu().numpy()
+    d0, d1 = slice_ref.scaled_decode(q0, s0), slice_ref.scaled_decode(q1, s1)
+    x = np.zeros_like(d1)
+    slice_ref.fd_step_reference(d0[:, :NY], d1[:, :NY], x[:, :NY], dt, dx, dy, 1.0)
+    x[:, 0] = 0
+    assert all(_int16_pins(q2[18:-1], s2[18:-1], x[18:-1]))
+    blk = np.abs(x[4:9]).reshape(5, 4, 32).max(axis=2)
+    limit = 32767 / np.finfo(np.float32).max
+    assert np.all((blk[:, [0, 3]] > 0) & (blk[:, [0, 3]] < limit)) and np.all(blk[:, [1, 2]] == 0)
+    assert np.all(q2[4:9] == 0) and np.all(np.isfinite(s2[4:9]))
