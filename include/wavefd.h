@@ -405,6 +405,22 @@ int wfd_ac_backward(const wfd_grid* g, void* u_a, void* u_b, void* mu_a, void* m
                     const wfd_points* strip, const wfd_points* mu_src, int64_t nsteps,
                     void* g_work, double* G, wfd_stream_t stream);
 
+/* Allocation-free variant of wfd_ac_backward: the work fields (mu_a, mu_b and g_work) live in
+ * a file-scope device array of WFD_AC_SCRATCH_BYTES that each device holds from the library's
+ * first use on it until the process ends, so the call makes no hipMalloc / hipFree and no
+ * allocating library call; the caller passes only the inputs (u_a, u_b as the forward left them,
+ * consumed; medium; point lists) and the outputs (G, mu_src data), with the same results, bit for
+ * bit, as wfd_ac_backward.  Its work fields need (2 + 2K+1) * nshots * elems * itemsize bytes
+ * (each field rounded up to 256); larger problems return WFD_EINVAL.  Every call on one device
+ * shares that device's array, whose contents persist between calls (the kernels clear what must
+ * start at zero): calls that overlap in time on one device race, so run them on one stream or
+ * serialise them.  No counterpart in the reference: it exists for callers that cannot allocate
+ * device memory -- code under hipGraph capture, tight serving loops, harnesses that forbid it. */
+#define WFD_AC_SCRATCH_BYTES (64ull << 20)
+int wfd_ac_backward_noalloc(const wfd_grid* g, void* u_a, void* u_b, const wfd_ac_medium* med,
+                            const wfd_points* src, const wfd_points* adj, const wfd_points* strip,
+                            const wfd_points* mu_src, int64_t nsteps, double* G, wfd_stream_t stream);
+
 /* Rows per wave of the wfd_ac_* kernels: 0 (default) chooses per launch (the forward halves 8
  * rows until about 16 K waves are in flight; the adjoint aims at 32 K waves while its 8 + 4K
  * fields fit in 192 MiB and at 8 K otherwise); 1, 2, 4 or 8 forces that count for every later

@@ -15,6 +15,7 @@ LIB_PATH = _PKG / "libwavefd.so"
 
 WFD_OK, WFD_EINVAL, WFD_EHIP, WFD_ENOMEM, WFD_ENONFINITE = 0, 1, 2, 3, 4
 WFD_F32, WFD_F64 = 0, 1
+WFD_AC_SCRATCH_BYTES = 64 << 20  # include/wavefd.h
 
 
 class WaveError(RuntimeError):
@@ -153,6 +154,8 @@ SIGNATURES = {
     "wfd_ac_forward": (ctypes.c_int, [_GP, _P, _P, ctypes.POINTER(AcMedium), _PP, _PP, _PP, _I64, _I64, _P]),
     "wfd_ac_backward": (ctypes.c_int, [_GP, _P, _P, _P, _P, ctypes.POINTER(AcMedium), _PP, _PP, _PP, _PP, _I64,
                                        _P, _P, _P]),
+    "wfd_ac_backward_noalloc": (ctypes.c_int, [_GP, _P, _P, ctypes.POINTER(AcMedium), _PP, _PP, _PP, _PP, _I64,
+                                               _P, _P]),
     "wfd_ac_set_rows": (ctypes.c_int, [ctypes.c_int32]),
     "wfd_ref_step2d": (ctypes.c_int, [_P, _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,

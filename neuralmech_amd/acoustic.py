@@ -341,13 +341,15 @@ def forward(grid: WaveGrid, med: AcMedium, nsteps: int, src: Optional[PointSet] 
 
 
 def backward(grid: WaveGrid, med: AcMedium, nsteps: int, state, src: Optional[PointSet], sig, rec: PointSet,
-             dJdtr, strip: Optional[PointSet] = None, store=None):
+             dJdtr, strip: Optional[PointSet] = None, store=None, noalloc: bool = False):
     """Adjoint sweep with reconstruction for ``J(traces)`` with ``dJ/dtraces = dJdtr``
     ``(nshots, nsteps, R)``; ``state = (p^{N-1}, p^N)`` from :func:`forward` (consumed).
 
     Returns ``(G, mu_src)``: ``G`` float64 ``(2K+1, nshots, n0+K-1, n1+K-1)`` (mass term on the nodes,
     then the north and the west bond sums per distance; see ``wfd_ac_backward``) and ``mu_src``
-    ``(nshots, nsteps, S)`` or None.
+    ``(nshots, nsteps, S)`` or None.  ``noalloc`` runs ``wfd_ac_backward_noalloc``: the library's
+    work fields come from its per-device scratch array (at most ``WFD_AC_SCRATCH_BYTES``), the same
+    bits; calls on one device must not overlap.
     """
     lib = _lib.load()
     g = grid
@@ -366,15 +368,20 @@ def backward(grid: WaveGrid, med: AcMedium, nsteps: int, state, src: Optional[Po
     if strip is not None and strip.count and store is None:
         raise ValueError("the strip needs the values the forward recorded")
     tp = strip.struct(store.contiguous()) if strip is not None and strip.count else None
-    mu_a, mu_b = g.zeros(), g.zeros()
     nf = 2 * med.K + 1
-    work = torch.zeros(nf * g.nshots * g.elems, dtype=g.dtype, device=g.device)
     G = torch.zeros(nf, g.nshots, g.elems, dtype=torch.float64, device=g.device)
     ua, ub = state
-    check(lib.wfd_ac_backward(ctypes.byref(g.c), _ptr(ua), _ptr(ub), _ptr(mu_a), _ptr(mu_b), ctypes.byref(med.c),
-                              ctypes.byref(sp) if sp else None, ctypes.byref(ap) if ap else None,
-                              ctypes.byref(tp) if tp else None, ctypes.byref(mp) if mp else None, int(nsteps),
-                              _ptr(work), _ptr(G), _stream_ptr(g.device)))
+    pts = (ctypes.byref(sp) if sp else None, ctypes.byref(ap) if ap else None,
+           ctypes.byref(tp) if tp else None, ctypes.byref(mp) if mp else None)
+    if noalloc:  # work fields in the library's per-device scratch array
+        check(lib.wfd_ac_backward_noalloc(ctypes.byref(g.c), _ptr(ua), _ptr(ub), ctypes.byref(med.c), *pts,
+                                          int(nsteps), _ptr(G), _stream_ptr(g.device)))
+    else:
+        mu_a, mu_b = g.zeros(), g.zeros()
+        work = torch.zeros(nf * g.nshots * g.elems, dtype=g.dtype, device=g.device)
+        check(lib.wfd_ac_backward(ctypes.byref(g.c), _ptr(ua), _ptr(ub), _ptr(mu_a), _ptr(mu_b),
+                                  ctypes.byref(med.c), *pts, int(nsteps), _ptr(work), _ptr(G),
+                                  _stream_ptr(g.device)))
     return torch.stack([ext_view(g, G[a], med.K) for a in range(nf)]), mu_src
 
 

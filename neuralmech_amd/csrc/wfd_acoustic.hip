@@ -20,6 +20,9 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cstdio>
+#include <mutex>
+#include <vector>
 
 #include <cstring>
 
@@ -558,6 +561,73 @@ extern "C" int wfd_ac_backward(const wfd_grid* g, void* u_a, void* u_b, void* mu
     return g->dtype == WFD_F64
                ? ac_backward_t<double>(g, u_a, u_b, mu_a, mu_b, med, src, adj, strip, mu_src, nsteps, g_work, G, s)
                : ac_backward_t<float>(g, u_a, u_b, mu_a, mu_b, med, src, adj, strip, mu_src, nsteps, g_work, G, s);
+}
+
+// ------------------------------------------------ allocation-free adjoint (file-scope scratch)
+// The work fields of wfd_ac_backward (mu_a, mu_b and the 2K+1 partial-sum fields) in one
+// file-scope device array: each device holds its own copy until the process ends.  Host code
+// never uses the array's name as an address; it uses the device address hipGetSymbolAddress
+// returns for the current device, looked up once per device.
+static __device__ __align__(256) unsigned char scratch[WFD_AC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // per device, nullptr until looked up
+
+int scratch_address(void** out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0) return ac_fail(WFD_EHIP, "wfd_ac_backward_noalloc: hipGetDevice failed");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            char msg[160];
+            snprintf(msg, sizeof msg, "wfd_ac_backward_noalloc: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            return ac_fail(WFD_EHIP, msg);
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return WFD_OK;
+}
+
+}  // namespace
+
+extern "C" int wfd_ac_backward_noalloc(const wfd_grid* g, void* u_a, void* u_b, const wfd_ac_medium* med,
+                                       const wfd_points* src, const wfd_points* adj, const wfd_points* strip,
+                                       const wfd_points* mu_src, int64_t nsteps, double* G, wfd_stream_t stream)
+{
+    int rc = ac_check(g, med);
+    if (rc) return rc;
+    if (!u_a || !u_b || u_a == u_b || !G) return ac_fail(WFD_EINVAL, "wfd_ac_backward_noalloc: NULL or aliased field");
+    if (nsteps < 0) return ac_fail(WFD_EINVAL, "wfd_ac_backward_noalloc: negative step count");
+    // mu_a, mu_b, then the 2K+1 partial-sum fields, each 256-byte aligned
+    const uint64_t es = g->dtype == WFD_F64 ? 8 : 4;
+    const uint64_t fb = ((uint64_t)g->elems * (uint64_t)g->nshots * es + 255) / 256 * 256;
+    const uint64_t nf = 2 * (uint64_t)(g->order / 2) + 1;
+    const uint64_t need = (2 + nf) * fb;
+    if (need > (uint64_t)WFD_AC_SCRATCH_BYTES) {
+        char msg[240];
+        snprintf(msg, sizeof msg,
+                 "wfd_ac_backward_noalloc: the work fields need %llu bytes; the allocation-free variant holds "
+                 "at most %llu (use wfd_ac_backward with a workspace)",
+                 (unsigned long long)need, (unsigned long long)WFD_AC_SCRATCH_BYTES);
+        return ac_fail(WFD_EINVAL, msg);
+    }
+    void* base = nullptr;
+    rc = scratch_address(&base);
+    if (rc) return rc;
+    char* p = static_cast<char*>(base);
+    hipStream_t s = (hipStream_t)stream;
+    // the kernels clear mu_a, mu_b and the partial sums before use (fill kernels, no memset)
+    return g->dtype == WFD_F64
+               ? ac_backward_t<double>(g, u_a, u_b, p, p + fb, med, src, adj, strip, mu_src, nsteps, p + 2 * fb, G, s)
+               : ac_backward_t<float>(g, u_a, u_b, p, p + fb, med, src, adj, strip, mu_src, nsteps, p + 2 * fb, G, s);
 }
 
 extern "C" int wfd_ac_set_rows(int32_t rows)

@@ -328,3 +328,23 @@ def test_graph_object_argument_validation_needs_no_gpu():
     assert lib.wfd_fgraph_create(None, ctypes.byref(g), p, p + 256, ctypes.byref(med), None, None, 20) == _lib.WFD_EINVAL
     assert lib.wfd_fgraph_run(None, 0, 10, None) == _lib.WFD_EINVAL
     assert lib.wfd_fgraph_destroy(None) == _lib.WFD_OK
+
+
+def test_noalloc_adjoint_rejects_problems_beyond_its_scratch_needs_no_gpu():
+    """wfd_ac_backward_noalloc refuses, before any device work, a problem whose work fields
+    ((2 + 2K+1) fields of nshots * elems, each rounded to 256 bytes) exceed WFD_AC_SCRATCH_BYTES."""
+    lib = _lib.load()
+    assert _lib.WFD_AC_SCRATCH_BYTES == 64 << 20
+    buf = (ctypes.c_float * 256)()
+    p = (ctypes.addressof(buf) + 255) // 256 * 256
+    med = _lib.AcMedium()
+    med.k = med.cn = med.cw = p
+    rc, g = _grid((1024, 1024), (1.0, 1.0), order=8, nshots=8)
+    need = 11 * ((g.elems * 8 * 4 + 255) // 256 * 256)
+    assert need > _lib.WFD_AC_SCRATCH_BYTES
+    G = (ctypes.c_double * 4)()
+    rc = lib.wfd_ac_backward_noalloc(ctypes.byref(g), p, p + 64, ctypes.byref(med), None, None, None, None, 4,
+                                     ctypes.addressof(G), None)
+    assert rc == _lib.WFD_EINVAL and "at most" in _lib.last_error()
+    assert lib.wfd_ac_backward_noalloc(ctypes.byref(g), p, p, ctypes.byref(med), None, None, None, None, 4,
+                                       ctypes.addressof(G), None) == _lib.WFD_EINVAL  # aliased state
