@@ -222,7 +222,28 @@ def uneven_main(args):
     if not args.no_sequential:
         seq, t_seq = uneven.sequential(shots)
         diff = max(float((a[1] - s_).norm() / s_.norm()) for a, s_ in zip(grads, seq))
+        del seq
     nshots = len(shots)
+    plain = None
+    if args.uneven_plain > 0:  # plain per-shot torch autograd on a sample, extrapolated by cell-steps
+        from neuralmech_amd import fwi
+
+        torch.cuda.empty_cache()
+        pick = uneven.plain_sample(shots, args.uneven_plain)
+        t_p, cs_p, errs = 0.0, 0, []
+        for k in pick:
+            (_, gp), dt_k = fwi.timed(uneven.plain_shot_gradient, shots[k])
+            t_p += dt_k
+            cs_p += uneven.cell_steps([shots[k]])
+            errs.append(float((gp.double() - grads[k][1]).norm() / grads[k][1].norm()))
+            del gp
+            torch.cuda.empty_cache()
+        t_plain_all = t_p / cs_p * cs
+        plain = {"plain_s_per_shot": round(t_plain_all / nshots, 4),
+                 "speedup_vs_plain": round(t_plain_all / t_batch, 2),
+                 "plain_sample": f"shots {pick} ({cs_p / 1e9:.1f} G cell-steps, {t_p:.1f} s) of torch-on-GPU "
+                                 "slice-form step + autograd, extrapolated to all shots by cell-steps",
+                 "plain_grad_rel_l2_vs_batch": errs}
     gbs = 64.0 * cs / t_run / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "uneven_batch_traffic.json")
@@ -257,6 +278,7 @@ def uneven_main(args):
         "sequential_s_per_shot": None if t_seq is None else round(t_seq / nshots, 5),
         "speedup_vs_sequential": None if t_seq is None else round(t_seq / t_batch, 2),
         "grad_rel_l2_vs_sequential": diff,
+        **(plain or {}),
         "setup_s": round(t_setup, 1),
         "tune_s": round(t_tune, 1),
     }
@@ -689,6 +711,8 @@ def main():
     ap.add_argument("--max-active", type=int, default=0, help="uneven: cells in flight (0 = default)")
     ap.add_argument("--no-phases", action="store_true", help="uneven: skip the per-phase launch timing")
     ap.add_argument("--no-sequential", action="store_true", help="uneven: skip the shot-by-shot baseline")
+    ap.add_argument("--uneven-plain", type=int, default=3,
+                    help="uneven: shots timed with plain per-shot torch autograd (0 = skip)")
     ap.add_argument("--csv-dir", default="", help="roofline / scaling: write the reference's CSV files here")
     ap.add_argument("--scaling-max", type=int, default=100000, help="scaling: largest grid side")
     ap.add_argument("--scaling-steps", type=int, default=400, help="scaling: steps timed per grid")

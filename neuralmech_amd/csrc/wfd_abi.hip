@@ -977,7 +977,7 @@ int wfd_tune(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med, int
         }
     } else if (g->ndim == 3) {
         for (int v : {0, (int)WFD_VARIANT_STREAM})
-            for (int r : {4, 8, 16, 32, 64}) cands.push_back({r, v});
+            for (int r : {4, 8, 16, 32, 64, 128}) cands.push_back({r, v});
     } else {
         cands.push_back({1, 0});
     }

@@ -9,6 +9,7 @@ torch ops on the GPU, recorded by autograd, then ``loss.backward()``.
 
 from __future__ import annotations
 
+import math
 import time
 
 import numpy as np
@@ -50,16 +51,31 @@ def batched_gradient(w, obs, v=None, seg=0):
     return misfit_gradient(torch.as_tensor(v), w["grid"], w["src"], w["sig"], w["rec"], obs, w["beta"], seg=seg)
 
 
-def _lap8(u, h):
-    a = [-205 / 72, 8 / 5, -1 / 5, 8 / 315, -1 / 560]
-    p = F.pad(u, (4, 4, 4, 4))
+def central_weights(order: int) -> list:
+    """``a_0 .. a_K`` of the central second difference of ``order`` (closed form; SURVEY
+    Appendix B's table: order 8 gives -205/72, 8/5, -1/5, 8/315, -1/560)."""
+    K = order // 2
+    a = [2.0 * (-1) ** (k + 1) * math.factorial(K) ** 2 / (k * k * math.factorial(K - k) * math.factorial(K + k))
+         for k in range(1, K + 1)]
+    return [-2.0 * sum(a)] + a
+
+
+def lap_slices(u, h, order: int = 8):
+    """The slice-form Laplacian of ``order`` with zero extension (torch ops, differentiable)."""
+    a = central_weights(order)
+    K = order // 2
+    p = F.pad(u, (K, K, K, K))
     n0, n1 = u.shape
-    c = p[4:4 + n0, 4:4 + n1]
+    c = p[K:K + n0, K:K + n1]
     lap = (a[0] / h[0] ** 2 + a[0] / h[1] ** 2) * c
-    for k in range(1, 5):
-        lap = lap + (a[k] / h[0] ** 2) * (p[4 - k:4 - k + n0, 4:4 + n1] + p[4 + k:4 + k + n0, 4:4 + n1])
-        lap = lap + (a[k] / h[1] ** 2) * (p[4:4 + n0, 4 - k:4 - k + n1] + p[4:4 + n0, 4 + k:4 + k + n1])
+    for k in range(1, K + 1):
+        lap = lap + (a[k] / h[0] ** 2) * (p[K - k:K - k + n0, K:K + n1] + p[K + k:K + k + n0, K:K + n1])
+        lap = lap + (a[k] / h[1] ** 2) * (p[K:K + n0, K - k:K - k + n1] + p[K:K + n0, K + k:K + k + n1])
     return lap
+
+
+def _lap8(u, h):
+    return lap_slices(u, h, 8)
 
 
 def plain_shot_gradient(w, obs_shot, shot, v=None):

@@ -6,7 +6,9 @@ The observed records come from the true models; every shot's gradient is taken a
 
 ``batched`` runs all 64 gradients in one persistent launch (:class:`~neuralmech_amd.batch.ShotBatch`);
 ``sequential`` runs the same shots one by one through the single-shot gradient plan (one launch
-per step), the plain per-shot form it replaces.
+per step); ``plain_shot_gradient`` is the plain per-shot forward-then-autograd form (the slice-form
+step in torch ops, recorded by autograd, then ``loss.backward()``), the per-shot loop of
+``projects/18_fwi/2D_fwi.py:39-41`` written in torch.
 """
 
 from __future__ import annotations
@@ -19,6 +21,7 @@ import torch
 
 from . import media, solver
 from .batch import Shot, ShotBatch
+from .fwi import lap_slices
 
 SPACING = (10.0, 10.0)
 
@@ -88,3 +91,50 @@ def sequential(shots, dtype=torch.float32):
         del plan
     torch.cuda.synchronize()
     return out, time.perf_counter() - t0
+
+
+def plain_shot_gradient(sh: Shot, dtype=torch.float32):
+    """``(J, dJ/dv)`` of one shot, the plain way: the shot's slice-form step (its order, sponge,
+    source divided by the cell volume in the signal) in torch ops on the GPU, recorded by autograd
+    over every step, then ``loss.backward()`` with ``J = 1/2 sum (d - d_obs)^2``.  The damping
+    factors are formed once, outside the step loop, so the tape holds about one field per step
+    (the largest shot, 1536^2 x 5000 steps, needs ~50 GB)."""
+    dev = torch.device("cuda")
+    n0, n1 = sh.shape
+    v = torch.as_tensor(sh.velocity, dtype=dtype, device=dev).requires_grad_(True)
+    if sh.beta_profiles is not None:
+        beta = torch.as_tensor(np.add.outer(sh.beta_profiles[0], sh.beta_profiles[1]), dtype=dtype, device=dev)
+    else:
+        beta = torch.zeros(n0, n1, dtype=dtype, device=dev)
+    keep, inv = 1 - beta, 1 / (1 + beta)
+    mask = torch.zeros(n0, n1, dtype=dtype, device=dev)
+    mask[1:-1, 1:-1] = 1.0
+    kc = (v * sh.dt) ** 2
+    up = torch.zeros(n0, n1, dtype=dtype, device=dev)
+    uc = torch.zeros(n0, n1, dtype=dtype, device=dev)
+    (si, sj), = [tuple(int(c) for c in p) for p in sh.src_nodes]
+    ri = torch.as_tensor([int(r[0]) for r in sh.rec_nodes], device=dev)
+    rj = torch.as_tensor([int(r[1]) for r in sh.rec_nodes], device=dev)
+    sig = torch.as_tensor(np.asarray(sh.signal).reshape(sh.nsteps, -1)[:, 0], dtype=dtype, device=dev)
+    hit = torch.zeros(n0, n1, dtype=dtype, device=dev)
+    hit[si, sj] = 1.0
+    traces = []
+    for t in range(sh.nsteps):
+        num = 2 * uc - keep * up + kc * (lap_slices(uc, sh.spacing, sh.order) + hit * sig[t])
+        un = num * inv * mask
+        traces.append(un[ri, rj])
+        up, uc = uc, un
+    d = torch.stack(traces)
+    r = d - torch.as_tensor(sh.observed, dtype=dtype, device=dev)
+    J = 0.5 * (r * r).sum()
+    J.backward()
+    return float(J.detach()), v.grad.detach()
+
+
+def plain_sample(shots, k: int = 3):
+    """Indices of ``k`` shots spread over the batch's range of cell-steps (smallest, ..., largest)."""
+    cs = [(s.shape[0] - 2) * (s.shape[1] - 2) * s.nsteps for s in shots]
+    order = sorted(range(len(shots)), key=lambda i: cs[i])
+    if k <= 1:
+        return [order[len(order) // 2]]
+    return sorted({order[round(j * (len(order) - 1) / (k - 1))] for j in range(k)})

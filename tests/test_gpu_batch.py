@@ -173,3 +173,19 @@ def test_batch_float64_gradient_matches_central_differences():
             vm[cell] -= hs
             fd = (J(vp) - J(vm)) / (2 * hs)
             assert fd == pytest.approx(g[cell], rel=1e-5, abs=1e-7 * np.abs(g).max())
+
+
+@pytest.mark.parametrize("order", [4, 8])
+def test_plain_autograd_baseline_computes_the_same_gradient(order):
+    """The plain per-shot baseline of the uneven workload (slice-form torch step + autograd) and
+    the hand-written adjoint give the same gradient up to fp32 rounding (they differ in operation
+    order, not in terms)."""
+    from neuralmech_amd import uneven
+
+    spec = [((60, 90), order, 70, 0, True)]
+    sh = _shots(specs=spec, seed=5)[0]
+    ref = _plan_reference(sh, torch.float32)
+    _, g = uneven.plain_shot_gradient(sh)
+    err = float((g.double() - ref["grad"]).norm() / ref["grad"].norm())
+    print(f"\nplain autograd vs adjoint (order {order}): {err:.2e}")
+    assert err < 1e-4
