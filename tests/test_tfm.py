@@ -131,3 +131,54 @@ def test_gpu_analytic_signal_fp32():
     got = analytic_signal(torch.as_tensor(x).cuda(), axis=-1).cpu().numpy()
     ref = tfm_ref.hilbert(x.astype(np.float64), axis=-1)
     assert np.abs(got - ref).max() < 1e-5 * np.abs(ref).max()
+
+
+def _end_of_record_bound(h, el, t0, dt, c, x, y, form, eps=2e-3):
+    """Per pixel, the largest difference an fp32 evaluation may show against the float64 delays of
+    the restatement at the end of the record: a pair whose delay t (in samples) lies within eps of
+    the skip threshold (all pairs: t < Nt - 2 kept; reciprocal: t < Nt - 1 kept) may be kept by one
+    and skipped by the other, which changes the sum by at most the pair's weight times the largest
+    |h| of its samples next to the threshold.  Zero on pixels with no such pair."""
+    E, _, Nt = h.shape
+    T = Nt - 2 if form == "all" else Nt - 1
+    tau = [np.hypot(x.astype(np.float64) - ex, y.astype(np.float64) - ey) / c for ex, ey in el.astype(np.float64)]
+    bound = np.zeros(x.shape)
+    for e in range(E):
+        for r in range(E) if form == "all" else range(e, E):
+            t = (t0 + tau[e] + tau[r]) / dt
+            near = np.abs(t - T) < eps
+            if near.any():
+                wgt = 1.0 if form == "all" or r == e else 2.0
+                bound += near * wgt * np.abs(h[e, r, T - 1:T + 1]).max()
+    return bound
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["all", "reciprocal"])
+def test_gpu_tfm_end_of_record(form):
+    """A record cut while the pulses are still arriving (330 samples: the traces are large at the
+    last samples), on a 1030^2 image: pixels where no pair's delay lies near the skip threshold
+    match the restatement within 2e-5 of the image maximum; on the pixels where one does, the
+    difference stays within the skipped pairs' contribution (``_end_of_record_bound``)."""
+    from neuralmech_amd import tfm
+
+    h, el, t0, dt, c = _capture(E=12, Nt=330)
+    assert np.abs(h[:, :, -3:]).max() > 0.5  # the regime: nonzero at the end of the record
+    x, y = np.meshgrid(np.linspace(0, 1, 1030), np.linspace(0, 1, 1033), indexing="ij")
+    args32 = (h.astype(np.complex64), el.astype(np.float32), np.float32(t0), np.float32(dt), np.float32(c),
+              x.astype(np.float32), y.astype(np.float32))
+    if form == "all":
+        got, ref = tfm.tfm(h, el, t0, dt, c, x, y), tfm_ref.tfm_all_pairs(*args32)
+    else:
+        got, ref = tfm.tfm_kernel(h, el, t0, dt, c, x, y), tfm_ref.tfm_reciprocal(*args32)
+    got = got.cpu().numpy()
+    bound = _end_of_record_bound(*args32, form)
+    diff = np.abs(got - ref)
+    tol = 2e-5 * ref.max()
+    border = bound > 0
+    print(f"\n[TFM end of record, {form}] {border.sum()} pixels near the threshold; elsewhere largest "
+          f"difference {diff[~border].max() / ref.max():.1e} of the maximum; near it "
+          f"{diff[border].max() / ref.max():.1e} (bound {bound[border].max() / ref.max():.1e})")
+    assert border.sum() > 0
+    assert diff[~border].max() <= tol
+    assert np.all(diff[border] <= bound[border] + tol)
