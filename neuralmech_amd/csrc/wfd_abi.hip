@@ -318,11 +318,16 @@ int launch_t(const Launch& L, hipStream_t s)
         if (e != hipSuccess) return hip_fail(e, "gather kernel launch");
         return WFD_OK;
     }
-    if (g->ndim == 3)
-        e = launch_step3d<T>(a, g->order, varc, sponge, L.img, std::max(1, L.cfg.ri),
-                             (L.cfg.tile & WFD_VARIANT_STREAM) != 0, g->nshots, s);
-    else
+    if (g->ndim == 3) {
+        // The tuner times forward launches.  The imaging (adjoint) instance holds more registers
+        // (3 blocks per CU instead of 4), so above 64 planes per block its grid would end in a
+        // partly filled round of blocks: it marches at most 64 (any RK gives the same bits).
+        const int rk = L.img ? std::min(64, std::max(1, L.cfg.ri)) : std::max(1, L.cfg.ri);
+        e = launch_step3d<T>(a, g->order, varc, sponge, L.img, rk, (L.cfg.tile & WFD_VARIANT_STREAM) != 0, g->nshots,
+                             s);
+    } else {
         e = launch_step2d<T>(a, g->ndim == 2, g->order, varc, sponge, L.img, L.cfg.tile, L.cfg.ri, g->nshots, s);
+    }
     if (e != hipSuccess) return hip_fail(e, "step kernel launch");
     return WFD_OK;
 }

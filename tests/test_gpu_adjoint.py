@@ -157,7 +157,7 @@ def test_adjoint_launch_shapes_bitexact(variant, rows):
     assert torch.equal(tr, base[1]) and torch.equal(grad, base[3])
 
 
-@pytest.mark.parametrize("variant,rows", [(0, 4), (2, 4), (2, 16)])
+@pytest.mark.parametrize("variant,rows", [(0, 4), (2, 4), (2, 16), (2, 128)])
 def test_adjoint_3d_launch_shapes_bitexact(variant, rows):
     """3D gradient plan (forward, recompute, adjoint with imaging) under forced launch shapes,
     streaming access included, against the default shape: same traces and gradient bits."""
