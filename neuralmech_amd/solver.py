@@ -380,6 +380,10 @@ class ForwardRun:
         self._rp = rec.struct(self.traces) if rec is not None else None
         self._work = (grid.zeros(), grid.zeros()) if self.two_step else None
         self._graphs = {}
+        # the call's fixed arguments, built once: advance() only adds the buffers, step and count
+        self._lib = _lib.load()
+        sp, rp = self._pts()
+        self._fixed = (ctypes.byref(grid.c), ctypes.byref(medium.c), sp, rp)
 
     def _pts(self):
         return (ctypes.byref(self._sp) if self._sp else None, ctypes.byref(self._rp) if self._rp else None)
@@ -406,20 +410,19 @@ class ForwardRun:
             raise ValueError(f"{nsteps} steps from step {st.step} pass total = {self.total}")
         if nsteps == 0:
             return st
-        lib = _lib.load()
-        sp, rp = self._pts()
+        gp, mp, sp, rp = self._fixed
         stream = _stream_ptr(g.device)
         if self.two_step:
             wc, wd = self._work
-            check(lib.wfd_forward2(ctypes.byref(g.c), _ptr(st.prev), _ptr(st.cur), _ptr(wc), _ptr(wd),
-                                   ctypes.byref(self.medium.c), sp, rp, st.step, nsteps, self.two_step, stream))
+            check(self._lib.wfd_forward2(gp, st.prev.data_ptr(), st.cur.data_ptr(), wc.data_ptr(), wd.data_ptr(), mp,
+                                         sp, rp, st.step, nsteps, self.two_step, stream))
             st.step += nsteps  # (u^{N-1}, u^N) stay in (prev, cur)
             return st
         if self.graph_chunk:
             self._graph().run(st.step, nsteps)
         else:
-            check(lib.wfd_forward(ctypes.byref(g.c), _ptr(st.prev), _ptr(st.cur), ctypes.byref(self.medium.c),
-                                  sp, rp, st.step, nsteps, stream))
+            check(self._lib.wfd_forward(gp, st.prev.data_ptr(), st.cur.data_ptr(), mp, sp, rp, st.step, nsteps,
+                                        stream))
         if nsteps % 2:  # the newest level is now in the buffer that held u^{n-1}
             st.prev, st.cur = st.cur, st.prev
         st.step += nsteps
