@@ -119,6 +119,44 @@ __device__ __forceinline__ void stage_plane(T (*t)[T3_LW], const T* u, const V4<
     }
 }
 
+// The in-plane halo chunk this thread loads for plane k of field u (threads 0..127: the 4 rows above
+// and below the tile, 16 chunks each; threads 128..159: the 4-column chunks left and right of each
+// tile row; zero for the others and for rows past the layout), and where it goes in the LDS tile.
+constexpr int T3_HALO_THREADS = 128 + 2 * T3_TI;
+template <typename T>
+__device__ __forceinline__ V4<T> halo_fetch(const T* u, int64_t k, int64_t i0, int64_t j0, int64_t nr, int64_t ld,
+                                            int64_t ps)
+{
+    const int tid = threadIdx.x;
+    V4<T> v = {{T(0), T(0), T(0), T(0)}};
+    if (tid < 128) {
+        const int hr = tid >> 4, hc = tid & 15;
+        const int64_t row = hr < 4 ? i0 - 4 + hr : i0 + T3_TI + (hr - 4);
+        if (row <= nr + 3) v = gld4(u + k * ps + row * ld + j0 + hc * 4);
+    } else if (tid < T3_HALO_THREADS) {
+        const int s = tid - 128, side = s / T3_TI, rr = s % T3_TI;
+        const int64_t row = i0 + rr;
+        if (row <= nr + 3) v = gld4(u + k * ps + row * ld + (side ? j0 + T3_TJ : j0 - 4));
+    }
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ void halo_put(T (*t)[T3_LW], const V4<T>& v)
+{
+    const int tid = threadIdx.x;
+    if (tid < 128) {
+        const int hr = tid >> 4, hc = tid & 15;
+        const int lr = hr < 4 ? hr : T3_TI + HALO + (hr - 4);
+#pragma unroll
+        for (int e = 0; e < 4; e++) t[lr][HALO + hc * 4 + e] = v.e[e];
+    } else if (tid < T3_HALO_THREADS) {
+        const int s = tid - 128, side = s / T3_TI, rr = s % T3_TI;
+        const int lc = side ? HALO + T3_TJ : 0;
+#pragma unroll
+        for (int e = 0; e < 4; e++) t[HALO + rr][lc + e] = v.e[e];
+    }
+}
+
 template <typename T, int K>
 __device__ __forceinline__ T lap3(const T wc, const T* wp, const T* wr, const T* wcol, const V4<T>* win,
                                   const T (*t)[T3_LW], int ty, int tx, int e)
@@ -134,10 +172,18 @@ __device__ __forceinline__ T lap3(const T wc, const T* wp, const T* wr, const T*
     return lap;
 }
 
+// Forward launches load the in-plane halo of plane k + K -- the plane whose centre chunks the
+// neighbouring tiles load into their z-windows in the same iteration, so the lines are fresh in
+// the L2 -- and keep it K iterations in an LDS ring of K + 1 halo slots, from which the padded
+// tile of plane k is completed.  Written at iteration q - K and read at iteration q (both before
+// that iteration's barrier), a slot's previous plane q - K - 1 was read one barrier earlier.
+// Imaging launches (two staged fields, more registers) load the halo of plane k directly.
 template <typename T, int K, bool VARC, bool SPONGE, bool IMG, bool NT>
 __global__ __launch_bounds__(T3_THREADS) void step3d_kernel(const StepArgs<T> a, int rk, int nkb)
 {
+    constexpr bool RING = !IMG;
     __shared__ T tile[IMG ? 2 : 1][2][T3_LH][T3_LW];
+    __shared__ V4<T> ring[RING ? K + 1 : 1][RING ? T3_HALO_THREADS : 1];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -192,6 +238,12 @@ __global__ __launch_bounds__(T3_THREADS) void step3d_kernel(const StepArgs<T> a,
     uint64_t src_mask = 0;
     int64_t mask_k = k0 - 64;
     int buf = 0;
+    if constexpr (RING) {  // halos of the first K planes
+        if (threadIdx.x < T3_HALO_THREADS)
+            for (int t = 0; t < K; t++)
+                if (k0 + t < k1) ring[(k0 + t) % (K + 1)][threadIdx.x] = halo_fetch(uc, k0 + t, i0, j0, a.nr, ld, ps);
+        __syncthreads();
+    }
     for (int64_t k = k0; k < k1; k++, buf ^= 1) {
 #pragma unroll
         for (int t = 0; t < 2 * K; t++) win[t] = win[t + 1];
@@ -201,8 +253,18 @@ __global__ __launch_bounds__(T3_THREADS) void step3d_kernel(const StepArgs<T> a,
             for (int t = 0; t < 2 * K; t++) wf[t] = wf[t + 1];
             wf[2 * K] = load_row ? gld4(uf + (k + K) * ps + rowoff) : zero;
         }
-        stage_plane<T>(tile[0][buf], uc, win[K], k, i0, j0, ty, tx, a.nr, ld, ps);
-        if constexpr (IMG) stage_plane<T>(tile[IMG ? 1 : 0][buf], uf, wf[K], k, i0, j0, ty, tx, a.nr, ld, ps);
+        if constexpr (RING) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) tile[0][buf][ty + HALO][HALO + tx * 4 + e] = win[K].e[e];
+            if (threadIdx.x < T3_HALO_THREADS) {
+                const V4<T> next = k + K < k1 ? halo_fetch(uc, k + K, i0, j0, a.nr, ld, ps) : zero;
+                halo_put(tile[0][buf], ring[k % (K + 1)][threadIdx.x]);
+                if (k + K < k1) ring[(k + K) % (K + 1)][threadIdx.x] = next;
+            }
+        } else {
+            stage_plane<T>(tile[0][buf], uc, win[K], k, i0, j0, ty, tx, a.nr, ld, ps);
+            if constexpr (IMG) stage_plane<T>(tile[IMG ? 1 : 0][buf], uf, wf[K], k, i0, j0, ty, tx, a.nr, ld, ps);
+        }
         __syncthreads();
         const int64_t o = k * ps + rowoff;
         const V4<T> P = out ? gld4s<NT>(up + o) : zero;
