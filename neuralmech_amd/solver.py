@@ -19,9 +19,16 @@ from ._lib import WFD_F32, WFD_F64, check
 _DTYPES = {torch.float32: WFD_F32, torch.float64: WFD_F64}
 
 
+_RAW_STREAM = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+
+
 def _stream_ptr(device: torch.device):
+    """torch's current stream on ``device`` as a ``hipStream_t`` (the raw handle query skips the
+    construction of a ``torch.cuda.Stream`` object: a few microseconds per launching call)."""
     if device.type != "cuda":
         raise RuntimeError("the wave kernels run on a ROCm GPU; got a tensor on " + str(device))
+    if _RAW_STREAM is not None:
+        return ctypes.c_void_p(_RAW_STREAM(device.index if device.index is not None else torch.cuda.current_device()))
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
