@@ -166,7 +166,9 @@ int wfd_forward2(const wfd_grid* g, void* u_a, void* u_b, void* u_c, void* u_d,
  * plain launches; u_a, u_b rotate as there.  The buffers, the medium's fields and the point
  * lists' arrays must stay allocated while the object exists (their descriptors are copied at
  * creation).  Runs of one object must not overlap in time (they share its counter); objects on
- * different streams run concurrently.  wfd_fgraph_destroy waits for the object's last run. */
+ * different streams run concurrently.  wfd_fgraph_destroy waits for the object's last run.  On a
+ * stream that the caller is capturing into a graph, a run records plain launches instead, so the
+ * caller's graph holds nothing the object owns. */
 typedef struct wfd_fgraph wfd_fgraph;
 int wfd_fgraph_create(wfd_fgraph** fg, const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med,
                       const wfd_points* src, const wfd_points* rec, int32_t chunk);
@@ -176,7 +178,8 @@ int wfd_fgraph_destroy(wfd_fgraph* fg);
 /* Same as wfd_forward, replayed from a captured hipGraph of `chunk` steps (chunk even).  Graphs
  * are kept between calls in a process-wide cache (16 entries, least recently used dropped) keyed
  * by the stream and everything a capture depends on, so repeated calls capture once and calls on
- * different streams never share a step counter; runs shorter than one chunk are plain launches.
+ * different streams never share a step counter; runs shorter than one chunk, and calls on a
+ * stream being captured by the caller, are plain launches.
  * Same contract as wfd_fgraph_run (replaces the loop of 2D_wave_cuda_graph.py:91-96). */
 int wfd_forward_graph(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med,
                       const wfd_points* src, const wfd_points* rec, int64_t step0,

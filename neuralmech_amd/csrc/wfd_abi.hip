@@ -478,14 +478,29 @@ int fgraph_make(wfd_fgraph** out, const wfd_grid* g, void* u_a, void* u_b, const
     return WFD_OK;
 }
 
+// True when `st` is being captured into a graph by the caller.  A failed query (which HIP also
+// leaves as the last error, read by the launch checks) counts as "not capturing".
+bool capturing(hipStream_t st)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return cs != hipStreamCaptureStatusNone;
+}
+
 int fgraph_run(wfd_fgraph* f, int64_t step0, int64_t nsteps, hipStream_t st)
 {
     if (nsteps < 0) return fail(WFD_EINVAL, "nsteps must be >= 0");
     const wfd_points* src = f->has_src ? &f->src : nullptr;
     const wfd_points* rec = f->has_rec ? &f->rec : nullptr;
+    // Under the caller's own capture the steps are recorded as plain launches (same kernels,
+    // same results), so the caller's graph references nothing this object owns.
+    if (capturing(st)) return wfd_forward(&f->g, f->u_a, f->u_b, &f->med, src, rec, step0, nsteps, st);
     const int64_t reps = nsteps / f->chunk;
     if (reps > 0) {
-        // counter = step0 (a kernel, stream-ordered, so the graph may itself be captured), replays
+        // counter = step0 (a stream-ordered kernel), then the replays
         hipError_t e = launch_set_i64(f->ctr, step0, st);
         for (int64_t r = 0; e == hipSuccess && r < reps; r++) e = hipGraphLaunch(f->exec, st);
         if (e == hipSuccess) e = hipEventRecord(f->done, st);
@@ -793,7 +808,7 @@ int wfd_forward_graph(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium*
     if (nsteps < 0) return fail(WFD_EINVAL, "nsteps must be >= 0");
     if (chunk < 2 || chunk % 2) return fail(WFD_EINVAL, "chunk must be even and >= 2 (got %d)", chunk);
     hipStream_t st = (hipStream_t)stream;
-    if (nsteps < chunk) return wfd_forward(g, u_a, u_b, med, src, rec, step0, nsteps, stream);
+    if (nsteps < chunk || capturing(st)) return wfd_forward(g, u_a, u_b, med, src, rec, step0, nsteps, stream);
     const std::string key = graph_key(g, u_a, u_b, med, src, rec, chunk);
     std::lock_guard<std::mutex> lk(g_graph_mu);
     wfd_fgraph* f = nullptr;

@@ -1,7 +1,8 @@
 """GPU tests of resuming a run and of graph replay: run(N1) then run(N2) from the returned state
 equals run(N1 + N2) bit for bit (plain launches, captured graphs, two steps per launch; odd and
 even N1), graphs are captured once and replayed, concurrent graph replays on two streams give the
-single-stream bits, and media are checked for NaN / non-positive values and the CFL limit."""
+single-stream bits, both graph interfaces can be captured into a caller's graph, and media
+are checked for NaN / non-positive values and the CFL limit."""
 
 import ctypes
 
@@ -154,6 +155,49 @@ def test_concurrent_graph_replays_on_two_streams(api):
         assert np.array_equal(ref[2][0].cpu().numpy(), tr_o)
     for fg in stg_graph.values():
         lib.wfd_fgraph_destroy(fg)
+
+
+@pytest.mark.parametrize("api", ["object", "cached"])
+def test_graph_apis_under_caller_capture(api):
+    """wfd_fgraph_run / wfd_forward_graph called while the caller captures the stream into its own
+    graph record plain launches: each of three replays (fields and traces reset by eager work
+    between them) gives the oracle's bits."""
+    from neuralmech_amd import _lib, solver
+
+    print(f"\n[caller capture] api={api}", flush=True)
+    lib = _lib.load()
+    case = _case()
+    g, med, src, rec, sig = _gpu(case)
+    st = solver.WaveState.rest(g)
+    s = solver._sig_tensor(g, src, sig, 50)
+    tr = torch.zeros(1, 50, rec.count, device="cuda")
+    sp, rp = src.struct(s), rec.struct(tr)
+    fg = ctypes.c_void_p()
+    if api == "object":
+        _lib.check(lib.wfd_fgraph_create(ctypes.byref(fg), ctypes.byref(g.c), st.prev.data_ptr(), st.cur.data_ptr(),
+                                         ctypes.byref(med.c), ctypes.byref(sp), ctypes.byref(rp), 4))
+    torch.cuda.synchronize()
+    cg = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(cg):
+        h = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        if api == "object":
+            _lib.check(lib.wfd_fgraph_run(fg, 0, 50, h))
+        else:
+            _lib.check(lib.wfd_forward_graph(ctypes.byref(g.c), st.prev.data_ptr(), st.cur.data_ptr(),
+                                             ctypes.byref(med.c), ctypes.byref(sp), ctypes.byref(rp), 0, 50, 4, h))
+    tr_o, up_o, uc_o = oracle_forward(case, np.float32)
+    for k in range(3):
+        st.prev.zero_()
+        st.cur.zero_()
+        tr.fill_(float(k + 1))
+        torch.cuda.synchronize()
+        cg.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(tr[0].cpu().numpy(), tr_o), f"replay {k}: traces"
+        assert np.array_equal(g.nodes(st.cur)[0].cpu().numpy(), uc_o), f"replay {k}: field"
+        assert np.array_equal(g.nodes(st.prev)[0].cpu().numpy(), up_o), f"replay {k}: previous field"
+    if api == "object":
+        _lib.check(lib.wfd_fgraph_destroy(fg))
 
 
 def test_medium_check_rejects_unstable_and_nonfinite_fields():
