@@ -416,9 +416,10 @@ def volume_main(args):
     tuned = None
     if args.launch:
         g.set_launch(*[int(x) for x in args.launch.split(",")])
-    elif not args.no_tune:
+    elif not args.no_tune and not args.two_step:
         tuned = g.tune(med, steps=6)
-    run = solver.ForwardRun(g, med, nsteps, src=src, sig=sig, rec=rec)  # staged outside the timing
+    # staged outside the timing; --two-step P: two steps per launch, P planes per block
+    run = solver.ForwardRun(g, med, nsteps, src=src, sig=sig, rec=rec, two_step=args.two_step)
     run.advance(4)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -431,15 +432,18 @@ def volume_main(args):
     steps = nsteps - 4
     cells = g.interior_cells
     ms_step = ms / steps
-    achieved = 16 * cells / (ms_step / 1e3) / 1e9
+    bpc = 10 if args.two_step else 16  # u_prev, u_cur, kc read and u_next written once per (two) step(s)
+    achieved = bpc * cells / (ms_step / 1e3) / 1e9
     out = {
         "metric": "Gcell-updates/s (3D forward, fp32)", "value": round(cells * steps / (ms / 1e3) / 1e9, 3),
         "unit": "Gcell-updates/s", "n_gpus": 1, "steps": steps, "warmup": 4, "ms_per_step": round(ms_step, 4),
         "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded smoothed random velocity)",
         "config": {"workload": f"3D forward {n}^3 nodes, order 8 (25-point), 32-node sponge, Ricker 8 Hz, "
-                               f"{rec.count} receivers", "launch": {"rows": g.launch()[1], "tuned": tuned}},
+                               f"{rec.count} receivers",
+                   "launch": ({"two_step_planes": args.two_step} if args.two_step
+                              else {"rows": g.launch()[1], "tuned": tuned})},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_cell": 16},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_cell": bpc},
         "finite": bool(torch.isfinite(traces).all()),
     }
     del run, traces
@@ -703,7 +707,8 @@ def main():
                                            "acoustic", "plate", "tfm"],
                     default="forward")
     ap.add_argument("--launch", default="", help="forward / 3d: force VARIANT,ROWS instead of tuning")
-    ap.add_argument("--two-step", type=int, default=0, help="forward: two steps per launch, ROWS output rows per wave")
+    ap.add_argument("--two-step", type=int, default=0,
+                    help="forward: two steps per launch, ROWS output rows per wave (2D) or planes per block (3D)")
     ap.add_argument("--n3d", type=int, default=512)
     ap.add_argument("--steps3d", type=int, default=2000)
     ap.add_argument("--grad", action="store_true", help="3D: also time the checkpointed gradient")

@@ -145,13 +145,15 @@ int wfd_forward(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med,
                 const wfd_points* src, const wfd_points* rec, int64_t step0, int64_t nsteps,
                 wfd_stream_t stream);
 
-/* nsteps steps like wfd_forward, two per launch (2D): each launch reads u^{n-1}, u^n and kc once
- * and writes u^{n+1}, u^{n+2} (20 bytes per node per two steps instead of 32), recomputing a
- * 2K-row / 8-column halo of step 1; bit-identical to single steps.  u_c, u_d are work fields of
- * the same size (all four distinct).  On return u_a = u^{N-1}, u_b = u^N.  rows = output rows
- * per wave: 2, 4, 8 (0 = 8) keep the intermediate level of a 4-wave block's rows in LDS; 16, 32
- * march both levels in registers.  Replaces temporal_step (step2D_wave.cu:244-304) and its 4-array
- * rotation (2D_wave_cuda_temporal.py:110-112) for the general update. */
+/* nsteps steps like wfd_forward, two per launch (2D, 3D): each launch reads u^{n-1}, u^n and kc
+ * once and writes u^{n+1}, u^{n+2} (20 bytes per node per two steps instead of 32), recomputing a
+ * halo of step 1; bit-identical to single steps.  u_c, u_d are work fields of the same size (all
+ * four distinct).  On return u_a = u^{N-1}, u_b = u^N.  2D: rows = output rows per wave: 2, 4, 8
+ * (0 = 8) keep the intermediate level of a 4-wave block's rows in LDS; 16, 32 march both levels
+ * in registers.  3D (float32 only): rows = planes marched per block, 1 .. 4096 (0 = 32); step 1
+ * on a 48 x 64 in-plane tile, step 2 on its inner 40 x 56.  Replaces temporal_step
+ * (step2D_wave.cu:244-304) and its 4-array rotation (2D_wave_cuda_temporal.py:110-112) for the
+ * general update. */
 int wfd_forward2(const wfd_grid* g, void* u_a, void* u_b, void* u_c, void* u_d,
                  const wfd_medium* med, const wfd_points* src, const wfd_points* rec,
                  int64_t step0, int64_t nsteps, int32_t rows, wfd_stream_t stream);

@@ -309,7 +309,8 @@ int launch_t(const Launch& L, hipStream_t s)
     if (L.un2) {  // two steps: the gather blocks record both input levels
         a.un2 = static_cast<T*>(L.un2) + off;
         a.ngb *= 2;
-        e = launch_step2d_t2<T>(a, g->order, varc, sponge, L.t2rows, g->nshots, s);
+        e = g->ndim == 3 ? launch_step3d_t2<T>(a, g->order, varc, sponge, L.t2rows, g->nshots, s)
+                         : launch_step2d_t2<T>(a, g->order, varc, sponge, L.t2rows, g->nshots, s);
         if (e != hipSuccess) return hip_fail(e, "two-step kernel launch");
         return WFD_OK;
     }
@@ -744,10 +745,13 @@ int wfd_forward2(const wfd_grid* g, void* u_a, void* u_b, void* u_c, void* u_d, 
 {
     int rc = check_step_args(g, u_a, u_b, u_a, med);
     if (rc) return rc;
-    if (g->ndim != 2) return fail(WFD_EINVAL, "wfd_forward2: 2D grids only");
+    if (g->ndim == 1) return fail(WFD_EINVAL, "wfd_forward2: 2D and 3D grids only");
     if (nsteps < 0) return fail(WFD_EINVAL, "nsteps must be >= 0");
-    if (rows != 0 && rows != 2 && rows != 4 && rows != 8 && rows != 16 && rows != 32)
+    if (g->ndim == 2 && rows != 0 && rows != 2 && rows != 4 && rows != 8 && rows != 16 && rows != 32)
         return fail(WFD_EINVAL, "wfd_forward2: rows must be 0, 2, 4, 8, 16 or 32");
+    if (g->ndim == 3 && (rows < 0 || rows > 4096))
+        return fail(WFD_EINVAL, "wfd_forward2: planes per block must be 0 (default) or 1 .. 4096");
+    if (g->ndim == 3 && g->dtype != WFD_F32) return fail(WFD_EINVAL, "wfd_forward2: 3D grids in float32 only");
     if (!u_c || !u_d) return fail(WFD_EINVAL, "wfd_forward2: NULL work field");
     const void* f[4] = {u_a, u_b, u_c, u_d};
     for (int i = 0; i < 4; i++)
@@ -771,7 +775,7 @@ int wfd_forward2(const wfd_grid* g, void* u_a, void* u_b, void* u_c, void* u_d, 
         // u^{n+1} -> X, u^{n+2} -> Y; the launch records u^{n-1}, u^n into rows n-2, n-1
         Launch L{g, P, C, X, med, src, rec, step0 + k, nullptr, nullptr, nullptr, false, default_cfg(g), -1};
         L.un2 = Y;
-        L.t2rows = rows ? rows : 8;
+        L.t2rows = rows ? rows : (g->ndim == 3 ? 32 : 8);
         rc = do_launch(L, st);
         if (rc) return rc;
         std::swap(P, X);

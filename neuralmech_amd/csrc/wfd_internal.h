@@ -141,6 +141,11 @@ template <typename T>
 hipError_t launch_step2d_t2(const StepArgs<T>& a, int order, bool varc, bool sponge, int rows, int nshots,
                             hipStream_t s);
 
+// Two steps per launch (3D), same contract; rk = planes marched per block.
+template <typename T>
+hipError_t launch_step3d_t2(const StepArgs<T>& a, int order, bool varc, bool sponge, int rk, int nshots,
+                            hipStream_t s);
+
 hipError_t launch_add_i64(int64_t* p, int64_t v, hipStream_t s);
 hipError_t launch_set_i64(int64_t* p, int64_t v, hipStream_t s);
 hipError_t launch_zero(void* p, int64_t bytes, hipStream_t s);

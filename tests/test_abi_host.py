@@ -259,7 +259,12 @@ def test_measurement_and_two_step_arguments_need_no_gpu():
                                 None) == _lib.WFD_EINVAL
         assert "rows" in _lib.last_error()
     _, g3 = _grid((9, 11, 13), (1.0, 1.0, 1.0))
-    assert lib.wfd_forward2(ctypes.byref(g3), f[0], f[1], f[2], f[3], ctypes.byref(med), None, None, 0, 4, 2,
+    for planes in (-1, 4097):  # 3D: rows = planes per block
+        assert lib.wfd_forward2(ctypes.byref(g3), f[0], f[1], f[2], f[3], ctypes.byref(med), None, None, 0, 4, planes,
+                                None) == _lib.WFD_EINVAL
+        assert "planes" in _lib.last_error()
+    _, g1 = _grid((40,), (1.0,))
+    assert lib.wfd_forward2(ctypes.byref(g1), f[0], f[1], f[2], f[3], ctypes.byref(med), None, None, 0, 4, 0,
                             None) == _lib.WFD_EINVAL
 
 
