@@ -1000,8 +1000,15 @@ int wfd_tune(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med, int
             }
         }
     } else if (g->ndim == 3) {
+        // plus the depths that split the interior planes into 2, 3 or 6 equal marches: with 3 or 4
+        // resident blocks per CU one of them fills whole rounds of the launch
+        std::vector<int> depths = {4, 8, 16, 32, 64, 128};
+        for (int m : {2, 3, 6}) {
+            const int d = (int)((g->n[0] - 2 + m - 1) / m);
+            if (d > 0 && std::find(depths.begin(), depths.end(), d) == depths.end()) depths.push_back(d);
+        }
         for (int v : {0, (int)WFD_VARIANT_STREAM})
-            for (int r : {4, 8, 16, 32, 64, 128}) cands.push_back({r, v});
+            for (int r : depths) cands.push_back({r, v});
     } else {
         cands.push_back({1, 0});
     }

@@ -66,6 +66,22 @@ struct StepArgs {
     T* un2;
 };
 
+// Block barrier that orders LDS only: each wave waits for its own LDS accesses (lgkmcnt), then
+// meets the others at s_barrier.  __syncthreads() also waits for every outstanding global load
+// and store of the wave (vmcnt(0)), which drains loads issued for later planes or rows; with this
+// barrier they stay in flight.  Use it only where no thread of the block reads global memory
+// another thread of the block wrote before the barrier.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// s_waitcnt immediate (gfx9 encoding) that waits for every outstanding vector memory access of
+// the wave and for nothing else: vmcnt 0, expcnt 7, lgkmcnt 15.
+constexpr int WAIT_VMCNT0 = 0x0F70;
+
 // Position of this block in a step launch: a gather block or a stencil tile.  The hardware hands
 // block b to XCD b % 8 (each XCD has its own L2) and starts an XCD's blocks in order b / 8.  The
 // first `gper` blocks of every XCD are gather blocks (so they start first); the remaining blocks

@@ -90,35 +90,6 @@ constexpr int T3_TJ = 64;  // tile columns
 constexpr int T3_LH = T3_TI + 2 * HALO;
 constexpr int T3_LW = T3_TJ + 2 * HALO;
 
-// Stage the centre plane chunk (already in registers) and the in-plane halo of plane `k`
-// of field `u` into LDS tile `t`.
-template <typename T>
-__device__ __forceinline__ void stage_plane(T (*t)[T3_LW], const T* u, const V4<T>& centre, int64_t k,
-                                            int64_t i0, int64_t j0, int ty, int tx, int64_t nr,
-                                            int64_t ld, int64_t ps)
-{
-#pragma unroll
-    for (int e = 0; e < 4; e++) t[ty + HALO][HALO + tx * 4 + e] = centre.e[e];
-    const int tid = threadIdx.x;
-    if (tid < 128) {
-        const int hr = tid >> 4, hc = tid & 15;
-        const int64_t row = hr < 4 ? i0 - 4 + hr : i0 + T3_TI + (hr - 4);
-        const int lr = hr < 4 ? hr : T3_TI + HALO + (hr - 4);
-        V4<T> v = {{T(0), T(0), T(0), T(0)}};
-        if (row <= nr + 3) v = gld4(u + k * ps + row * ld + j0 + hc * 4);
-#pragma unroll
-        for (int e = 0; e < 4; e++) t[lr][HALO + hc * 4 + e] = v.e[e];
-    } else if (tid < 128 + 2 * T3_TI) {
-        const int s = tid - 128, side = s / T3_TI, rr = s % T3_TI;
-        const int64_t row = i0 + rr;
-        V4<T> v = {{T(0), T(0), T(0), T(0)}};
-        if (row <= nr + 3) v = gld4(u + k * ps + row * ld + (side ? j0 + T3_TJ : j0 - 4));
-        const int lc = side ? HALO + T3_TJ : 0;
-#pragma unroll
-        for (int e = 0; e < 4; e++) t[HALO + rr][lc + e] = v.e[e];
-    }
-}
-
 // The in-plane halo chunk this thread loads for plane k of field u (threads 0..127: the 4 rows above
 // and below the tile, 16 chunks each; threads 128..159: the 4-column chunks left and right of each
 // tile row; zero for the others and for rows past the layout), and where it goes in the LDS tile.
@@ -159,12 +130,13 @@ __device__ __forceinline__ void halo_put(T (*t)[T3_LW], const V4<T>& v)
 
 template <typename T, int K>
 __device__ __forceinline__ T lap3(const T wc, const T* wp, const T* wr, const T* wcol, const V4<T>* win,
-                                  const T (*t)[T3_LW], int ty, int tx, int e)
+                                  const V4<T>& top, const T (*t)[T3_LW], int ty, int tx, int e)
 {
+    // z-window: planes k-K..k+K-1 in win[0..2K-1], plane k+K in top
     const int lr = ty + HALO, lc = HALO + tx * 4 + e;
     T lap = wc * win[K].e[e];
 #pragma unroll
-    for (int m = 1; m <= K; m++) lap = fma3(wp[m - 1], win[K - m].e[e] + win[K + m].e[e], lap);
+    for (int m = 1; m <= K; m++) lap = fma3(wp[m - 1], win[K - m].e[e] + (m < K ? win[K + m].e[e] : top.e[e]), lap);
 #pragma unroll
     for (int m = 1; m <= K; m++) lap = fma3(wr[m - 1], t[lr - m][lc] + t[lr + m][lc], lap);
 #pragma unroll
@@ -172,167 +144,241 @@ __device__ __forceinline__ T lap3(const T wc, const T* wp, const T* wr, const T*
     return lap;
 }
 
-// Forward launches load the in-plane halo of plane k + K -- the plane whose centre chunks the
-// neighbouring tiles load into their z-windows in the same iteration, so the lines are fresh in
-// the L2 -- and keep it K iterations in an LDS ring of K + 1 halo slots, from which the padded
-// tile of plane k is completed.  Written at iteration q - K and read at iteration q (both before
-// that iteration's barrier), a slot's previous plane q - K - 1 was read one barrier earlier.
-// Imaging launches (two staged fields, more registers) load the halo of plane k directly.
+// Per-block constants of a 3D tile and its lane's chunk.
+template <typename T>
+struct Tile3 {
+    int64_t i0, j0, i, c, k0, k1, rowoff, ld, ps, sb;
+    int shot, wave, lane, tx, ty;
+    bool load_row, out, halo_thread;
+    bool valid[4];
+    T bc[4];
+    T br;
+    int64_t wr0, wr1;  // rows of this wave (source mask)
+};
+
+// Source bookkeeping of the march: which of the next 64 planes hold sources in this wave's rows.
+struct SrcMask {
+    uint64_t bits;
+    int64_t k;
+};
+
+// Update of the lane's 4-node chunk of plane k from the z-window (`win`: planes k-K..k+K-1, `top`:
+// plane k+K), the staged in-plane tile `t`, u_prev P and coefficient kv (the update documented in
+// wavefd.h, in exactly that order), into val (zero where the node is not interior); the caller
+// stores it.
+template <typename T, int K, bool VARC, bool SPONGE, bool NT>
+__device__ __forceinline__ void plane_update(const StepArgs<T>& a, const Tile3<T>& b, const V4<T>* win,
+                                             const V4<T>& top, const T (*t)[T3_LW], const V4<T>& P, const V4<T>& kv, T bp, int64_t k,
+                                             int64_t step, SrcMask& sm, T (&val)[4])
+{
+    T kc[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) kc[e] = VARC ? kv.e[e] : a.kconst;
+    T bb[4] = {T(0), T(0), T(0), T(0)};
+    bool damp = false;
+    if constexpr (SPONGE) {
+        const T bpr = bp + b.br;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            bb[e] = bpr + b.bc[e];
+            damp |= bb[e] != T(0);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const T ctr = win[K].e[e];
+        const T lap = lap3<T, K>(a.wc, a.wp, a.wr, a.wcol, win, top, t, b.ty, b.tx, e);
+        const T tt = damp ? T(2) * ctr - (T(1) - bb[e]) * P.e[e] : T(2) * ctr - P.e[e];
+        val[e] = fma3(kc[e], lap, tt);
+    }
+    // source pass: the wave holds 4 rows of the tile, 16 lanes each.  Which of the next 64 planes
+    // have sources in these rows is found once per 64 planes (lane l checks plane k + l, one
+    // ballot), so no per-plane load latency sits in the march.
+    if (a.src.count && k >= sm.k + 64) {
+        sm.k = k;
+        const int64_t kk = k + b.lane;
+        bool has = false;
+        if (kk < b.k1) {
+            const int32_t* rp = a.src.row_ptr + (int64_t)b.shot * (a.src.nrows + 1) + kk * a.nr;
+            has = rp[b.wr0] != rp[b.wr1];
+        }
+        sm.bits = __ballot(has);
+    }
+    const bool src_here = a.src.count && ((sm.bits >> (k - sm.k)) & 1);
+    if (src_here) {
+        for (int rr = 0; rr < 4; rr++) {
+            const int64_t ir = b.i0 + 4 * b.wave + rr;
+            if (ir <= a.nr - 2)
+                wave_inject(a.src, b.shot, k * a.nr + ir, b.j0 - 64 * rr, 16 * rr, 16 * rr + 15, step, b.lane, kc, val);
+        }
+    }
+    if (damp) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) val[e] = val[e] / (T(1) + bb[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+        if (!b.valid[e]) val[e] = T(0);
+}
+
+// Forward launches: each iteration issues the loads of plane k + 1 (its u_prev, kc and sponge
+// value, the z-window chunk of plane k + K + 1 and that plane's in-plane halo chunk), updates plane
+// k from operands that have already landed, then stages plane k + 1's tile and meets the others at
+// a barrier that orders LDS only -- so the look-ahead loads are in flight through the stencil,
+// and no load is outstanding across the loop's back edge but the store.  The halo of plane
+// k + K + 1 is loaded with the neighbouring tiles' loads of that plane's centre chunks (fresh in
+// the L2) and kept in an LDS ring of K + 1 slots until the tile of that plane is staged from it;
+// the ring is private to each thread (a thread reads back only the entries it wrote), and slot
+// (k + K + 1) % (K + 1), written at iteration k, was read (for plane k) at iteration k - 1.
+// Imaging launches (two staged fields, more registers) load the halo of plane k directly and the
+// operands of plane k after the barrier.
 template <typename T, int K, bool VARC, bool SPONGE, bool IMG, bool NT>
 __global__ __launch_bounds__(T3_THREADS) void step3d_kernel(const StepArgs<T> a, int rk, int nkb)
 {
     constexpr bool RING = !IMG;
     __shared__ T tile[IMG ? 2 : 1][2][T3_LH][T3_LW];
-    __shared__ V4<T> ring[RING ? K + 1 : 1][RING ? T3_HALO_THREADS : 1];
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    __shared__ V4<T> ring[RING ? K + 1 : 1][T3_HALO_THREADS];
     const StepBlock bp = step_block(a.ngb, a.gx, a.gy);
     const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
     if (bp.gather) {
         if (threadIdx.x < 256) gather_points(a, bp.g, step);  // 256 points per gather block
         return;
     }
-    const int shot = bp.z / nkb, kb = bp.z % nkb;
-    const int64_t j0 = (int64_t)bp.x * T3_TJ;
-    const int64_t c = j0 + tx * 4;
-    const int64_t i0 = 1 + (int64_t)bp.y * T3_TI;
-    const int64_t i = i0 + ty;
-    const int64_t k0 = 1 + (int64_t)kb * rk;
-    const int64_t k1 = min(k0 + rk, a.np - 1);
-    const bool load_row = i <= a.nr + 3;
-    const bool out = (i <= a.nr - 2) && (c < a.nc);
-
-    const int64_t sb = (int64_t)shot * a.shot_stride;
-    const T* __restrict__ uc = a.uc + sb;
-    const T* up = a.up + sb;
-    T* un = a.un + sb;
-    const T* __restrict__ coef = VARC ? a.coef + (int64_t)shot * a.coef_shot_stride : nullptr;
-    const T* __restrict__ uf = IMG ? a.uf + sb : nullptr;
-    T* gp = IMG ? a.g + sb : nullptr;
-    const int64_t ld = a.ld, ps = a.ps;
-    const int64_t rowoff = i * ld + c;
-    const V4<T> zero = {{T(0), T(0), T(0), T(0)}};
-
-    V4<T> win[2 * K + 1];
-    V4<T> wf[IMG ? 2 * K + 1 : 1];
-#pragma unroll
-    for (int t = 1; t <= 2 * K; t++) {
-        const int64_t k = k0 - K - 1 + t;
-        win[t] = load_row ? gld4(uc + k * ps + rowoff) : zero;
-        if constexpr (IMG) wf[t] = load_row ? gld4(uf + k * ps + rowoff) : zero;
-    }
-
-    bool valid[4];
-    T bc[4];
+    Tile3<T> b;
+    b.tx = threadIdx.x & 15;
+    b.ty = threadIdx.x >> 4;
+    b.lane = threadIdx.x & 63;
+    b.wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    b.shot = bp.z / nkb;
+    const int kb = bp.z % nkb;
+    b.j0 = (int64_t)bp.x * T3_TJ;
+    b.c = b.j0 + b.tx * 4;
+    b.i0 = 1 + (int64_t)bp.y * T3_TI;
+    b.i = b.i0 + b.ty;
+    b.k0 = 1 + (int64_t)kb * rk;
+    b.k1 = min(b.k0 + rk, a.np - 1);
+    b.load_row = b.i <= a.nr + 3;
+    b.out = (b.i <= a.nr - 2) && (b.c < a.nc);
+    b.halo_thread = threadIdx.x < T3_HALO_THREADS;
+    b.ld = a.ld;
+    b.ps = a.ps;
+    b.rowoff = b.i * b.ld + b.c;
+    b.sb = (int64_t)b.shot * a.shot_stride;
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-        valid[e] = out && (c + e >= 1) && (c + e <= a.nc - 2);
-        bc[e] = T(0);
-        if constexpr (SPONGE) bc[e] = (c + e < a.nc) ? a.beta_c[c + e] : T(0);
+        b.valid[e] = b.out && (b.c + e >= 1) && (b.c + e <= a.nc - 2);
+        b.bc[e] = T(0);
+        if constexpr (SPONGE) b.bc[e] = (b.c + e < a.nc) ? a.beta_c[b.c + e] : T(0);
     }
-    T br = T(0);
-    if constexpr (SPONGE) br = (i < a.nr) ? a.beta_r[i] : T(0);
+    b.br = T(0);
+    if constexpr (SPONGE) b.br = (b.i < a.nr) ? a.beta_r[b.i] : T(0);
+    b.wr0 = min(b.i0 + 4 * b.wave, a.nr - 1);
+    b.wr1 = min(b.i0 + 4 * b.wave + 4, a.nr - 1);
 
-    const int64_t wr0 = min(i0 + 4 * wave, a.nr - 1), wr1 = min(i0 + 4 * wave + 4, a.nr - 1);
-    uint64_t src_mask = 0;
-    int64_t mask_k = k0 - 64;
-    int buf = 0;
-    if constexpr (RING) {  // halos of the first K planes
-        if (threadIdx.x < T3_HALO_THREADS)
-            for (int t = 0; t < K; t++)
-                if (k0 + t < k1) ring[(k0 + t) % (K + 1)][threadIdx.x] = halo_fetch(uc, k0 + t, i0, j0, a.nr, ld, ps);
-        __syncthreads();
-    }
-    for (int64_t k = k0; k < k1; k++, buf ^= 1) {
+    const T* __restrict__ uc = a.uc + b.sb;
+    const T* up = a.up + b.sb;
+    T* un = a.un + b.sb;
+    const T* __restrict__ coef = VARC ? a.coef + (int64_t)b.shot * a.coef_shot_stride : nullptr;
+    const int64_t k0 = b.k0, k1 = b.k1, ps = b.ps, rowoff = b.rowoff;
+    const V4<T> zero = {{T(0), T(0), T(0), T(0)}};
+    SrcMask sm{0, k0 - 64};
+
+    V4<T> win[2 * K + 1];
 #pragma unroll
-        for (int t = 0; t < 2 * K; t++) win[t] = win[t + 1];
-        win[2 * K] = load_row ? gld4(uc + (k + K) * ps + rowoff) : zero;
-        if constexpr (IMG) {
-#pragma unroll
-            for (int t = 0; t < 2 * K; t++) wf[t] = wf[t + 1];
-            wf[2 * K] = load_row ? gld4(uf + (k + K) * ps + rowoff) : zero;
+    for (int m = 0; m < 2 * K; m++) win[m] = b.load_row ? gld4(uc + (k0 - K + m) * ps + rowoff) : zero;
+
+    if constexpr (RING) {
+        V4<T> wc = b.load_row ? gld4(uc + (k0 + K) * ps + rowoff) : zero;  // plane k + K
+        V4<T> P = b.out ? gld4s<NT>(up + k0 * ps + rowoff) : zero;
+        V4<T> kv = zero;
+        if constexpr (VARC) kv = b.out ? gld4s<NT>(coef + k0 * ps + rowoff) : zero;
+        T bpk = T(0);
+        if constexpr (SPONGE) bpk = a.beta_p[k0];
+        if (b.halo_thread) {  // halos of planes k0 .. k0 + K
+            for (int m = 0; m <= K; m++)
+                if (k0 + m < k1) ring[(k0 + m) % (K + 1)][threadIdx.x] = halo_fetch(uc, k0 + m, b.i0, b.j0, a.nr, b.ld, ps);
         }
-        if constexpr (RING) {
 #pragma unroll
-            for (int e = 0; e < 4; e++) tile[0][buf][ty + HALO][HALO + tx * 4 + e] = win[K].e[e];
-            if (threadIdx.x < T3_HALO_THREADS) {
-                const V4<T> next = k + K < k1 ? halo_fetch(uc, k + K, i0, j0, a.nr, ld, ps) : zero;
-                halo_put(tile[0][buf], ring[k % (K + 1)][threadIdx.x]);
-                if (k + K < k1) ring[(k + K) % (K + 1)][threadIdx.x] = next;
+        for (int e = 0; e < 4; e++) tile[0][0][b.ty + HALO][HALO + b.tx * 4 + e] = win[K].e[e];
+        if (b.halo_thread) halo_put(tile[0][0], ring[k0 % (K + 1)][threadIdx.x]);
+        __syncthreads();
+        __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+        int buf = 0;
+        for (int64_t k = k0; k < k1; k++, buf ^= 1) {
+            // plane k + 1 <= k1 <= np - 1 and plane k + K + 1 <= np - 1 + K lie in the layout
+            // (K <= 4 zero halo planes), so the last plane's look-ahead loads stay in bounds
+            const V4<T> wn = b.load_row ? gld4(uc + (k + K + 1) * ps + rowoff) : zero;
+            const V4<T> Pn = b.out ? gld4s<NT>(up + (k + 1) * ps + rowoff) : zero;
+            V4<T> kn = zero;
+            if constexpr (VARC) kn = b.out ? gld4s<NT>(coef + (k + 1) * ps + rowoff) : zero;
+            T bpn = T(0);
+            if constexpr (SPONGE) bpn = a.beta_p[k + 1];
+            const V4<T> hn = b.halo_thread && k + K + 1 < k1 ? halo_fetch(uc, k + K + 1, b.i0, b.j0, a.nr, b.ld, ps) : zero;
+            T val[4];
+            plane_update<T, K, VARC, SPONGE, NT>(a, b, win, wc, tile[0][buf], P, kv, bpk, k, step, sm, val);
+            // Take over the look-ahead operands before storing plane k: a wait for a load issued
+            // before an outstanding store also waits for the store.  The explicit wait lets the
+            // compiler see every look-ahead register as landed on every path to the back edge.
+            __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+#pragma unroll
+            for (int m = 0; m < 2 * K - 1; m++) win[m] = win[m + 1];
+            win[2 * K - 1] = wc;
+            wc = wn;
+            P = Pn;
+            kv = kn;
+            bpk = bpn;
+            if (b.halo_thread && k + K + 1 < k1) ring[(k + K + 1) % (K + 1)][threadIdx.x] = hn;
+            if (b.out) gst4s<NT>(un + k * ps + rowoff, val);
+            if (k + 1 < k1) {
+#pragma unroll
+                for (int e = 0; e < 4; e++) tile[0][buf ^ 1][b.ty + HALO][HALO + b.tx * 4 + e] = win[K].e[e];
+                if (b.halo_thread) halo_put(tile[0][buf ^ 1], ring[(k + 1) % (K + 1)][threadIdx.x]);
+                lds_barrier();
             }
-        } else {
-            stage_plane<T>(tile[0][buf], uc, win[K], k, i0, j0, ty, tx, a.nr, ld, ps);
-            if constexpr (IMG) stage_plane<T>(tile[IMG ? 1 : 0][buf], uf, wf[K], k, i0, j0, ty, tx, a.nr, ld, ps);
         }
-        __syncthreads();
-        const int64_t o = k * ps + rowoff;
-        const V4<T> P = out ? gld4s<NT>(up + o) : zero;
-        T kc[4];
-        if constexpr (VARC) {
-            const V4<T> kv = out ? gld4s<NT>(coef + o) : zero;
+    } else {
+        const T* __restrict__ uf = a.uf + b.sb;
+        T* gp = a.g + b.sb;
+        V4<T> wf[2 * K + 1];
 #pragma unroll
-            for (int e = 0; e < 4; e++) kc[e] = kv.e[e];
-        } else {
-#pragma unroll
-            for (int e = 0; e < 4; e++) kc[e] = a.kconst;
-        }
-        T b[4] = {T(0), T(0), T(0), T(0)};
-        bool damp = false;
-        if constexpr (SPONGE) {
-            const T bpr = a.beta_p[k] + br;
+        for (int m = 0; m < 2 * K; m++) wf[m] = b.load_row ? gld4(uf + (k0 - K + m) * ps + rowoff) : zero;
+        int buf = 0;
+        for (int64_t k = k0; k < k1; k++, buf ^= 1) {
+            win[2 * K] = b.load_row ? gld4(uc + (k + K) * ps + rowoff) : zero;
+            wf[2 * K] = b.load_row ? gld4(uf + (k + K) * ps + rowoff) : zero;
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                b[e] = bpr + bc[e];
-                damp |= b[e] != T(0);
+                tile[0][buf][b.ty + HALO][HALO + b.tx * 4 + e] = win[K].e[e];
+                tile[IMG ? 1 : 0][buf][b.ty + HALO][HALO + b.tx * 4 + e] = wf[K].e[e];
             }
-        }
-        T val[4];
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const T ctr = win[K].e[e];
-            const T lap = lap3<T, K>(a.wc, a.wp, a.wr, a.wcol, win, tile[0][buf], ty, tx, e);
-            const T t = damp ? T(2) * ctr - (T(1) - b[e]) * P.e[e] : T(2) * ctr - P.e[e];
-            val[e] = fma3(kc[e], lap, t);
-        }
-        // source pass: the wave holds 4 rows of the tile, 16 lanes each.  Which of the next 64
-        // planes have sources in these rows is found once per 64 planes (lane l checks plane
-        // k + l, one ballot), so no per-plane load latency sits in the loop.
-        if (a.src.count && k >= mask_k + 64) {
-            mask_k = k;
-            const int64_t kk = k + lane;
-            bool has = false;
-            if (kk < k1) {
-                const int32_t* rp = a.src.row_ptr + (int64_t)shot * (a.src.nrows + 1) + kk * a.nr;
-                has = rp[wr0] != rp[wr1];
+            if (b.halo_thread) {
+                halo_put(tile[0][buf], halo_fetch(uc, k, b.i0, b.j0, a.nr, b.ld, ps));
+                halo_put(tile[IMG ? 1 : 0][buf], halo_fetch(uf, k, b.i0, b.j0, a.nr, b.ld, ps));
             }
-            src_mask = __ballot(has);
-        }
-        const bool src_here = a.src.count && ((src_mask >> (k - mask_k)) & 1);
-        if (src_here) {
-            for (int rr = 0; rr < 4; rr++) {
-                const int64_t ir = i0 + 4 * wave + rr;
-                if (ir <= a.nr - 2)
-                    wave_inject(a.src, shot, k * a.nr + ir, j0 - 64 * rr, 16 * rr, 16 * rr + 15, step, lane, kc, val);
-            }
-        }
-        if (damp) {
-#pragma unroll
-            for (int e = 0; e < 4; e++) val[e] = val[e] / (T(1) + b[e]);
-        }
-#pragma unroll
-        for (int e = 0; e < 4; e++)
-            if (!valid[e]) val[e] = T(0);
-        if (out) gst4s<NT>(un + o, val);
-        if constexpr (IMG) {
+            lds_barrier();
+            const int64_t o = k * ps + rowoff;
+            const V4<T> P = b.out ? gld4s<NT>(up + o) : zero;
+            V4<T> kv = zero;
+            if constexpr (VARC) kv = b.out ? gld4s<NT>(coef + o) : zero;
+            T bpk = T(0);
+            if constexpr (SPONGE) bpk = a.beta_p[k];
+            T val[4];
+            plane_update<T, K, VARC, SPONGE, NT>(a, b, win, win[2 * K], tile[0][buf], P, kv, bpk, k, step, sm, val);
+            if (b.out) gst4s<NT>(un + o, val);
             T gv[4];
 #pragma unroll
-            for (int e = 0; e < 4; e++) gv[e] = lap3<T, K>(a.wc, a.wp, a.wr, a.wcol, wf, tile[IMG ? 1 : 0][buf], ty, tx, e);
-            if (out) {
+            for (int e = 0; e < 4; e++)
+                gv[e] = lap3<T, K>(a.wc, a.wp, a.wr, a.wcol, wf, wf[2 * K], tile[IMG ? 1 : 0][buf], b.ty, b.tx, e);
+            if (b.out) {
                 const V4<T> G = gld4(gp + o);
 #pragma unroll
-                for (int e = 0; e < 4; e++) gv[e] = valid[e] ? fma3(val[e], gv[e], G.e[e]) : T(0);
+                for (int e = 0; e < 4; e++) gv[e] = b.valid[e] ? fma3(val[e], gv[e], G.e[e]) : T(0);
                 gst4(gp + o, gv);
+            }
+#pragma unroll
+            for (int m = 0; m < 2 * K; m++) {
+                win[m] = win[m + 1];
+                wf[m] = wf[m + 1];
             }
         }
     }

@@ -201,7 +201,7 @@ def test_every_2d_launch_shape_bitexact(variant, rows):
         assert np.array_equal(tr[0].cpu().numpy(), tr_o)
 
 
-@pytest.mark.parametrize("variant,rows", [(0, 4), (0, 8), (0, 16), (0, 64), (2, 4), (2, 32)])
+@pytest.mark.parametrize("variant,rows", [(0, 1), (0, 3), (0, 4), (0, 8), (0, 16), (0, 64), (2, 4), (2, 7), (2, 32)])
 def test_every_3d_launch_shape_bitexact(variant, rows):
     from neuralmech_amd import solver
 
