@@ -304,7 +304,7 @@ int wfd_batch_run(wfd_batch* batch, int32_t use_graph, wfd_stream_t stream);
  * timed out; the next run then clears the partial sums first). */
 int wfd_batch_status(wfd_batch* batch, wfd_stream_t stream);
 /* Tile shapes per size class (largest grid side <= 768, 1024, 1280, 1536, larger): time `steps`
- * forward steps of each class's shots under every candidate (variant 0/1, 2/4 rows per wave)
+ * forward steps of each class's shots under every candidate (variant 0/1, 2/8 rows per wave)
  * inside the persistent kernel and keep the fastest; host-synchronising.  Replaces the fixed
  * launch shapes of the reference drivers (2D_wave_cuda_padding.py:22,62-63). */
 int wfd_batch_tune(wfd_batch* batch, int32_t steps, wfd_stream_t stream);

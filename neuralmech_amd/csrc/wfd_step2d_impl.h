@@ -202,16 +202,34 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
         if constexpr (SPONGE) bc[e] = (c + e >= 0 && c + e < a.nc) ? a.beta_c[c + e] : T(0);
     }
 
-    // Rows are updated in groups of PD.  The per-row operands of a group (u_prev, kc and the
-    // imaging partial) are all loaded before the group stores anything, so a group waits for one
-    // memory latency instead of one per row, at the cost of registers (worth it where occupancy
-    // cannot hide the latency).  u_next may alias u_prev only element for element, and a group's
-    // loads precede its own stores: the values read are the same as row by row.
+    // Rows are updated in groups of PD.  The per-row operands of a group (u_prev, kc, the sponge
+    // row value and the imaging partial) are all loaded before the group stores anything, so a
+    // group waits for one memory latency instead of one per row, at the cost of registers (worth it
+    // where occupancy cannot hide the latency).  The column-halo chunks of the cache variant are
+    // loaded one row ahead in forward tiles (at the row's start in imaging tiles, which hold twice
+    // the registers).  With PD > 1 each row waits for every outstanding access right before its
+    // stores:
+    // a wait for a load issued before an outstanding store also waits for the store, so waits are
+    // placed where the loads they cover (the next row's halo, issued before this row's stencil) and
+    // the previous row's stores have had a row's arithmetic to complete.  u_next may alias u_prev
+    // only element for element, and a group's loads precede its own stores: the values read are the
+    // same as row by row.
     constexpr int PD = RI < PF ? RI : PF;
+    const T* __restrict__ ufp = IMG ? a.uf + sb : nullptr;
+    auto halo_of = [&](int64_t r, C4<T>& L, C4<T>& R, C4<T>& Lf, C4<T>& Rf) {
+        const int64_t o = r * a.ld + c;
+        L = load_ok ? ld4(uc + o - 4) : zero4<T>();
+        R = load_ok ? ld4(uc + o + 4) : zero4<T>();
+        if constexpr (IMG) {
+            Lf = load_ok ? ld4(ufp + o - 4) : zero4<T>();
+            Rf = load_ok ? ld4(ufp + o + 4) : zero4<T>();
+        }
+    };
 #pragma unroll
     for (int q0 = 0; q0 < RI; q0 += PD) {
         if (r0 + q0 >= a.row_hi) break;  // whole wave
         C4<T> Pv[PD], Kv[VARC ? PD : 1], Gv[IMG ? PD : 1];
+        T Bv[SPONGE ? PD : 1];
 #pragma unroll
         for (int i = 0; i < PD; i++) {
             const int64_t r = r0 + q0 + i;
@@ -220,14 +238,17 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
             Pv[i] = ld ? sld4<NT>(up + o) : zero4<T>();
             if constexpr (VARC) Kv[i] = ld ? sld4<NT>(coef + o) : zero4<T>();
             if constexpr (IMG) Gv[i] = ld ? ld4(a.g + sb + o) : zero4<T>();
+            if constexpr (SPONGE) Bv[i] = (VROW && r < a.row_hi) ? a.beta_r[r] : T(0);
         }
+        C4<T> L, R, Lf, Rf;  // column halo of the current row (cache variant)
+        if constexpr (!XL) halo_of(r0 + q0, L, R, Lf, Rf);
 #pragma unroll
         for (int i = 0; i < PD; i++) {
             const int q = q0 + i;
             const int64_t r = r0 + q;
             if (r >= a.row_hi) break;  // whole wave
             const int64_t o = r * a.ld + c;
-            C4<T> L, R, Lf, Rf;
+            C4<T> Ln, Rn, Lfn, Rfn;  // next row's halo, loaded during this row's stencil
             if constexpr (XL) {
                 L = from_lane_left(win[q + KR]);
                 R = from_lane_right(win[q + KR]);
@@ -235,14 +256,10 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
                     Lf = from_lane_left(wf[q + KR]);
                     Rf = from_lane_right(wf[q + KR]);
                 }
-            } else {
-                L = load_ok ? ld4(uc + o - 4) : zero4<T>();
-                R = load_ok ? ld4(uc + o + 4) : zero4<T>();
-                if constexpr (IMG) {
-                    const T* __restrict__ uf = a.uf + sb;
-                    Lf = load_ok ? ld4(uf + o - 4) : zero4<T>();
-                    Rf = load_ok ? ld4(uf + o + 4) : zero4<T>();
-                }
+            } else if (IMG) {
+                if (i > 0) halo_of(r, L, R, Lf, Rf);
+            } else if (i + 1 < PD && r + 1 < a.row_hi) {
+                halo_of(r + 1, Ln, Rn, Lfn, Rfn);
             }
             const C4<T>& P = Pv[i];
             T kc[4];
@@ -251,10 +268,9 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
             T b[4] = {T(0), T(0), T(0), T(0)};
             bool damp = false;
             if constexpr (SPONGE) {
-                const T br = VROW ? a.beta_r[r] : T(0);
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    b[e] = VROW ? br + bc[e] : bc[e];
+                    b[e] = VROW ? Bv[i] + bc[e] : bc[e];
                     damp |= b[e] != T(0);
                 }
             }
@@ -275,11 +291,14 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
 #pragma unroll
             for (int e = 0; e < 4; e++)
                 if (!valid[e]) val[e] = T(0);
-            if (own) sst4<NT>(un + o, val);
+            T gv[4];
             if constexpr (IMG) {
-                T gv[4];
 #pragma unroll
                 for (int e = 0; e < 4; e++) gv[e] = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, wf, q, Lf, Rf, e);
+            }
+            if constexpr (PD > 1) __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+            if (own) sst4<NT>(un + o, val);
+            if constexpr (IMG) {
                 if (own) {
                     T* gp = a.g + sb;
                     const C4<T>& G = Gv[IMG ? i : 0];
@@ -296,6 +315,10 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
                     st4(gp + o, gv);
                 }
             }
+            if constexpr (!XL && !IMG) {
+                L = Ln;
+                R = Rn;
+            }
         }
     }
 }
@@ -310,6 +333,8 @@ __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
         gather_points(a, bp.g, step);
         return;
     }
+    // row by row: the standalone launches run enough waves to hide the per-row latency, and
+    // loading a wave's operands up front (PF = RI) measured slower for lack of those waves
     step2d_tile<T, K, VROW, VARC, SPONGE, IMG, RI, XL, 1, NT>(a, bp.x, bp.y, (int)bp.z, step);
 }
 

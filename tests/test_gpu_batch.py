@@ -127,14 +127,14 @@ def test_batch_tile_shapes_and_tuner_keep_bits(f32_batch):
     shots, _, ref = f32_batch
     b = ShotBatch(shots, torch.float32)
     for variant in (0, 1):
-        for rows in (2, 4):
+        for rows in (2, 8):
             print("tile shape", variant, rows, flush=True)
             b.set_config(variant, rows)
             b.run(graph=False)
             b.status()
             _assert_same(_outputs(b), ref)
     cfg = b.tune(steps=4)
-    assert all(v in (0, 1) and r in (2, 4) for v, r in cfg)
+    assert all(v in (0, 1) and r in (2, 8) for v, r in cfg)
     b.run(graph=True)
     b.status()
     _assert_same(_outputs(b), ref)
