@@ -140,22 +140,22 @@ __device__ __forceinline__ uint32_t hw_xcc_id() { return __builtin_amdgcn_s_getr
 // Forward / recompute tiles march cfg_rows(cfg) rows per wave (8: fewer re-reads of the window
 // rows and fewer tiles per phase; 3.44 s against 3.53 s with 4 rows for the 64-shot batch);
 // adjoint tiles (two register windows) and every float64 tile march 2, which keeps the kernel at
-// 3 waves per SIMD.  At that occupancy the tiles load their rows' operands up front (PF = rows,
-// in groups of 4 rows for the 8-row tiles).
+// 3 waves per SIMD.  At that occupancy the 8-row tiles load their rows' operands 2 rows ahead;
+// the 2-row tiles row by row (a look-ahead there spills 13 VGPRs).
 template <typename T, int K, bool IMG>
 __device__ __forceinline__ void batch_tile_k(const StepArgs<T>& a, int cfg, uint32_t tx, uint32_t ty, int64_t step)
 {
     if constexpr (sizeof(T) == 8 || IMG) {
         if (cfg_variant(cfg) == 1)
-            step2d_tile<T, K, true, true, true, IMG, 2, true, 2>(a, tx, ty, 0, step);
+            step2d_tile<T, K, true, true, true, IMG, 2, true, 0>(a, tx, ty, 0, step);
         else
-            step2d_tile<T, K, true, true, true, IMG, 2, false, 2>(a, tx, ty, 0, step);
+            step2d_tile<T, K, true, true, true, IMG, 2, false, 0>(a, tx, ty, 0, step);
     } else {
         switch (cfg) {
-            case 0: step2d_tile<T, K, true, true, true, IMG, 8, true, 4>(a, tx, ty, 0, step); break;
-            case 1: step2d_tile<T, K, true, true, true, IMG, 2, true, 2>(a, tx, ty, 0, step); break;
-            case 2: step2d_tile<T, K, true, true, true, IMG, 8, false, 4>(a, tx, ty, 0, step); break;
-            default: step2d_tile<T, K, true, true, true, IMG, 2, false, 2>(a, tx, ty, 0, step); break;
+            case 0: step2d_tile<T, K, true, true, true, IMG, 8, true, 2>(a, tx, ty, 0, step); break;
+            case 1: step2d_tile<T, K, true, true, true, IMG, 2, true, 0>(a, tx, ty, 0, step); break;
+            case 2: step2d_tile<T, K, true, true, true, IMG, 8, false, 2>(a, tx, ty, 0, step); break;
+            default: step2d_tile<T, K, true, true, true, IMG, 2, false, 0>(a, tx, ty, 0, step); break;
         }
     }
 }

@@ -145,10 +145,10 @@ __device__ __forceinline__ T lap_elem(const T wc, const T* wr, const T* wcol, co
 }
 
 // One stencil tile of both variants: column block tx, row block ty of shot `shot`.  XL selects
-// the lane-exchange halo; VROW = false is the 1D case (a single row, no vertical stencil).  PF is
-// the number of rows whose per-row operands are loaded together (see the row loop); NT selects
-// streaming access to u_prev, kc and u_next.
-template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, int PF = 1,
+// the lane-exchange halo; VROW = false is the 1D case (a single row, no vertical stencil).  LA is
+// how many rows ahead the per-row operands are loaded (see the row loop); NT selects streaming
+// access to u_prev, kc and u_next.
+template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, int LA = 0,
           bool NT = false>
 __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, uint32_t ty, int shot, int64_t step)
 {
@@ -202,122 +202,111 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
         if constexpr (SPONGE) bc[e] = (c + e >= 0 && c + e < a.nc) ? a.beta_c[c + e] : T(0);
     }
 
-    // Rows are updated in groups of PD.  The per-row operands of a group (u_prev, kc, the sponge
-    // row value and the imaging partial) are all loaded before the group stores anything, so a
-    // group waits for one memory latency instead of one per row, at the cost of registers (worth it
-    // where occupancy cannot hide the latency).  The column-halo chunks of the cache variant are
-    // loaded one row ahead in forward tiles (at the row's start in imaging tiles, which hold twice
-    // the registers).  With PD > 1 each row waits for every outstanding access right before its
-    // stores:
-    // a wait for a load issued before an outstanding store also waits for the store, so waits are
-    // placed where the loads they cover (the next row's halo, issued before this row's stencil) and
-    // the previous row's stores have had a row's arithmetic to complete.  u_next may alias u_prev
-    // only element for element, and a group's loads precede its own stores: the values read are the
-    // same as row by row.
-    constexpr int PD = RI < PF ? RI : PF;
+    // The per-row operands (u_prev, kc, the sponge row value, the imaging partial and, in the cache
+    // variant, the column-halo chunks) are loaded LA rows ahead of the row being updated.  With
+    // LA > 0 each row waits for everything outstanding right before its stores, so the next rows'
+    // loads are in flight through this row's stencil; a wait placed after a store would also wait
+    // for the store (the compiler cannot wait for one load while a store is outstanding without
+    // waiting for the store).  u_next may alias u_prev only element for element, and a row's
+    // operands are loaded before its own store: the values read are those of a row-by-row update.
     const T* __restrict__ ufp = IMG ? a.uf + sb : nullptr;
-    auto halo_of = [&](int64_t r, C4<T>& L, C4<T>& R, C4<T>& Lf, C4<T>& Rf) {
+    constexpr int NH = XL ? 1 : RI, NHF = (XL || !IMG) ? 1 : RI;
+    C4<T> Pv[RI], Kv[VARC ? RI : 1], Gv[IMG ? RI : 1], Lv[NH], Rv[NH], Lfv[NHF], Rfv[NHF];
+    T Bv[SPONGE ? RI : 1];
+    auto issue = [&](int q) {
+        const int64_t r = r0 + q;
+        if (q >= RI || r >= a.row_hi) return;  // whole wave
+        const bool ld = own && r < a.row_hi;
         const int64_t o = r * a.ld + c;
-        L = load_ok ? ld4(uc + o - 4) : zero4<T>();
-        R = load_ok ? ld4(uc + o + 4) : zero4<T>();
-        if constexpr (IMG) {
-            Lf = load_ok ? ld4(ufp + o - 4) : zero4<T>();
-            Rf = load_ok ? ld4(ufp + o + 4) : zero4<T>();
+        Pv[q] = ld ? sld4<NT>(up + o) : zero4<T>();
+        if constexpr (VARC) Kv[q] = ld ? sld4<NT>(coef + o) : zero4<T>();
+        if constexpr (IMG) Gv[q] = ld ? ld4(a.g + sb + o) : zero4<T>();
+        if constexpr (SPONGE) Bv[q] = VROW ? a.beta_r[r] : T(0);
+        if constexpr (!XL) {
+            Lv[q] = load_ok ? ld4(uc + o - 4) : zero4<T>();
+            Rv[q] = load_ok ? ld4(uc + o + 4) : zero4<T>();
+            if constexpr (IMG) {
+                Lfv[q] = load_ok ? ld4(ufp + o - 4) : zero4<T>();
+                Rfv[q] = load_ok ? ld4(ufp + o + 4) : zero4<T>();
+            }
         }
     };
 #pragma unroll
-    for (int q0 = 0; q0 < RI; q0 += PD) {
-        if (r0 + q0 >= a.row_hi) break;  // whole wave
-        C4<T> Pv[PD], Kv[VARC ? PD : 1], Gv[IMG ? PD : 1];
-        T Bv[SPONGE ? PD : 1];
+    for (int q = 0; q <= LA; q++) issue(q);
 #pragma unroll
-        for (int i = 0; i < PD; i++) {
-            const int64_t r = r0 + q0 + i;
-            const bool ld = own && r < a.row_hi;
-            const int64_t o = r * a.ld + c;
-            Pv[i] = ld ? sld4<NT>(up + o) : zero4<T>();
-            if constexpr (VARC) Kv[i] = ld ? sld4<NT>(coef + o) : zero4<T>();
-            if constexpr (IMG) Gv[i] = ld ? ld4(a.g + sb + o) : zero4<T>();
-            if constexpr (SPONGE) Bv[i] = (VROW && r < a.row_hi) ? a.beta_r[r] : T(0);
+    for (int q = 0; q < RI; q++) {
+        const int64_t r = r0 + q;
+        if (r >= a.row_hi) break;  // whole wave
+        if (q > 0) issue(q + LA);
+        const int64_t o = r * a.ld + c;
+        C4<T> L, R, Lf, Rf;
+        if constexpr (XL) {
+            L = from_lane_left(win[q + KR]);
+            R = from_lane_right(win[q + KR]);
+            if constexpr (IMG) {
+                Lf = from_lane_left(wf[q + KR]);
+                Rf = from_lane_right(wf[q + KR]);
+            }
+        } else {
+            L = Lv[XL ? 0 : q];
+            R = Rv[XL ? 0 : q];
+            if constexpr (IMG) {
+                Lf = Lfv[NHF > 1 ? q : 0];
+                Rf = Rfv[NHF > 1 ? q : 0];
+            }
         }
-        C4<T> L, R, Lf, Rf;  // column halo of the current row (cache variant)
-        if constexpr (!XL) halo_of(r0 + q0, L, R, Lf, Rf);
+        const C4<T>& P = Pv[q];
+        T kc[4];
 #pragma unroll
-        for (int i = 0; i < PD; i++) {
-            const int q = q0 + i;
-            const int64_t r = r0 + q;
-            if (r >= a.row_hi) break;  // whole wave
-            const int64_t o = r * a.ld + c;
-            C4<T> Ln, Rn, Lfn, Rfn;  // next row's halo, loaded during this row's stencil
-            if constexpr (XL) {
-                L = from_lane_left(win[q + KR]);
-                R = from_lane_right(win[q + KR]);
-                if constexpr (IMG) {
-                    Lf = from_lane_left(wf[q + KR]);
-                    Rf = from_lane_right(wf[q + KR]);
-                }
-            } else if (IMG) {
-                if (i > 0) halo_of(r, L, R, Lf, Rf);
-            } else if (i + 1 < PD && r + 1 < a.row_hi) {
-                halo_of(r + 1, Ln, Rn, Lfn, Rfn);
-            }
-            const C4<T>& P = Pv[i];
-            T kc[4];
-#pragma unroll
-            for (int e = 0; e < 4; e++) kc[e] = VARC ? Kv[VARC ? i : 0].e[e] : a.kconst;
-            T b[4] = {T(0), T(0), T(0), T(0)};
-            bool damp = false;
-            if constexpr (SPONGE) {
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    b[e] = VROW ? Bv[i] + bc[e] : bc[e];
-                    damp |= b[e] != T(0);
-                }
-            }
-            T val[4];
+        for (int e = 0; e < 4; e++) kc[e] = VARC ? Kv[VARC ? q : 0].e[e] : a.kconst;
+        T b[4] = {T(0), T(0), T(0), T(0)};
+        bool damp = false;
+        if constexpr (SPONGE) {
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                const T ctr = win[q + KR].e[e];
-                const T lap = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, win, q, L, R, e);
-                const T t = damp ? T(2) * ctr - (T(1) - b[e]) * P.e[e] : T(2) * ctr - P.e[e];
-                val[e] = fmaT(kc[e], lap, t);
+                b[e] = VROW ? Bv[q] + bc[e] : bc[e];
+                damp |= b[e] != T(0);
             }
-            const int64_t prow = VROW ? r : 0;  // CSR row of the point lists
-            if (src_here) wave_inject(a.src, shot, prow, base, lane_lo, lane_hi, step, lane, kc, val);
-            if (damp) {
+        }
+        T val[4];
 #pragma unroll
-                for (int e = 0; e < 4; e++) val[e] = val[e] / (T(1) + b[e]);
-            }
+        for (int e = 0; e < 4; e++) {
+            const T ctr = win[q + KR].e[e];
+            const T lap = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, win, q, L, R, e);
+            const T t = damp ? T(2) * ctr - (T(1) - b[e]) * P.e[e] : T(2) * ctr - P.e[e];
+            val[e] = fmaT(kc[e], lap, t);
+        }
+        const int64_t prow = VROW ? r : 0;  // CSR row of the point lists
+        if (src_here) wave_inject(a.src, shot, prow, base, lane_lo, lane_hi, step, lane, kc, val);
+        if (damp) {
 #pragma unroll
-            for (int e = 0; e < 4; e++)
-                if (!valid[e]) val[e] = T(0);
-            T gv[4];
-            if constexpr (IMG) {
+            for (int e = 0; e < 4; e++) val[e] = val[e] / (T(1) + b[e]);
+        }
 #pragma unroll
-                for (int e = 0; e < 4; e++) gv[e] = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, wf, q, Lf, Rf, e);
-            }
-            if constexpr (PD > 1) __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
-            if (own) sst4<NT>(un + o, val);
-            if constexpr (IMG) {
-                if (own) {
-                    T* gp = a.g + sb;
-                    const C4<T>& G = Gv[IMG ? i : 0];
+        for (int e = 0; e < 4; e++)
+            if (!valid[e]) val[e] = T(0);
+        T gv[4];
+        if constexpr (IMG) {
 #pragma unroll
-                    for (int e = 0; e < 4; e++) gv[e] = valid[e] ? fmaT(val[e], gv[e], G.e[e]) : T(0);
-                    if (a.gsum) {  // fused flush: gsum += partial, partial = 0 (same bits as the flush kernel)
-                        double* gs = a.gsum + sb + o;
+            for (int e = 0; e < 4; e++) gv[e] = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, wf, q, Lf, Rf, e);
+        }
+        if constexpr (LA > 0) __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+        if (own) sst4<NT>(un + o, val);
+        if constexpr (IMG) {
+            if (own) {
+                T* gp = a.g + sb;
+                const C4<T>& G = Gv[IMG ? q : 0];
 #pragma unroll
-                        for (int e = 0; e < 4; e++) {
-                            if (valid[e]) gs[e] = (a.gsum_first ? 0.0 : gs[e]) + (double)gv[e];
-                            gv[e] = T(0);
-                        }
+                for (int e = 0; e < 4; e++) gv[e] = valid[e] ? fmaT(val[e], gv[e], G.e[e]) : T(0);
+                if (a.gsum) {  // fused flush: gsum += partial, partial = 0 (same bits as the flush kernel)
+                    double* gs = a.gsum + sb + o;
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        if (valid[e]) gs[e] = (a.gsum_first ? 0.0 : gs[e]) + (double)gv[e];
+                        gv[e] = T(0);
                     }
-                    st4(gp + o, gv);
                 }
-            }
-            if constexpr (!XL && !IMG) {
-                L = Ln;
-                R = Rn;
+                st4(gp + o, gv);
             }
         }
     }
@@ -333,9 +322,10 @@ __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
         gather_points(a, bp.g, step);
         return;
     }
-    // row by row: the standalone launches run enough waves to hide the per-row latency, and
-    // loading a wave's operands up front (PF = RI) measured slower for lack of those waves
-    step2d_tile<T, K, VROW, VARC, SPONGE, IMG, RI, XL, 1, NT>(a, bp.x, bp.y, (int)bp.z, step);
+    // forward tiles one row ahead; imaging tiles (two windows) row by row: a look-ahead costs
+    // them a wave per SIMD.  Loading a wave's operands up front (LA = RI - 1) measured slower for
+    // lack of waves.
+    step2d_tile<T, K, VROW, VARC, SPONGE, IMG, RI, XL, (RI > 1 && !IMG ? 1 : 0), NT>(a, bp.x, bp.y, (int)bp.z, step);
 }
 
 template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool NT>
