@@ -217,8 +217,31 @@ __device__ __forceinline__ void t2l_tile(const StepArgs<T>& a, uint32_t tx, uint
     auto clamp_row = [&](int64_t r) { return r < 0 ? (int64_t)0 : (r > nr ? nr : r); };
     auto node_row = [&](int64_t r) { return r >= 0 && r < nr && load_ok; };
 
+    // Both steps load each row's operands one row ahead (step 1: the window's next row of u^n,
+    // the column-halo chunks, u^{n-1} and kc; step 2: u^n and kc) and wait for everything
+    // outstanding right before a row's stores: the next row's loads are in flight through this
+    // row's stencil, and a wait placed after a store would also wait for the store.  The first
+    // step-2 rows' operands are issued before the barrier, which orders LDS only.
+
     // ---- step 1: rows R0 - K + q, q = q0 .. q1 - 1 of the band
     const int q0 = wave * S, q1 = min(q0 + S, NR1);
+    struct Row1 {
+        C4<T> nb, L, R, P, kv;
+        T br;
+    };
+    auto issue1 = [&](Row1& o, int q) {
+        const int64_t r1 = R0 - K + q;
+        o.nb = node_row(r1 + K) ? ld4(B + (r1 + K) * ld + c) : zero4<T>();
+        const bool row1 = r1 >= 1 && r1 <= nr - 2;
+        const int64_t off = r1 * ld + c;
+        o.L = row1 && load_ok ? ld4(B + off - 4) : zero4<T>();
+        o.R = row1 && load_ok ? ld4(B + off + 4) : zero4<T>();
+        o.P = row1 && load_ok ? ld4(A + off) : zero4<T>();
+        o.kv = zero4<T>();
+        if constexpr (VARC) o.kv = row1 && load_ok ? ld4(coef + off) : zero4<T>();
+        o.br = T(0);
+        if constexpr (SPONGE) o.br = row1 ? a.beta_r[r1] : T(0);
+    };
     if (q0 < q1) {
         const int64_t rf = R0 - K + q0;  // first step-1 row of this wave
         const bool src1 = rp && rp[clamp_row(rf)] != rp[clamp_row(R0 - K + q1)];
@@ -228,38 +251,40 @@ __device__ __forceinline__ void t2l_tile(const StepArgs<T>& a, uint32_t tx, uint
             const int64_t r = rf - K + (t - 1);
             win[t] = node_row(r) ? ld4(B + r * ld + c) : zero4<T>();
         }
-        for (int q = q0; q < q1; q++) {
-            const int64_t r1 = R0 - K + q;
+        // rolled (the source pass would be replicated per row otherwise); the look-ahead row is
+        // taken over (window shifted, operands moved) after the row's wait and before its store,
+        // so the back edge carries no outstanding load
+        Row1 cur, nxt;
+        issue1(cur, q0);
+        __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
 #pragma unroll
-            for (int t = 0; t + 1 < W; t++) win[t] = win[t + 1];
-            win[W - 1] = node_row(r1 + K) ? ld4(B + (r1 + K) * ld + c) : zero4<T>();
+        for (int t = 0; t + 1 < W; t++) win[t] = win[t + 1];
+        win[W - 1] = cur.nb;
+#pragma unroll 1
+        for (int q = q0; q < q1; q++) {
+            if (q + 1 < q1) issue1(nxt, q + 1);
+            const int64_t r1 = R0 - K + q;
             const bool row1 = r1 >= 1 && r1 <= nr - 2;
             C4<T> out = zero4<T>();
             if (row1) {
-                const int64_t o = r1 * ld + c;
-                const C4<T> L = load_ok ? ld4(B + o - 4) : zero4<T>();
-                const C4<T> R = load_ok ? ld4(B + o + 4) : zero4<T>();
-                const C4<T> P = load_ok ? ld4(A + o) : zero4<T>();
-                C4<T> kv = zero4<T>();
-                if constexpr (VARC) kv = load_ok ? ld4(coef + o) : zero4<T>();
                 T kc[4];
 #pragma unroll
-                for (int e = 0; e < 4; e++) kc[e] = VARC ? kv.e[e] : a.kconst;
+                for (int e = 0; e < 4; e++) kc[e] = VARC ? cur.kv.e[e] : a.kconst;
                 T b[4] = {T(0), T(0), T(0), T(0)};
                 bool damp = false;
                 if constexpr (SPONGE) {
-                    const T br = a.beta_r[r1];
 #pragma unroll
                     for (int e = 0; e < 4; e++) {
-                        b[e] = br + bc[e];
+                        b[e] = cur.br + bc[e];
                         damp |= b[e] != T(0);
                     }
                 }
                 T val[4];
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    const T lap = lap_elem<T, K, true>(a.wc, a.wr, a.wcol, win, 0, L, R, e);
-                    const T t = damp ? T(2) * win[K].e[e] - (T(1) - b[e]) * P.e[e] : T(2) * win[K].e[e] - P.e[e];
+                    const T lap = lap_elem<T, K, true>(a.wc, a.wr, a.wcol, win, 0, cur.L, cur.R, e);
+                    const T t = damp ? T(2) * win[K].e[e] - (T(1) - b[e]) * cur.P.e[e]
+                                     : T(2) * win[K].e[e] - cur.P.e[e];
                     val[e] = fmaT(kc[e], lap, t);
                 }
                 if (src1) wave_inject(a.src, shot, r1, base, 0, 63, step, lane, kc, val);
@@ -268,18 +293,37 @@ __device__ __forceinline__ void t2l_tile(const StepArgs<T>& a, uint32_t tx, uint
                     const T x = damp ? val[e] / (T(1) + b[e]) : val[e];
                     out.e[e] = valid[e] ? x : T(0);
                 }
-                if (own && r1 >= R0 && r1 < R0 + 4 * RI && r1 < a.row_hi) {
-                    const T v[4] = {out.e[0], out.e[1], out.e[2], out.e[3]};
-                    st4(C + o, v);
-                }
             }
             band[q][lane] = out;
+            __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+#pragma unroll
+            for (int t = 0; t + 1 < W; t++) win[t] = win[t + 1];
+            win[W - 1] = nxt.nb;
+            cur = nxt;
+            if (row1 && own && r1 >= R0 && r1 < R0 + 4 * RI && r1 < a.row_hi) {
+                const T v[4] = {out.e[0], out.e[1], out.e[2], out.e[3]};
+                st4(C + r1 * ld + c, v);
+            }
         }
     }
-    __syncthreads();
 
     // ---- step 2: rows R0 + wave RI .. + RI - 1, band rows q = r2 - (R0 - K)
     const int64_t r2f = R0 + (int64_t)wave * RI;
+    struct Row2 {
+        C4<T> P, kv;
+        T br;
+    };
+    auto issue2 = [&](Row2& o, int64_t r2) {
+        const int64_t off = r2 * ld + c;
+        o.P = load_ok ? ld4(B + off) : zero4<T>();  // u_prev of step 2 is u^n
+        o.kv = zero4<T>();
+        if constexpr (VARC) o.kv = load_ok ? ld4(coef + off) : zero4<T>();
+        o.br = T(0);
+        if constexpr (SPONGE) o.br = a.beta_r[r2];
+    };
+    Row2 op2[RI];
+    if (r2f < a.row_hi) issue2(op2[0], r2f);
+    lds_barrier();
     if (r2f >= a.row_hi) return;  // whole wave; no barrier follows
     const int qf = wave * RI;  // band row of r2f - K
     const int64_t r2l = r2f + RI < a.row_hi ? r2f + RI : a.row_hi;
@@ -291,24 +335,21 @@ __device__ __forceinline__ void t2l_tile(const StepArgs<T>& a, uint32_t tx, uint
     for (int i = 0; i < RI; i++) {
         const int64_t r2 = r2f + i;
         if (r2 >= a.row_hi) break;  // whole wave
+        if (i + 1 < RI && r2 + 1 < a.row_hi) issue2(op2[i + 1], r2 + 1);
 #pragma unroll
         for (int t = 0; t + 1 < W; t++) sw[t] = sw[t + 1];
         sw[W - 1] = band[qf + i + 2 * K][lane];
         const int64_t o = r2 * ld + c;
         const C4<T> L2 = from_lane_left(sw[K]), R2 = from_lane_right(sw[K]);
-        const C4<T> P2 = load_ok ? ld4(B + o) : zero4<T>();  // u_prev of step 2 is u^n
-        C4<T> kv2 = zero4<T>();
-        if constexpr (VARC) kv2 = load_ok ? ld4(coef + o) : zero4<T>();
         T kc2[4];
 #pragma unroll
-        for (int e = 0; e < 4; e++) kc2[e] = VARC ? kv2.e[e] : a.kconst;
+        for (int e = 0; e < 4; e++) kc2[e] = VARC ? op2[i].kv.e[e] : a.kconst;
         T b2[4] = {T(0), T(0), T(0), T(0)};
         bool damp2 = false;
         if constexpr (SPONGE) {
-            const T br = a.beta_r[r2];
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                b2[e] = br + bc[e];
+                b2[e] = op2[i].br + bc[e];
                 damp2 |= b2[e] != T(0);
             }
         }
@@ -316,7 +357,7 @@ __device__ __forceinline__ void t2l_tile(const StepArgs<T>& a, uint32_t tx, uint
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const T lap = lap_elem<T, K, true>(a.wc, a.wr, a.wcol, sw, 0, L2, R2, e);
-            const T t = damp2 ? T(2) * sw[K].e[e] - (T(1) - b2[e]) * P2.e[e] : T(2) * sw[K].e[e] - P2.e[e];
+            const T t = damp2 ? T(2) * sw[K].e[e] - (T(1) - b2[e]) * op2[i].P.e[e] : T(2) * sw[K].e[e] - op2[i].P.e[e];
             val2[e] = fmaT(kc2[e], lap, t);
         }
         if (src2) wave_inject(a.src, shot, r2, base, 1, 62, step + 1, lane, kc2, val2);
@@ -325,6 +366,7 @@ __device__ __forceinline__ void t2l_tile(const StepArgs<T>& a, uint32_t tx, uint
             const T x = damp2 ? val2[e] / (T(1) + b2[e]) : val2[e];
             val2[e] = valid[e] ? x : T(0);
         }
+        __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
         if (own) st4(D + o, val2);
     }
 }
