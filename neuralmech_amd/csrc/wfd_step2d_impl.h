@@ -322,10 +322,11 @@ __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
         gather_points(a, bp.g, step);
         return;
     }
-    // forward tiles one row ahead; imaging tiles (two windows) row by row: a look-ahead costs
-    // them a wave per SIMD.  Loading a wave's operands up front (LA = RI - 1) measured slower for
-    // lack of waves.
-    step2d_tile<T, K, VROW, VARC, SPONGE, IMG, RI, XL, (RI > 1 && !IMG ? 1 : 0), NT>(a, bp.x, bp.y, (int)bp.z, step);
+    // one row ahead, except imaging tiles of 4 rows (two windows), where the look-ahead's
+    // registers cost a wave per SIMD (2 and 8 rows keep their 3 and 2 waves).  Loading a wave's
+    // operands up front (LA = RI - 1) measured slower for lack of waves.
+    constexpr int LA = RI > 1 && !(IMG && RI == 4) ? 1 : 0;
+    step2d_tile<T, K, VROW, VARC, SPONGE, IMG, RI, XL, LA, NT>(a, bp.x, bp.y, (int)bp.z, step);
 }
 
 template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool NT>
