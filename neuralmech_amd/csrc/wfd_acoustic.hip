@@ -132,30 +132,15 @@ __device__ __forceinline__ T ac_inject(const PtsDev<T>& p, int64_t shot, int64_t
     return num;
 }
 
+// part[o] = v (the loaded partial plus the row's term), or with flush: sum[o] += v, part[o] = 0.
 template <typename T>
-__device__ __forceinline__ void acc(T* part, double* sum, int64_t o, T add, bool flush)
+__device__ __forceinline__ void acc_store(T* part, double* sum, int64_t o, T v, bool flush)
 {
-    const T v = part[o] + add;
     if (flush) {
         sum[o] += (double)v;
         part[o] = T(0);
     } else {
         part[o] = v;
-    }
-}
-
-// Column neighbours at distance 1..K of this lane's node in row offset `o`: from lanes l -+ m,
-// loaded directly by the lanes whose neighbour lies in another wave's columns.  Every lane of
-// the wave takes part in the shuffles.
-template <typename T, int K>
-__device__ __forceinline__ void ac_cols(const T* f, int64_t o, int lane, bool in, T c, T (&w)[K], T (&e)[K])
-{
-#pragma unroll
-    for (int m = 1; m <= K; m++) {
-        w[m - 1] = __shfl_up(c, m);
-        e[m - 1] = __shfl_down(c, m);
-        if (lane < m && in) w[m - 1] = f[o - m];
-        if (lane >= 64 - m && in) e[m - 1] = f[o + m];
     }
 }
 
@@ -174,8 +159,45 @@ __device__ __forceinline__ T ac_q(const T (&cn)[K], const T (&cs)[K], const T (&
     return qx + qy;
 }
 
+// Column neighbours at distance 1..K of this lane's node (value c): from lanes l -+ m, or the
+// loaded values w0 / e0 for the lanes whose neighbour lies in another wave's columns.  Every lane
+// of the wave takes part in the shuffles.
+template <typename T, int K>
+__device__ __forceinline__ void ac_cols(int lane, bool in, T c, const T (&w0)[K], const T (&e0)[K], T (&w)[K],
+                                        T (&e)[K])
+{
+#pragma unroll
+    for (int m = 1; m <= K; m++) {
+        w[m - 1] = __shfl_up(c, m);
+        e[m - 1] = __shfl_down(c, m);
+        if (lane < m && in) w[m - 1] = w0[m - 1];
+        if (lane >= 64 - m && in) e[m - 1] = e0[m - 1];
+    }
+}
+
+// Everything a row of the march loads, issued one row ahead.
+template <typename T, int K>
+struct AcRow {
+    T p, m;               // window rows r + K of p and mu
+    T cn[K];              // cn_m at row r + m
+    T cw[K], ce[K];       // cw_m at (r, j), and at (r, j + m) for the last m lanes
+    T pw[K], pe[K];       // p at (r, j - m) for the first m lanes, at (r, j + m) for the last m
+    T mw[K], me[K];       // mu likewise (backward)
+    T k, prev, upv;       // k; p^{n-1} (forward) or mu^{n+2} (backward); p^{n+1} (reconstruction)
+    T g0, gn[K], gw[K];   // imaging partials at the node (mass, north and west bonds)
+    T gnf[K], gwf[K];     // imaging partials of the bonds stored at their far end
+};
+
 // Launch: 8 * ceil(aux / 8) aux blocks then gx * gy * nshots tiles (see step_block); a tile is 64
 // columns x 4 waves x RI rows.  K = order / 2 bonds per side and axis.
+//
+// A wave marches its RI rows with rolling register windows and loads each row's operands (AcRow)
+// before any of the row's stores -- one row ahead where that pays (see LA) -- then waits for
+// everything outstanding right before the row's stores: the loads are in flight through the
+// row's arithmetic (the next row's, with the look-ahead), and a wait placed after a store would
+// also wait for the store.  Rows with sources are found once per wave.  Stores and in-place
+// partners (u_next / u_prev, mu_n / mu_prev) touch each element from one thread only, and every
+// imaging partial (node or far end of a bond) is read and written by one thread, at one row.
 template <typename T, int K, bool SPONGE, bool BWD, int RI>
 __global__ __launch_bounds__(256) void ac_kernel(const AcArgs<T> a)
 {
@@ -196,12 +218,66 @@ __global__ __launch_bounds__(256) void ac_kernel(const AcArgs<T> a)
     const int64_t j0 = (int64_t)bp.x * 64, j = j0 + lane;
     const int64_t r0 = 1 + ((int64_t)bp.y * 4 + wave) * RI;
     if (r0 > n0 - 2) return;  // whole wave
+    const int64_t rl = min(r0 + RI - 1, n0 - 2);  // last row of the wave
     const bool in = j < n1;
     const bool inter = j >= 1 && j <= n1 - 2;
     const int64_t so = shot * a.shot_stride;
     const T* uc = a.uc + so;
     const T* mc = BWD ? a.mc + so : nullptr;
     const T bcj = SPONGE && in ? a.bc[j] : T(0);
+    // sponge rows r0 - K .. rl and columns j - 1 .. j - K (bonds to damped neighbours)
+    T brw[RI + K], bcw[K];
+#pragma unroll
+    for (int t = 0; t < RI + K; t++) brw[t] = SPONGE && r0 - K + t >= 0 && r0 - K + t <= rl ? a.br[r0 - K + t] : T(0);
+#pragma unroll
+    for (int m = 1; m <= K; m++) bcw[m - 1] = SPONGE && BWD && j - m >= 0 && in ? a.bc[j - m] : T(0);
+    // rows of this wave with points of the source / adjoint lists (one check per wave)
+    auto has_rows = [&](const PtsDev<T>& p) {
+        if (p.count == 0 || a.step < 0 || a.step >= p.nt) return false;
+        const int32_t* rp = p.row_ptr + shot * (p.nrows + 1);
+        return rp[r0] != rp[rl + 1];
+    };
+    const bool src_w = has_rows(a.src);
+    const bool adj_w = BWD && has_rows(a.adj);
+    const bool fl = BWD && a.flush != 0;
+
+    auto load_row = [&](AcRow<T, K>& R, int s) {
+        const int64_t r = r0 + s;
+        const int64_t o = r * ld + j;
+        R.p = ld0(uc, o + K * ld, in);
+        R.m = BWD ? ld0(mc, o + K * ld, in) : T(0);
+#pragma unroll
+        for (int m = 1; m <= K; m++) {
+            const T* cwm = a.cw + (m - 1) * cst;
+            R.cn[m - 1] = ld0(a.cn + (m - 1) * cst, o + m * ld, in);
+            // bonds of the last interior columns reach up to K-1 columns past the nodes (< ld)
+            R.cw[m - 1] = ld0(cwm, o, j < n1 + K - 1);
+            R.ce[m - 1] = lane >= 64 - m && in ? cwm[o + m] : T(0);
+            R.pw[m - 1] = lane < m && in ? uc[o - m] : T(0);
+            R.pe[m - 1] = lane >= 64 - m && in ? uc[o + m] : T(0);
+            R.mw[m - 1] = BWD && lane < m && in ? mc[o - m] : T(0);
+            R.me[m - 1] = BWD && lane >= 64 - m && in ? mc[o + m] : T(0);
+        }
+        R.k = ld0(a.k, o, in);
+        R.prev = BWD ? ld0(a.mp + so, o, in) : ld0(a.up + so, o, in);
+        if (BWD && a.recon) {
+            const bool und = !SPONGE || brw[s + K] + bcj == T(0);
+            const bool ui = inter && und;
+            R.upv = ui ? a.up[so + o] : T(0);
+            R.g0 = ui ? a.gm[so + o] : T(0);
+#pragma unroll
+            for (int m = 1; m <= K; m++) {
+                const T* fn = a.gm + m * pst + so;
+                const T* fw = a.gm + (K + m) * pst + so;
+                const bool nund = r - m >= 1 && (!SPONGE || brw[s + K - m] + bcj == T(0));
+                const bool wund = j - m >= 1 && (!SPONGE || brw[s + K] + bcw[m - 1] == T(0));
+                R.gn[m - 1] = inter && (und || nund) ? fn[o] : T(0);
+                R.gw[m - 1] = inter && (und || wund) ? fw[o] : T(0);
+                R.gnf[m - 1] = ui && r + m >= n0 - 1 ? fn[o + m * ld] : T(0);
+                R.gwf[m - 1] = ui && j + m >= n1 - 1 ? fw[o + m] : T(0);
+            }
+        }
+    };
 
     // rolling windows: rows r-K .. r+K of p (and mu), bond coefficients cn_m of rows r .. r+m
     T P[2 * K + 1], M[2 * K + 1], CN[K][K + 1];
@@ -214,25 +290,33 @@ __global__ __launch_bounds__(256) void ac_kernel(const AcArgs<T> a)
     for (int m = 1; m <= K; m++)
 #pragma unroll
         for (int t = 0; t < m; t++) CN[m - 1][t] = ld0(a.cn + (m - 1) * cst, (r0 + t) * ld + j, in);
+    // One row ahead: adjoint rows at orders 2 and 4 (higher orders: two rows' worth of operands
+    // would cost waves), forward rows of 2-row waves (small grids, latency-bound: 8 x 512^2 order 4
+    // plate iteration 122.5 -> 110.5 ms; 4-row waves of the 3300 x 1800 grid ran 3 % slower
+    // ahead).  The others load a row's operands at its start.
+    constexpr int LA = (BWD ? K <= 2 : RI <= 2) ? 1 : 0;
+    AcRow<T, K> rows[RI];
+    if (LA) load_row(rows[0], 0);
 #pragma unroll
     for (int s = 0; s < RI; s++) {
         const int64_t r = r0 + s;
-        if (r > n0 - 2) break;  // whole wave
+        if (r > rl) break;  // whole wave
+        if (!LA) load_row(rows[s], s);
+        else if (s + 1 < RI && r + 1 <= rl) load_row(rows[s + 1], s + 1);
+        const AcRow<T, K>& R = rows[s];
         const int64_t o = r * ld + j;
-        P[2 * K] = ld0(uc, o + K * ld, in);
-        if (BWD) M[2 * K] = ld0(mc, o + K * ld, in);
+        P[2 * K] = R.p;
+        if (BWD) M[2 * K] = R.m;
 #pragma unroll
-        for (int m = 1; m <= K; m++) CN[m - 1][m] = ld0(a.cn + (m - 1) * cst, o + m * ld, in);
+        for (int m = 1; m <= K; m++) CN[m - 1][m] = R.cn[m - 1];
         const T pc = P[K];
         T pw[K], pe[K], cwv[K], cev[K], cnv[K], csv[K], dN[K], dS[K], dW[K], dE[K];
-        ac_cols<T, K>(uc, o, lane, in, pc, pw, pe);
+        ac_cols<T, K>(lane, in, pc, R.pw, R.pe, pw, pe);
 #pragma unroll
         for (int m = 1; m <= K; m++) {
-            const T* cwm = a.cw + (m - 1) * cst;
-            // bonds of the last interior columns reach up to K-1 columns past the nodes (< ld)
-            cwv[m - 1] = ld0(cwm, o, j < n1 + K - 1);
+            cwv[m - 1] = R.cw[m - 1];
             cev[m - 1] = __shfl_down(cwv[m - 1], m);
-            if (lane >= 64 - m && in) cev[m - 1] = cwm[o + m];
+            if (lane >= 64 - m && in) cev[m - 1] = R.ce[m - 1];
             cnv[m - 1] = CN[m - 1][0];
             csv[m - 1] = CN[m - 1][m];
             dN[m - 1] = P[K - m] - pc;
@@ -240,19 +324,20 @@ __global__ __launch_bounds__(256) void ac_kernel(const AcArgs<T> a)
             dW[m - 1] = pw[m - 1] - pc;
             dE[m - 1] = pe[m - 1] - pc;
         }
-        const T kx = ld0(a.k, o, in);
+        const T kx = R.k;
         const T q = ac_q<T, K>(cnv, csv, cwv, cev, dN, dS, dW, dE);
-        const T b = SPONGE ? a.br[r] + bcj : T(0);
+        const T b = SPONGE ? brw[s + K] + bcj : T(0);
         if (!BWD) {
-            const T pp = ld0(a.up + so, o, in);
-            const T t = SPONGE ? T(2) * pc - (T(1) - b) * pp : T(2) * pc - pp;
+            const T t = SPONGE ? T(2) * pc - (T(1) - b) * R.prev : T(2) * pc - R.prev;
             T num = t + kx * q;
-            num = ac_inject(a.src, shot, r, j0, lane, a.step, kx, num);
-            if (inter) a.un[so + o] = SPONGE ? num / (T(1) + b) : num;
+            if (src_w) num = ac_inject(a.src, shot, r, j0, lane, a.step, kx, num);
+            const T outv = SPONGE ? num / (T(1) + b) : num;
+            __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+            if (inter) a.un[so + o] = outv;
         } else {
             const T mcv = M[K];
             T mw[K], me[K], eN[K], eS[K], eW[K], eE[K];
-            ac_cols<T, K>(mc, o, lane, in, mcv, mw, me);
+            ac_cols<T, K>(lane, in, mcv, R.mw, R.me, mw, me);
 #pragma unroll
             for (int m = 1; m <= K; m++) {
                 eN[m - 1] = M[K - m] - mcv;
@@ -261,19 +346,31 @@ __global__ __launch_bounds__(256) void ac_kernel(const AcArgs<T> a)
                 eE[m - 1] = me[m - 1] - mcv;
             }
             const T qm = ac_q<T, K>(cnv, csv, cwv, cev, eN, eS, eW, eE);
-            const T mpv = ld0(a.mp + so, o, in);
-            const T tm = SPONGE ? T(2) * mcv - (T(1) - b) * mpv : T(2) * mcv - mpv;
+            const T tm = SPONGE ? T(2) * mcv - (T(1) - b) * R.prev : T(2) * mcv - R.prev;
             T numm = tm + kx * qm;
-            numm = ac_inject(a.adj, shot, r, j0, lane, a.step, kx, numm);
-            if (inter) a.mn[so + o] = SPONGE ? numm / (T(1) + b) : numm;
+            if (adj_w) numm = ac_inject(a.adj, shot, r, j0, lane, a.step, kx, numm);
+            const T mnv = SPONGE ? numm / (T(1) + b) : numm;
+            const bool und = !SPONGE || b == T(0);
+            T unv = T(0), g0 = T(0), gn[K], gw[K], gnf[K], gwf[K];
             if (a.recon) {
-                const bool und = !SPONGE || b == T(0);
                 T rv = T(2) * pc + kx * q;
-                rv = ac_inject(a.src, shot, r, j0, lane, a.step, kx, rv);
-                const bool fl = a.flush != 0;
+                if (src_w) rv = ac_inject(a.src, shot, r, j0, lane, a.step, kx, rv);
+                unv = rv - R.upv;
+                g0 = R.g0 + mcv * q;
+#pragma unroll
+                for (int m = 1; m <= K; m++) {
+                    gn[m - 1] = R.gn[m - 1] + dN[m - 1] * eN[m - 1];
+                    gw[m - 1] = R.gw[m - 1] + dW[m - 1] * eW[m - 1];
+                    gnf[m - 1] = R.gnf[m - 1] + dS[m - 1] * eS[m - 1];
+                    gwf[m - 1] = R.gwf[m - 1] + dE[m - 1] * eE[m - 1];
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+            if (inter) a.mn[so + o] = mnv;
+            if (a.recon) {
                 if (inter && und) {
-                    a.un[so + o] = rv - a.up[so + o];
-                    acc(a.gm + so, a.Gm + so, o, mcv * q, fl);
+                    a.un[so + o] = unv;
+                    acc_store(a.gm + so, a.Gm + so, o, g0, fl);
                 }
                 if (inter) {
 #pragma unroll
@@ -283,13 +380,13 @@ __global__ __launch_bounds__(256) void ac_kernel(const AcArgs<T> a)
                         T* fw = a.gm + (K + m) * pst + so;
                         double* Fw = a.Gm + (K + m) * pst + so;
                         // a bond is imaged when either end is an undamped interior node
-                        const bool nund = r - m >= 1 && (!SPONGE || a.br[r - m] + bcj == T(0));
-                        if (und || nund) acc(fn, Fn, o, dN[m - 1] * eN[m - 1], fl);
-                        const bool wund = j - m >= 1 && (!SPONGE || a.br[r] + a.bc[j - m] == T(0));
-                        if (und || wund) acc(fw, Fw, o, dW[m - 1] * eW[m - 1], fl);
+                        const bool nund = r - m >= 1 && (!SPONGE || brw[s + K - m] + bcj == T(0));
+                        if (und || nund) acc_store(fn, Fn, o, gn[m - 1], fl);
+                        const bool wund = j - m >= 1 && (!SPONGE || brw[s + K] + bcw[m - 1] == T(0));
+                        if (und || wund) acc_store(fw, Fw, o, gw[m - 1], fl);
                         // bonds reaching past the last interior row / column: stored at their far end
-                        if (und && r + m >= n0 - 1) acc(fn, Fn, o + m * ld, dS[m - 1] * eS[m - 1], fl);
-                        if (und && j + m >= n1 - 1) acc(fw, Fw, o + m, dE[m - 1] * eE[m - 1], fl);
+                        if (und && r + m >= n0 - 1) acc_store(fn, Fn, o + m * ld, gnf[m - 1], fl);
+                        if (und && j + m >= n1 - 1) acc_store(fw, Fw, o + m, gwf[m - 1], fl);
                     }
                 }
             }
@@ -323,6 +420,9 @@ hipError_t ac_launch_k(AcArgs<T> a, bool tiles_on, hipStream_t s)
     const uint64_t target = !BWD ? 16384u : (bwd_bytes <= ((uint64_t)192 << 20) ? 32768u : 8192u);
     int ri = 8;
     while (ri > 1 && waves(ri) < target) ri /= 2;
+    // orders 6 and 8: an adjoint row's operands take 40-54 registers; 2 rows per wave keep 4 waves
+    // per SIMD (113-118 VGPRs), 4 or 8 rows fall to 2-3
+    if (BWD && K >= 3 && ri > 2) ri = 2;
     const int forced = g_ac_rows.load(std::memory_order_relaxed);
     if (forced) ri = forced;
     a.gy = (uint32_t)((a.n0 - 2 + 4 * ri - 1) / (4 * ri));
