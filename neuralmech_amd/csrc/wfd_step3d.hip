@@ -233,7 +233,7 @@ __device__ __forceinline__ void plane_update(const StepArgs<T>& a, const Tile3<T
 // Imaging launches (two staged fields, more registers) load the halo of plane k directly and the
 // operands of plane k after the barrier.
 template <typename T, int K, bool VARC, bool SPONGE, bool IMG, bool NT>
-__global__ __launch_bounds__(T3_THREADS) void step3d_kernel(const StepArgs<T> a, int rk, int nkb)
+__global__ __launch_bounds__(T3_THREADS, sizeof(T) == 4 ? 3 : 1) void step3d_kernel(const StepArgs<T> a, int rk, int nkb)
 {
     constexpr bool RING = !IMG;
     __shared__ T tile[IMG ? 2 : 1][2][T3_LH][T3_LW];
@@ -360,6 +360,7 @@ __global__ __launch_bounds__(T3_THREADS) void step3d_kernel(const StepArgs<T> a,
             const V4<T> P = b.out ? gld4s<NT>(up + o) : zero;
             V4<T> kv = zero;
             if constexpr (VARC) kv = b.out ? gld4s<NT>(coef + o) : zero;
+            const V4<T> G = b.out ? gld4(gp + o) : zero;  // with u_prev and kc: one memory latency
             T bpk = T(0);
             if constexpr (SPONGE) bpk = a.beta_p[k];
             T val[4];
@@ -370,7 +371,6 @@ __global__ __launch_bounds__(T3_THREADS) void step3d_kernel(const StepArgs<T> a,
             for (int e = 0; e < 4; e++)
                 gv[e] = lap3<T, K>(a.wc, a.wp, a.wr, a.wcol, wf, wf[2 * K], tile[IMG ? 1 : 0][buf], b.ty, b.tx, e);
             if (b.out) {
-                const V4<T> G = gld4(gp + o);
 #pragma unroll
                 for (int e = 0; e < 4; e++) gv[e] = b.valid[e] ? fma3(val[e], gv[e], G.e[e]) : T(0);
                 gst4(gp + o, gv);
