@@ -126,9 +126,12 @@ def fwi_result(args) -> dict:
     torch.cuda.synchronize()
     fwi.batched_gradient(w, obs)  # warm-up (allocations, code objects)
     (J, grad), t_b = fwi.timed(fwi.batched_gradient, w, obs)
+    plan_info = [{"seg": p.seg, "checkpoints": p.nckpt, "GB": round(p.bytes / 1e9, 1)}
+                 for p in autograd._PLAN_CACHE.values()]
     w64 = fwi.setup(nsteps=nsteps, dtype=torch.float64)
     (J64, grad64), _ = fwi.timed(fwi.batched_gradient, w64, obs.double())
     err64 = float((grad - grad64).norm() / grad64.norm())
+    autograd.clear_plans()  # the plain leg's autograd tape needs the memory
     plain_shots = max(0, args.plain_shots)  # 0: no plain leg (e.g. for counter runs)
     t_p = 0.0
     for s in range(plain_shots):
@@ -151,8 +154,10 @@ def fwi_result(args) -> dict:
         "dtype": "f32",
         "data": "synthetic (seeded layered model +-10 %, smoothed start, Ricker shots)",
         "config": {"workload": "2D FWI gradient, 16 shots batched: 1024x1024 nodes, order 8, 201 receivers, "
-                               f"{nsteps} steps, sqrt(n) checkpointing", "shots": w["nshots"],
-                   "launch": {"in_use": list(w["grid"].launch()), "tuned": tuned}},
+                               f"{nsteps} steps, every forward level kept when it fits (else sqrt(n) "
+                               "checkpointing)", "shots": w["nshots"],
+                   "launch": {"in_use": list(w["grid"].launch()), "tuned": tuned},
+                   "plan": plan_info},
         "plain_s_per_shot": None if s_p is None else round(s_p, 4),
         "speedup_vs_plain": None if s_p is None else round(s_p / s_b, 2),
         "plain_sample": f"{plain_shots} shot(s) of torch-on-GPU slice-form step + autograd",

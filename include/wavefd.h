@@ -234,8 +234,12 @@ int wfd_get_config(const wfd_grid* g, int32_t* variant, int32_t* rows);
 /* A plan owns the checkpoint store, the segment buffer and the working fields for a forward
  * run of `nsteps` steps from rest and its adjoint.  seg = steps per segment (0: chosen to
  * minimise 2*ceil(nsteps/seg) + seg fields); max_bytes caps the allocation (0: no cap) and seg
- * shrinks the store to fit.  Replaces cuwave.utils.misfit_gradient's internal memory strategy
- * (projects/18_fwi/2D_fwi.py:40) with checkpoint + recompute. */
+ * shrinks the store to fit.  seg = WFD_SEG_FASTEST keeps every level (seg 1: nsteps + 6 fields,
+ * no recompute in the backward pass) when that store fits max_bytes -- or, with max_bytes 0,
+ * 0.8 of the device memory free at creation -- and is memory-optimal otherwise.  The gradient
+ * does not depend on seg (same bits).  Replaces cuwave.utils.misfit_gradient's internal memory
+ * strategy (projects/18_fwi/2D_fwi.py:40) with checkpoint + recompute. */
+#define WFD_SEG_FASTEST (-1)
 typedef struct wfd_plan wfd_plan;
 int wfd_plan_create(wfd_plan** plan, const wfd_grid* g, int64_t nsteps, int64_t seg,
                     size_t max_bytes);

@@ -65,8 +65,20 @@ int wfd_plan_create(wfd_plan** out, const wfd_grid* g, int64_t nsteps, int64_t s
     *out = nullptr;
     if (g->ld <= 0 || g->elems <= 0 || g->nshots < 1) return plan_fail(WFD_EINVAL, "wfd_plan_create: grid not initialised");
     if (nsteps < 1) return plan_fail(WFD_EINVAL, "wfd_plan_create: nsteps must be >= 1");
-    if (seg < 0 || seg > nsteps) return plan_fail(WFD_EINVAL, "wfd_plan_create: seg must be in [0, nsteps]");
+    if ((seg < 0 && seg != WFD_SEG_FASTEST) || seg > nsteps)
+        return plan_fail(WFD_EINVAL, "wfd_plan_create: seg must be in [0, nsteps] or WFD_SEG_FASTEST");
     const size_t batch = (size_t)g->elems * (size_t)g->nshots * (g->dtype == WFD_F64 ? 8 : 4);
+    if (seg == WFD_SEG_FASTEST) {
+        // every level kept (no recompute: 16 + 28 instead of 16 + 20 + 28 bytes per cell-step)
+        // when that store fits the budget
+        size_t budget = max_bytes;
+        if (!budget) {
+            size_t free_b = 0, total_b = 0;
+            if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return plan_fail(WFD_EHIP, "wfd_plan_create: hipMemGetInfo failed");
+            budget = (size_t)(0.8 * (double)free_b);
+        }
+        seg = fields_needed(nsteps, 1) * batch <= budget ? 1 : 0;
+    }
     if (seg == 0) {
         int64_t best = 1;
         size_t best_f = fields_needed(nsteps, 1);

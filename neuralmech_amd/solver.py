@@ -499,11 +499,16 @@ def kc_gradient(grid: WaveGrid, Gn: torch.Tensor, mu_src, sig, src: Optional[Poi
     return grad
 
 
+SEG_FASTEST = -1  # WFD_SEG_FASTEST: keep every level when it fits (see GradientPlan)
+
+
 class GradientPlan:
     """Checkpointed forward + adjoint for ``nsteps`` steps from rest (``wfd_plan_*``).
 
-    ``seg`` is the checkpoint spacing (0 = memory-optimal, about sqrt(2 nsteps)); ``max_bytes``
-    caps the device memory of the plan.
+    ``seg`` is the checkpoint spacing (0 = memory-optimal, about sqrt(2 nsteps); ``SEG_FASTEST``
+    = every level kept, no recompute, when that store fits ``max_bytes`` or, with ``max_bytes``
+    0, 0.8 of the free device memory, else memory-optimal); ``max_bytes`` caps the device memory
+    of the plan.  The gradient does not depend on the spacing.
     """
 
     def __init__(self, grid: WaveGrid, nsteps: int, seg: int = 0, max_bytes: int = 0):

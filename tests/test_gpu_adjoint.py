@@ -62,6 +62,27 @@ def test_gradient_independent_of_checkpoint_spacing():
         assert torch.equal(gr, grads[0])
 
 
+def test_fastest_spacing_keeps_every_level_when_it_fits():
+    """seg = SEG_FASTEST: every level kept (seg 1, no recompute) when the store fits the budget,
+    the memory-optimal spacing when it does not; the same gradient bits either way."""
+    from neuralmech_amd import solver
+    from neuralmech_amd._lib import WaveError
+
+    case = make_case((47, 39), 8, seed=4, nsteps=61)
+    obs = np.zeros((1, 61, len(case["rec"])))
+    ref = _plan_grad(case, torch.float32, seg=0, obs=obs)
+    plan, _, _, grad = _plan_grad(case, torch.float32, seg=solver.SEG_FASTEST, obs=obs)
+    assert plan.seg == 1 and plan.nckpt == 61
+    assert torch.equal(grad, ref[3])
+    g, _, _, _ = gpu_objects(case, torch.float32)
+    full = solver.GradientPlan(g, 61, seg=1).bytes
+    small = solver.GradientPlan(g, 61, seg=solver.SEG_FASTEST, max_bytes=full - 1)
+    assert small.seg == ref[0].seg and small.bytes == ref[0].bytes
+    assert solver.GradientPlan(g, 61, seg=solver.SEG_FASTEST, max_bytes=full).seg == 1
+    with pytest.raises(WaveError):
+        solver.GradientPlan(g, 61, seg=-2)
+
+
 def test_adjoint_dot_product_fp64():
     """<A s, y> == <s, A^T y> for the signal-to-trace map A, A^T y = -mu at the sources."""
     case = make_case((41, 52), 6, seed=9, nsteps=80, nsrc=3, nrec=7)
