@@ -215,16 +215,19 @@ int wfd_check_medium(const wfd_grid* g, const wfd_medium* med, wfd_stream_t stre
 /* The step kernels come in variants (2D: 0 = halo chunks from cache, 1 = halo chunks exchanged
  * between lanes; either, and the 3D kernel's 0, | WFD_VARIANT_STREAM = the fields a step touches
  * once (u_prev, kc, u_next) read and written with the nontemporal hint, which pays off when the
- * step's fields exceed the 256 MB Infinity Cache and costs when they fit) and march `rows` rows
- * (2D) or planes (3D) per wave / block.  The choice is kept per grid shape (ndim, order, dtype,
- * n, nshots) in a process-wide table; untuned grids stream when their four step fields exceed
- * 512 MiB.  Results do not depend on it (every variant evaluates the same expression in the
+ * step's fields exceed the 256 MB Infinity Cache and costs when they fit; 2D grids of several
+ * shots also | WFD_VARIANT_SHOTS = tiles taken shot-fastest, so that the tiles of one region in
+ * every shot run back to back on one XCD and read a coefficient field the shots share from its
+ * L2) and march `rows` rows (2D) or planes (3D) per wave / block.  The choice is kept per grid
+ * shape (ndim, order, dtype, n, nshots) in a process-wide table; untuned grids stream when their
+ * four step fields exceed 512 MiB and take the tiles of several shots shot-fastest.  Results do not depend on it (every variant evaluates the same expression in the
  * same order).
  * wfd_tune times every candidate for `steps` steps on the caller's scratch fields u_a, u_b
  * (overwritten; host-synchronising) and records the fastest; the reference's fixed launch
  * shapes (2D_wave_cuda_padding.py:22,62-63) are what it replaces.
  * wfd_set_config forces a shape (rows = 0 forgets it); wfd_get_config reports the one in use. */
 #define WFD_VARIANT_STREAM 2
+#define WFD_VARIANT_SHOTS 4
 int wfd_tune(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med, int32_t steps,
              int32_t* variant, int32_t* rows, double* us_per_step, wfd_stream_t stream);
 int wfd_set_config(const wfd_grid* g, int32_t variant, int32_t rows);

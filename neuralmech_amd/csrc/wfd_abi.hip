@@ -193,7 +193,9 @@ void forget_cfg(const wfd_grid* g)
 // SIMD (1024 on MI355X) several, then as many rows per wave as that allows; streaming access when
 // the four fields of a step (u_prev, u_cur, u_next, kc) exceed twice the 256 MB Infinity Cache (a
 // working set that still fits in good part gains more from the cache hits: 16 shots of 1024^2,
-// 279 MB, run the gradient plan 7 % slower streaming).
+// 279 MB, run the gradient plan 7 % slower streaming); 2D grids of several shots take their tiles
+// shot-fastest (16 shots of 1024^2, order 8, lane-exchange halo, 2 rows: 39.1 -> 37.1 us per
+// forward step; cache halo 38.6 -> 34.4 us).
 LaunchCfg default_cfg(const wfd_grid* g)
 {
     {
@@ -208,7 +210,7 @@ LaunchCfg default_cfg(const wfd_grid* g)
         c.ri = 32;
         c.tile = stream;
     } else if (g->ndim == 2) {
-        c.tile = 1 | stream;
+        c.tile = 1 | stream | (g->nshots > 1 ? WFD_VARIANT_SHOTS : 0);
         const int64_t chunks = ((g->n[1] + 247) / 248) * (int64_t)g->nshots;
         const int64_t rows = g->n[0] - 2;
         int ri = 8;
@@ -302,6 +304,7 @@ int launch_t(const Launch& L, hipStream_t s)
     if (rc) return rc;
     a.gather_off = L.gather_off;
     a.nshots = g->nshots;
+    a.shot_fast = g->ndim == 2 && g->nshots > 1 && (L.cfg.tile & WFD_VARIANT_SHOTS) && !L.un2 ? 1 : 0;
     a.ngb = a.rec.count ? (uint32_t)(((int64_t)g->nshots * a.rec.count + 255) / 256) : 0u;
     a.step = L.step;
     a.step_dev = L.step_dev;
@@ -961,7 +964,7 @@ int wfd_set_config(const wfd_grid* g, int32_t variant, int32_t rows)
         forget_cfg(g);
         return WFD_OK;
     }
-    const bool ok2d = g->ndim == 2 && variant >= 0 && variant <= (1 | WFD_VARIANT_STREAM) &&
+    const bool ok2d = g->ndim == 2 && variant >= 0 && variant <= (1 | WFD_VARIANT_STREAM | WFD_VARIANT_SHOTS) &&
                       (g->dtype == WFD_F64 ? rows == 4 : (rows == 1 || rows == 2 || rows == 4 || rows == 8));
     const bool ok3d = g->ndim == 3 && (variant == 0 || variant == WFD_VARIANT_STREAM) && rows >= 1 && rows <= 1024;
     const bool ok1d = g->ndim == 1 && variant == 0 && rows == 1;
@@ -992,7 +995,8 @@ int wfd_tune(const wfd_grid* g, void* u_a, void* u_b, const wfd_medium* med, int
     std::vector<LaunchCfg> cands;
     if (g->ndim == 2) {
         const int f32rows[] = {1, 2, 4, 8};
-        for (int v = 0; v < 4; v++) {
+        const int nv = g->nshots > 1 ? 8 : 4;  // the shot-fastest order only matters for several shots
+        for (int v = 0; v < nv; v++) {
             if (g->dtype == WFD_F64) {
                 cands.push_back({4, v});
             } else {

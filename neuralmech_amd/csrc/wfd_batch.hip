@@ -60,7 +60,7 @@ struct BShot {
     const T* obs;                   // observed traces [nsteps][rec.width]
     double* G;                      // node (0, 0) of the imaging sum
     uint32_t gx, gy, stiles;        // tile grid of forward / recompute steps
-    uint32_t gy_adj, stiles_adj;    // adjoint steps (2 rows per wave)
+    uint32_t gy_adj, stiles_adj;    // adjoint steps (ADJ_ROWS rows per wave, float64: 2)
     int32_t K, cfg;
 };
 
@@ -139,17 +139,26 @@ __device__ __forceinline__ uint32_t hw_xcc_id() { return __builtin_amdgcn_s_getr
 
 // Forward / recompute tiles march cfg_rows(cfg) rows per wave (8: fewer re-reads of the window
 // rows and fewer tiles per phase; 3.44 s against 3.53 s with 4 rows for the 64-shot batch);
-// adjoint tiles (two register windows) and every float64 tile march 2, which keeps the kernel at
-// 3 waves per SIMD.  At that occupancy the 8-row tiles load their rows' operands 2 rows ahead;
-// the 2-row tiles row by row (a look-ahead there spills 13 VGPRs).
+// float adjoint tiles (two register windows) march ADJ_ROWS = 4 and float64 tiles 2, which keeps
+// the kernel at 3 waves per SIMD.  At that occupancy the 8-row tiles load their rows' operands 2
+// rows ahead, the others row by row (a look-ahead on 2-row adjoint tiles spills 13 VGPRs, on
+// 4-row ones 41).  4-row adjoint tiles re-read 12 window rows per 4 output rows instead of 10 per
+// 2 and halve the adjoint tiles per phase: 3.350 -> 3.258 s for the 64-shot batch, though the
+// kernel then spills 3 VGPRs (2 slots per CU with 8-row adjoint tiles and no spills: 3.368 s).
+constexpr int ADJ_ROWS = 4;
 template <typename T, int K, bool IMG>
 __device__ __forceinline__ void batch_tile_k(const StepArgs<T>& a, int cfg, uint32_t tx, uint32_t ty, int64_t step)
 {
-    if constexpr (sizeof(T) == 8 || IMG) {
+    if constexpr (sizeof(T) == 8) {
         if (cfg_variant(cfg) == 1)
             step2d_tile<T, K, true, true, true, IMG, 2, true, 0>(a, tx, ty, 0, step);
         else
             step2d_tile<T, K, true, true, true, IMG, 2, false, 0>(a, tx, ty, 0, step);
+    } else if constexpr (IMG) {
+        if (cfg_variant(cfg) == 1)
+            step2d_tile<T, K, true, true, true, IMG, ADJ_ROWS, true, 0>(a, tx, ty, 0, step);
+        else
+            step2d_tile<T, K, true, true, true, IMG, ADJ_ROWS, false, 0>(a, tx, ty, 0, step);
     } else {
         switch (cfg) {
             case 0: step2d_tile<T, K, true, true, true, IMG, 8, true, 2>(a, tx, ty, 0, step); break;
@@ -555,7 +564,7 @@ void fill_pts(PtsDev<T>& p, const wfd_points& w, void* data, int64_t width, int6
 void stiles_of(const wfd_grid& g, int cfg, bool adj, uint32_t& gx, uint32_t& gy)
 {
     const int64_t cols = cfg_variant(cfg) == 1 ? 248 : 256;
-    const int64_t ri = (g.dtype == WFD_F64 || adj) ? 2 : cfg_rows(cfg);  // as batch_tile_k
+    const int64_t ri = g.dtype == WFD_F64 ? 2 : (adj ? ADJ_ROWS : cfg_rows(cfg));  // as batch_tile_k
     gx = (uint32_t)((g.n[1] + cols - 1) / cols);
     gy = (uint32_t)((g.n[0] - 2 + 4 * ri - 1) / (4 * ri));
 }

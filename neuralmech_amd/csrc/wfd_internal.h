@@ -61,6 +61,7 @@ struct StepArgs {
     uint32_t ngb;        // gather blocks (256 points each)
     uint32_t gx, gy;     // stencil block grid (x fastest); the launch is 1D: ngb + gx*gy*gz blocks
     int32_t nshots;
+    int32_t shot_fast;  // 2D step launches: tile index -> (shot fastest, then column, then row block)
     // Two-step launches: un receives u^{n+1}, un2 u^{n+2}; their gather blocks record up (u^{n-1})
     // and uc (u^n) into trace rows step + gather_off - 1 and step + gather_off.
     T* un2;
@@ -87,14 +88,17 @@ constexpr int WAIT_VMCNT0 = 0x0F70;
 // first `gper` blocks of every XCD are gather blocks (so they start first); the remaining blocks
 // of XCD x take one contiguous range of stencil tiles, start(x) + i, the same ranges whether or
 // not gather blocks are present.  Neighbouring tiles -- and the halo rows / planes they share --
-// then stay in one L2.  The launch has 8 * gper + tiles blocks (see step_grid).
+// then stay in one L2.  The launch has 8 * gper + tiles blocks (see step_grid).  With shot_fast
+// (gz shots) the tile index runs over the shots first: the tiles of one region in every shot are
+// neighbours, so a coefficient field shared by the shots is read from the L2 after the first.
 struct StepBlock {
     bool gather;
     uint32_t g;        // gather block index
     uint32_t x, y, z;  // stencil tile
 };
 constexpr uint32_t XCDS = 8;
-__device__ __forceinline__ StepBlock step_block(uint32_t ngb, uint32_t gx, uint32_t gy)
+__device__ __forceinline__ StepBlock step_block(uint32_t ngb, uint32_t gx, uint32_t gy, uint32_t gz = 1,
+                                                bool shot_fast = false)
 {
     const uint32_t gper = (ngb + XCDS - 1) / XCDS;
     const uint32_t b = blockIdx.x, x = b % XCDS, i = b / XCDS;
@@ -102,6 +106,10 @@ __device__ __forceinline__ StepBlock step_block(uint32_t ngb, uint32_t gx, uint3
     const uint32_t nt = gridDim.x - gper * XCDS;         // stencil tiles
     const uint32_t per = nt / XCDS, rem = nt % XCDS;
     const uint32_t j = x * per + (x < rem ? x : rem) + (i - gper);
+    if (shot_fast) {
+        const uint32_t t = j / gz;
+        return {false, 0, t % gx, t / gx, j % gz};
+    }
     return {false, 0, j % gx, (j / gx) % gy, j / (gx * gy)};
 }
 // blocks of a step launch with `tiles` stencil tiles and ngb gather blocks

@@ -316,7 +316,7 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
 template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, bool NT>
 __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
 {
-    const StepBlock bp = step_block(a.ngb, a.gx, a.gy);
+    const StepBlock bp = step_block(a.ngb, a.gx, a.gy, (uint32_t)a.nshots, a.shot_fast != 0);
     const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
     if (bp.gather) {
         gather_points(a, bp.g, step);

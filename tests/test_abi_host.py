@@ -205,7 +205,8 @@ def test_acoustic_entry_points_validate_arguments(case):
 
 def test_launch_config_rules_need_no_gpu():
     """Launch shapes are host state: the streaming bit (WFD_VARIANT_STREAM = 2) is accepted on both
-    2D halo variants and the 3D kernel, anything else is rejected, and an untuned grid streams
+    2D halo variants and the 3D kernel, the shot-order bit (WFD_VARIANT_SHOTS = 4) on 2D grids only,
+    anything else is rejected, and an untuned grid streams
     exactly when its four step fields exceed 512 MiB."""
     lib = _lib.load()
     v, r = ctypes.c_int32(), ctypes.c_int32()
@@ -225,14 +226,14 @@ def test_launch_config_rules_need_no_gpu():
     assert cfg(batch)[0] & 2 == 2  # 16 shots of 2048^2: 16 x 4 fields x 17 MB
     assert cfg(fwi)[0] & 2 == 0  # 16 shots of 1024^2: 279 MB, in good part cache-resident
     assert cfg(cube)[0] == 2 and cfg(small3)[0] == 0
-    for variant in (0, 1, 2, 3):
+    for variant in range(8):  # bit 2: shot-fastest tile order (WFD_VARIANT_SHOTS)
         assert lib.wfd_set_config(ctypes.byref(small), variant, 4) == 0
         assert cfg(small) == (variant, 4)
-    for variant in (-1, 4, 5):
+    for variant in (-1, 8, 9):
         assert lib.wfd_set_config(ctypes.byref(small), variant, 4) == _lib.WFD_EINVAL
     assert lib.wfd_set_config(ctypes.byref(cube), 2, 16) == 0 and cfg(cube) == (2, 16)
-    assert lib.wfd_set_config(ctypes.byref(cube), 1, 16) == _lib.WFD_EINVAL
-    assert lib.wfd_set_config(ctypes.byref(cube), 3, 16) == _lib.WFD_EINVAL
+    for variant in (1, 3, 4, 6):
+        assert lib.wfd_set_config(ctypes.byref(cube), variant, 16) == _lib.WFD_EINVAL
     for g in (small, cube):
         assert lib.wfd_set_config(ctypes.byref(g), 0, 0) == 0  # forget: back to the default
     assert cfg(small)[0] & 2 == 0 and cfg(cube) == (2, 32)

@@ -225,3 +225,34 @@ def test_two_outstanding_forwards_keep_their_own_plans():
     assert torch.equal(a.grad, ga) and torch.equal(b.grad, gb)
     assert torch.equal(grad_of(v0), ga)  # the kept plan again
     autograd.clear_plans()
+
+
+@pytest.mark.parametrize("variant,rows", [(4, 2), (5, 8), (6, 4), (7, 1)])
+def test_shot_fastest_tile_order_bitexact(variant, rows):
+    """Several shots on one grid: the traces and the velocity gradient with the tiles taken
+    shot-fastest (WFD_VARIANT_SHOTS = 4) equal those of the same halo / access variant in the
+    default order bit for bit (only the order in which tiles run changes), for 1..8 rows."""
+    from neuralmech_amd import solver
+
+    print(f"\n[shot order] variant={variant} rows={rows}", flush=True)
+    case = make_case((41, 530), 8, seed=31, nsteps=36, nrec=7)
+    shots = [[(5, 10 + 120 * s)] for s in range(4)]
+    sig = np.random.default_rng(8).standard_normal((4, 36, 1))
+    out = {}
+    for v in (variant & 3, variant):
+        g, med, _, _ = gpu_objects(case, torch.float32, nshots=4)
+        src = solver.PointSet(g, shots, per_shot=True)
+        rec = solver.PointSet(g, case["rec"])
+        g.set_launch(v, rows)
+        try:
+            assert g.launch() == (v, rows)
+            plan = solver.GradientPlan(g, 36, seg=6)
+            tr = plan.forward(med, src, torch.as_tensor(sig), rec)
+            grad = plan.backward(med, src, rec, tr)
+            torch.cuda.synchronize()
+        finally:
+            g.set_launch(0, 0)
+        out[v] = (tr, grad)
+    (tr0, g0), (tr1, g1) = out[variant & 3], out[variant]
+    assert torch.isfinite(g0).all() and g0.abs().max() > 0
+    assert torch.equal(tr0, tr1) and torch.equal(g0, g1)
