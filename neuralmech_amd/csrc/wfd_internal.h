@@ -139,7 +139,9 @@ __device__ __forceinline__ void gather_points(const StepArgs<T>& a, uint32_t blk
 
 struct LaunchCfg {
     int ri;    // rows (2D) or planes (3D) marched per wave / block
-    int tile;  // variant: bit 0 = lane-exchange halo (2D), bit 1 = streaming access (WFD_VARIANT_STREAM)
+    int tile;  // variant: bit 0 = lane-exchange halo (2D), bit 1 = streaming access (WFD_VARIANT_STREAM),
+               // bit 2 = shot-fastest tile order (WFD_VARIANT_SHOTS, 2D)
+    int img_ri = 0;  // 3D imaging launches: planes per block when forced (wfd_set_config), 0 = by grid size
 };
 
 template <typename T>

@@ -230,14 +230,17 @@ __device__ __forceinline__ void plane_update(const StepArgs<T>& a, const Tile3<T
 // the L2) and kept in an LDS ring of K + 1 slots until the tile of that plane is staged from it;
 // the ring is private to each thread (a thread reads back only the entries it wrote), and slot
 // (k + K + 1) % (K + 1), written at iteration k, was read (for plane k) at iteration k - 1.
-// Imaging launches (two staged fields, more registers) load the halo of plane k directly and the
-// operands of plane k after the barrier.
+// Imaging launches march the same way with two staged fields (the adjoint field and the forward
+// field whose Laplacian the imaging condition takes), each with its own z-window and halo ring,
+// and also load the imaging partial of plane k + 1 ahead.
 template <typename T, int K, bool VARC, bool SPONGE, bool IMG, bool NT>
-__global__ __launch_bounds__(T3_THREADS, sizeof(T) == 4 ? 3 : 1) void step3d_kernel(const StepArgs<T> a, int rk, int nkb)
+__global__ __launch_bounds__(T3_THREADS, sizeof(T) == 4 ? (IMG ? 2 : 3) : 1) void step3d_kernel(const StepArgs<T> a, int rk,
+                                                                                             int nkb)
 {
     constexpr bool RING = !IMG;
-    __shared__ T tile[IMG ? 2 : 1][2][T3_LH][T3_LW];
-    __shared__ V4<T> ring[RING ? K + 1 : 1][T3_HALO_THREADS];
+    constexpr int NF = IMG ? 2 : 1;  // staged fields
+    __shared__ T tile[NF][2][T3_LH][T3_LW];
+    __shared__ V4<T> ring[NF][K + 1][T3_HALO_THREADS];
     const StepBlock bp = step_block(a.ngb, a.gx, a.gy);
     const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
     if (bp.gather) {
@@ -296,11 +299,11 @@ __global__ __launch_bounds__(T3_THREADS, sizeof(T) == 4 ? 3 : 1) void step3d_ker
         if constexpr (SPONGE) bpk = a.beta_p[k0];
         if (b.halo_thread) {  // halos of planes k0 .. k0 + K
             for (int m = 0; m <= K; m++)
-                if (k0 + m < k1) ring[(k0 + m) % (K + 1)][threadIdx.x] = halo_fetch(uc, k0 + m, b.i0, b.j0, a.nr, b.ld, ps);
+                if (k0 + m < k1) ring[0][(k0 + m) % (K + 1)][threadIdx.x] = halo_fetch(uc, k0 + m, b.i0, b.j0, a.nr, b.ld, ps);
         }
 #pragma unroll
         for (int e = 0; e < 4; e++) tile[0][0][b.ty + HALO][HALO + b.tx * 4 + e] = win[K].e[e];
-        if (b.halo_thread) halo_put(tile[0][0], ring[k0 % (K + 1)][threadIdx.x]);
+        if (b.halo_thread) halo_put(tile[0][0], ring[0][k0 % (K + 1)][threadIdx.x]);
         __syncthreads();
         __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
         int buf = 0;
@@ -327,12 +330,12 @@ __global__ __launch_bounds__(T3_THREADS, sizeof(T) == 4 ? 3 : 1) void step3d_ker
             P = Pn;
             kv = kn;
             bpk = bpn;
-            if (b.halo_thread && k + K + 1 < k1) ring[(k + K + 1) % (K + 1)][threadIdx.x] = hn;
+            if (b.halo_thread && k + K + 1 < k1) ring[0][(k + K + 1) % (K + 1)][threadIdx.x] = hn;
             if (b.out) gst4s<NT>(un + k * ps + rowoff, val);
             if (k + 1 < k1) {
 #pragma unroll
                 for (int e = 0; e < 4; e++) tile[0][buf ^ 1][b.ty + HALO][HALO + b.tx * 4 + e] = win[K].e[e];
-                if (b.halo_thread) halo_put(tile[0][buf ^ 1], ring[(k + 1) % (K + 1)][threadIdx.x]);
+                if (b.halo_thread) halo_put(tile[0][buf ^ 1], ring[0][(k + 1) % (K + 1)][threadIdx.x]);
                 lds_barrier();
             }
         }
@@ -342,43 +345,90 @@ __global__ __launch_bounds__(T3_THREADS, sizeof(T) == 4 ? 3 : 1) void step3d_ker
         V4<T> wf[2 * K + 1];
 #pragma unroll
         for (int m = 0; m < 2 * K; m++) wf[m] = b.load_row ? gld4(uf + (k0 - K + m) * ps + rowoff) : zero;
+        V4<T> wc = b.load_row ? gld4(uc + (k0 + K) * ps + rowoff) : zero;   // plane k + K of both fields
+        V4<T> wcf = b.load_row ? gld4(uf + (k0 + K) * ps + rowoff) : zero;
+        V4<T> P = b.out ? gld4s<NT>(up + k0 * ps + rowoff) : zero;
+        V4<T> kv = zero;
+        if constexpr (VARC) kv = b.out ? gld4s<NT>(coef + k0 * ps + rowoff) : zero;
+        V4<T> G = b.out ? gld4(gp + k0 * ps + rowoff) : zero;
+        T bpk = T(0);
+        if constexpr (SPONGE) bpk = a.beta_p[k0];
+        if (b.halo_thread) {  // halos of planes k0 .. k0 + K
+            for (int m = 0; m <= K; m++)
+                if (k0 + m < k1) {
+                    ring[0][(k0 + m) % (K + 1)][threadIdx.x] = halo_fetch(uc, k0 + m, b.i0, b.j0, a.nr, b.ld, ps);
+                    ring[NF - 1][(k0 + m) % (K + 1)][threadIdx.x] = halo_fetch(uf, k0 + m, b.i0, b.j0, a.nr, b.ld, ps);
+                }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            tile[0][0][b.ty + HALO][HALO + b.tx * 4 + e] = win[K].e[e];
+            tile[NF - 1][0][b.ty + HALO][HALO + b.tx * 4 + e] = wf[K].e[e];
+        }
+        if (b.halo_thread) {
+            halo_put(tile[0][0], ring[0][k0 % (K + 1)][threadIdx.x]);
+            halo_put(tile[NF - 1][0], ring[NF - 1][k0 % (K + 1)][threadIdx.x]);
+        }
+        __syncthreads();
+        __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
         int buf = 0;
         for (int64_t k = k0; k < k1; k++, buf ^= 1) {
-            win[2 * K] = b.load_row ? gld4(uc + (k + K) * ps + rowoff) : zero;
-            wf[2 * K] = b.load_row ? gld4(uf + (k + K) * ps + rowoff) : zero;
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                tile[0][buf][b.ty + HALO][HALO + b.tx * 4 + e] = win[K].e[e];
-                tile[IMG ? 1 : 0][buf][b.ty + HALO][HALO + b.tx * 4 + e] = wf[K].e[e];
-            }
-            if (b.halo_thread) {
-                halo_put(tile[0][buf], halo_fetch(uc, k, b.i0, b.j0, a.nr, b.ld, ps));
-                halo_put(tile[IMG ? 1 : 0][buf], halo_fetch(uf, k, b.i0, b.j0, a.nr, b.ld, ps));
-            }
-            lds_barrier();
-            const int64_t o = k * ps + rowoff;
-            const V4<T> P = b.out ? gld4s<NT>(up + o) : zero;
-            V4<T> kv = zero;
-            if constexpr (VARC) kv = b.out ? gld4s<NT>(coef + o) : zero;
-            const V4<T> G = b.out ? gld4(gp + o) : zero;  // with u_prev and kc: one memory latency
-            T bpk = T(0);
-            if constexpr (SPONGE) bpk = a.beta_p[k];
+            // the look-ahead loads of plane k + 1 and of plane k + K + 1 stay in the layout (as in
+            // the forward march); the partial of plane k + 1 is not the one stored below (plane k)
+            const V4<T> wn = b.load_row ? gld4(uc + (k + K + 1) * ps + rowoff) : zero;
+            const V4<T> wfn = b.load_row ? gld4(uf + (k + K + 1) * ps + rowoff) : zero;
+            const V4<T> Pn = b.out ? gld4s<NT>(up + (k + 1) * ps + rowoff) : zero;
+            V4<T> kn = zero;
+            if constexpr (VARC) kn = b.out ? gld4s<NT>(coef + (k + 1) * ps + rowoff) : zero;
+            const V4<T> Gn = b.out ? gld4(gp + (k + 1) * ps + rowoff) : zero;
+            T bpn = T(0);
+            if constexpr (SPONGE) bpn = a.beta_p[k + 1];
+            const bool hk = b.halo_thread && k + K + 1 < k1;
+            const V4<T> hn = hk ? halo_fetch(uc, k + K + 1, b.i0, b.j0, a.nr, b.ld, ps) : zero;
+            const V4<T> hfn = hk ? halo_fetch(uf, k + K + 1, b.i0, b.j0, a.nr, b.ld, ps) : zero;
             T val[4];
-            plane_update<T, K, VARC, SPONGE, NT>(a, b, win, win[2 * K], tile[0][buf], P, kv, bpk, k, step, sm, val);
-            if (b.out) gst4s<NT>(un + o, val);
+            plane_update<T, K, VARC, SPONGE, NT>(a, b, win, wc, tile[0][buf], P, kv, bpk, k, step, sm, val);
             T gv[4];
 #pragma unroll
-            for (int e = 0; e < 4; e++)
-                gv[e] = lap3<T, K>(a.wc, a.wp, a.wr, a.wcol, wf, wf[2 * K], tile[IMG ? 1 : 0][buf], b.ty, b.tx, e);
-            if (b.out) {
-#pragma unroll
-                for (int e = 0; e < 4; e++) gv[e] = b.valid[e] ? fma3(val[e], gv[e], G.e[e]) : T(0);
-                gst4(gp + o, gv);
+            for (int e = 0; e < 4; e++) {
+                gv[e] = lap3<T, K>(a.wc, a.wp, a.wr, a.wcol, wf, wcf, tile[NF - 1][buf], b.ty, b.tx, e);
+                gv[e] = b.valid[e] ? fma3(val[e], gv[e], G.e[e]) : T(0);
             }
+            // take over the look-ahead operands before the stores (see the forward march)
+            __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
 #pragma unroll
-            for (int m = 0; m < 2 * K; m++) {
+            for (int m = 0; m < 2 * K - 1; m++) {
                 win[m] = win[m + 1];
                 wf[m] = wf[m + 1];
+            }
+            win[2 * K - 1] = wc;
+            wf[2 * K - 1] = wcf;
+            wc = wn;
+            wcf = wfn;
+            P = Pn;
+            kv = kn;
+            G = Gn;
+            bpk = bpn;
+            if (hk) {
+                ring[0][(k + K + 1) % (K + 1)][threadIdx.x] = hn;
+                ring[NF - 1][(k + K + 1) % (K + 1)][threadIdx.x] = hfn;
+            }
+            const int64_t o = k * ps + rowoff;
+            if (b.out) {
+                gst4s<NT>(un + o, val);
+                gst4(gp + o, gv);
+            }
+            if (k + 1 < k1) {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    tile[0][buf ^ 1][b.ty + HALO][HALO + b.tx * 4 + e] = win[K].e[e];
+                    tile[NF - 1][buf ^ 1][b.ty + HALO][HALO + b.tx * 4 + e] = wf[K].e[e];
+                }
+                if (b.halo_thread) {
+                    halo_put(tile[0][buf ^ 1], ring[0][(k + 1) % (K + 1)][threadIdx.x]);
+                    halo_put(tile[NF - 1][buf ^ 1], ring[NF - 1][(k + 1) % (K + 1)][threadIdx.x]);
+                }
+                lds_barrier();
             }
         }
     }
