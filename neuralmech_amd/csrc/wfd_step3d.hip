@@ -128,20 +128,55 @@ __device__ __forceinline__ void halo_put(T (*t)[T3_LW], const V4<T>& v)
     }
 }
 
-template <typename T, int K>
-__device__ __forceinline__ T lap3(const T wc, const T* wp, const T* wr, const T* wcol, const V4<T>* win,
-                                  const V4<T>& top, const T (*t)[T3_LW], int ty, int tx, int e)
+// The lane's 4-node chunk of LDS tile row r starting at column c (16-byte aligned: rows of T3_LW
+// elements, c a multiple of 4), read in one 16-byte access per 4 bytes of element.  Reading the
+// chunk element by element instead puts the 64 lanes of a wave on 8 of the 32 LDS banks (lanes
+// tx and tx + 8 of a row, and rows 2 apart, share banks), 4 conflicting passes per read.
+template <typename T>
+__device__ __forceinline__ V4<T> lds4(const T (*t)[T3_LW], int r, int c)
 {
-    // z-window: planes k-K..k+K-1 in win[0..2K-1], plane k+K in top
-    const int lr = ty + HALO, lc = HALO + tx * 4 + e;
-    T lap = wc * win[K].e[e];
+    if constexpr (sizeof(T) == 4) {
+        const s3_v4f v = *reinterpret_cast<const s3_v4f*>(&t[r][c]);
+        return {{v.x, v.y, v.z, v.w}};
+    } else {
+        const s3_v2d a = *reinterpret_cast<const s3_v2d*>(&t[r][c]);
+        const s3_v2d b = *reinterpret_cast<const s3_v2d*>(&t[r][c + 2]);
+        return {{a.x, a.y, b.x, b.y}};
+    }
+}
+
+// value at column offset j in [-4, 8) of a chunk, from the chunks left of, at and right of it
+template <typename T>
+__device__ __forceinline__ T pick3(const V4<T>& L, const V4<T>& C, const V4<T>& R, int j)
+{
+    return j < 0 ? L.e[4 + j] : (j < 4 ? C.e[j] : R.e[j - 4]);
+}
+
+// Laplacian of the lane's 4 nodes of plane k: z-window planes k-K..k+K-1 in win[0..2K-1], plane
+// k+K in top, in-plane neighbours from the staged tile t.  Per node the terms are summed in the
+// order of wavefd.h (centre, marching axis, row axis, column axis, m = 1..K each).
+template <typename T, int K>
+__device__ __forceinline__ void lap3x4(const T wc, const T* wp, const T* wr, const T* wcol, const V4<T>* win,
+                                       const V4<T>& top, const T (*t)[T3_LW], int ty, int tx, T (&lap)[4])
+{
+    const int lr = ty + HALO, c0 = HALO + tx * 4;
 #pragma unroll
-    for (int m = 1; m <= K; m++) lap = fma3(wp[m - 1], win[K - m].e[e] + (m < K ? win[K + m].e[e] : top.e[e]), lap);
+    for (int e = 0; e < 4; e++) lap[e] = wc * win[K].e[e];
 #pragma unroll
-    for (int m = 1; m <= K; m++) lap = fma3(wr[m - 1], t[lr - m][lc] + t[lr + m][lc], lap);
+    for (int m = 1; m <= K; m++)
 #pragma unroll
-    for (int m = 1; m <= K; m++) lap = fma3(wcol[m - 1], t[lr][lc - m] + t[lr][lc + m], lap);
-    return lap;
+        for (int e = 0; e < 4; e++) lap[e] = fma3(wp[m - 1], win[K - m].e[e] + (m < K ? win[K + m].e[e] : top.e[e]), lap[e]);
+#pragma unroll
+    for (int m = 1; m <= K; m++) {
+        const V4<T> up = lds4(t, lr - m, c0), dn = lds4(t, lr + m, c0);
+#pragma unroll
+        for (int e = 0; e < 4; e++) lap[e] = fma3(wr[m - 1], up.e[e] + dn.e[e], lap[e]);
+    }
+    const V4<T> L = lds4(t, lr, c0 - 4), C = lds4(t, lr, c0), R = lds4(t, lr, c0 + 4);
+#pragma unroll
+    for (int m = 1; m <= K; m++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) lap[e] = fma3(wcol[m - 1], pick3(L, C, R, e - m) + pick3(L, C, R, e + m), lap[e]);
 }
 
 // Per-block constants of a 3D tile and its lane's chunk.
@@ -184,12 +219,13 @@ __device__ __forceinline__ void plane_update(const StepArgs<T>& a, const Tile3<T
             damp |= bb[e] != T(0);
         }
     }
+    T lap[4];
+    lap3x4<T, K>(a.wc, a.wp, a.wr, a.wcol, win, top, t, b.ty, b.tx, lap);
 #pragma unroll
     for (int e = 0; e < 4; e++) {
         const T ctr = win[K].e[e];
-        const T lap = lap3<T, K>(a.wc, a.wp, a.wr, a.wcol, win, top, t, b.ty, b.tx, e);
         const T tt = damp ? T(2) * ctr - (T(1) - bb[e]) * P.e[e] : T(2) * ctr - P.e[e];
-        val[e] = fma3(kc[e], lap, tt);
+        val[e] = fma3(kc[e], lap[e], tt);
     }
     // source pass: the wave holds 4 rows of the tile, 16 lanes each.  Which of the next 64 planes
     // have sources in these rows is found once per 64 planes (lane l checks plane k + l, one
@@ -239,7 +275,7 @@ __global__ __launch_bounds__(T3_THREADS, sizeof(T) == 4 ? (IMG ? 2 : 3) : 1) voi
 {
     constexpr bool RING = !IMG;
     constexpr int NF = IMG ? 2 : 1;  // staged fields
-    __shared__ T tile[NF][2][T3_LH][T3_LW];
+    __shared__ __align__(16) T tile[NF][2][T3_LH][T3_LW];  // 16-byte chunk reads (lds4)
     __shared__ V4<T> ring[NF][K + 1][T3_HALO_THREADS];
     const StepBlock bp = step_block(a.ngb, a.gx, a.gy);
     const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
@@ -389,11 +425,9 @@ __global__ __launch_bounds__(T3_THREADS, sizeof(T) == 4 ? (IMG ? 2 : 3) : 1) voi
             T val[4];
             plane_update<T, K, VARC, SPONGE, NT>(a, b, win, wc, tile[0][buf], P, kv, bpk, k, step, sm, val);
             T gv[4];
+            lap3x4<T, K>(a.wc, a.wp, a.wr, a.wcol, wf, wcf, tile[NF - 1][buf], b.ty, b.tx, gv);
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                gv[e] = lap3<T, K>(a.wc, a.wp, a.wr, a.wcol, wf, wcf, tile[NF - 1][buf], b.ty, b.tx, e);
-                gv[e] = b.valid[e] ? fma3(val[e], gv[e], G.e[e]) : T(0);
-            }
+            for (int e = 0; e < 4; e++) gv[e] = b.valid[e] ? fma3(val[e], gv[e], G.e[e]) : T(0);
             // take over the look-ahead operands before the stores (see the forward march)
             __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
 #pragma unroll
