@@ -34,18 +34,43 @@ constexpr int T32_SW = T32_EC + 2 * HALO;
 constexpr int T32_THREADS = 16 * T32_ER;
 constexpr int T32_HALO_THREADS = 8 * 16 + 2 * T32_ER;
 
-template <typename T, int K>
-__device__ __forceinline__ T t32_lap(const StepArgs<T>& a, const C4<T>* win, const T* t, int lw, int e)
+// A 4-node chunk of an LDS tile (16-byte aligned) in one 16-byte access per 4 bytes of element:
+// element-by-element reads of the lanes' chunks put a wave on 8 of the 32 LDS banks.
+template <typename T>
+__device__ __forceinline__ C4<T> lds_chunk(const T* p)
 {
-    // t points at the centre element of the LDS row-major tile with row stride lw
-    T lap = a.wc * win[K].e[e];
+    if constexpr (sizeof(T) == 4) {
+        const wfd_v4f v = *reinterpret_cast<const wfd_v4f*>(p);
+        return {{v.x, v.y, v.z, v.w}};
+    } else {
+        const wfd_v2d x = *reinterpret_cast<const wfd_v2d*>(p);
+        const wfd_v2d y = *reinterpret_cast<const wfd_v2d*>(p + 2);
+        return {{x.x, x.y, y.x, y.y}};
+    }
+}
+
+// Laplacian of the lane's 4 nodes: t points at the lane's chunk in an LDS row-major tile with row
+// stride lw; per node the terms are summed in the order of wavefd.h.
+template <typename T, int K>
+__device__ __forceinline__ void t32_lap4(const StepArgs<T>& a, const C4<T>* win, const T* t, int lw, T (&lap)[4])
+{
 #pragma unroll
-    for (int m = 1; m <= K; m++) lap = fmaT(a.wp[m - 1], win[K - m].e[e] + win[K + m].e[e], lap);
+    for (int e = 0; e < 4; e++) lap[e] = a.wc * win[K].e[e];
 #pragma unroll
-    for (int m = 1; m <= K; m++) lap = fmaT(a.wr[m - 1], t[-m * lw + e] + t[m * lw + e], lap);
+    for (int m = 1; m <= K; m++)
 #pragma unroll
-    for (int m = 1; m <= K; m++) lap = fmaT(a.wcol[m - 1], t[e - m] + t[e + m], lap);
-    return lap;
+        for (int e = 0; e < 4; e++) lap[e] = fmaT(a.wp[m - 1], win[K - m].e[e] + win[K + m].e[e], lap[e]);
+#pragma unroll
+    for (int m = 1; m <= K; m++) {
+        const C4<T> up = lds_chunk(t - m * lw), dn = lds_chunk(t + m * lw);
+#pragma unroll
+        for (int e = 0; e < 4; e++) lap[e] = fmaT(a.wr[m - 1], up.e[e] + dn.e[e], lap[e]);
+    }
+    const C4<T> L = lds_chunk(t - 4), C = lds_chunk(t), R = lds_chunk(t + 4);
+#pragma unroll
+    for (int m = 1; m <= K; m++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) lap[e] = fmaT(a.wcol[m - 1], pick(L, C, R, e - m) + pick(L, C, R, e + m), lap[e]);
 }
 
 // Which of the planes pk .. pk + 63 (lane l: plane pk + l, within [lo, hi)) have sources in CSR
@@ -67,8 +92,8 @@ template <typename T, int K, bool VARC, bool SPONGE>
 __global__ __launch_bounds__(T32_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void step3d_t2_kernel(const StepArgs<T> a, int rk, int nkb)
 {
     constexpr int W = 2 * K + 1;
-    __shared__ T stage[2][T32_SH][T32_SW];
-    __shared__ T ring[K + 1][T32_ER][T32_EC];
+    __shared__ __align__(16) T stage[2][T32_SH][T32_SW];  // 16-byte chunk reads (lds_chunk)
+    __shared__ __align__(16) T ring[K + 1][T32_ER][T32_EC];
     const StepBlock bp = step_block(a.ngb, a.gx, a.gy);
     const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
     if (bp.gather) {  // first half: u^n into row step + gather_off; second half: u^{n-1} into the row before
@@ -202,11 +227,12 @@ __global__ __launch_bounds__(T32_THREADS) __attribute__((amdgpu_waves_per_eu(3))
         }
         T val[4];
         const T* tc = st + (ty + HALO) * T32_SW + HALO + 4 * tx;
+        T lap1[4];
+        t32_lap4<T, K>(a, w1, tc, T32_SW, lap1);
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            const T lap = t32_lap<T, K>(a, w1, tc, T32_SW, e);
             const T t = damp ? T(2) * w1[K].e[e] - (T(1) - b[e]) * P.e[e] : T(2) * w1[K].e[e] - P.e[e];
-            val[e] = fmaT(kc[e], lap, t);
+            val[e] = fmaT(kc[e], lap1[e], t);
         }
         if (a.src.count && plane1) {
             if (q >= m1k + 64) {
@@ -250,12 +276,13 @@ __global__ __launch_bounds__(T32_THREADS) __attribute__((amdgpu_waves_per_eu(3))
         }
         T val2[4];
         const T* rc = &ring[p % (K + 1)][ty][4 * tx];
+        T lap2[4];
+        t32_lap4<T, K>(a, w2, rc, T32_EC, lap2);
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            const T lap = t32_lap<T, K>(a, w2, rc, T32_EC, e);
             // u_prev of step 2 is u^n at plane p: the oldest plane of the u^n window
             const T t = damp2 ? T(2) * w2[K].e[e] - (T(1) - b2[e]) * w1[0].e[e] : T(2) * w2[K].e[e] - w1[0].e[e];
-            val2[e] = fmaT(kc2[e], lap, t);
+            val2[e] = fmaT(kc2[e], lap2[e], t);
         }
         if (a.src.count) {
             if (p >= m2k + 64) {
