@@ -225,6 +225,8 @@ def test_launch_config_rules_need_no_gpu():
     assert cfg(small)[0] & 2 == 0 and cfg(big)[0] & 2 == 2 and cfg(big64)[0] & 2 == 2
     assert cfg(batch)[0] & 2 == 2  # 16 shots of 2048^2: 16 x 4 fields x 17 MB
     assert cfg(fwi)[0] & 2 == 0  # 16 shots of 1024^2: 279 MB, in good part cache-resident
+    # several shots take their tiles shot-fastest (WFD_VARIANT_SHOTS = 4) unless tuned otherwise
+    assert cfg(fwi)[0] & 4 == 4 and cfg(batch)[0] & 4 == 4 and cfg(small)[0] & 4 == 0 and cfg(cube)[0] & 4 == 0
     assert cfg(cube)[0] == 2 and cfg(small3)[0] == 0
     for variant in range(8):  # bit 2: shot-fastest tile order (WFD_VARIANT_SHOTS)
         assert lib.wfd_set_config(ctypes.byref(small), variant, 4) == 0
