@@ -230,14 +230,14 @@ LaunchCfg default_cfg(const wfd_grid* g)
 // 256 CUs (832 us per adjoint step, against 892 us at 64 planes and 1094 us at 170).
 int img_depth(const wfd_grid* g)
 {
-    static std::atomic<int> cus{0};
-    int ncu = cus.load(std::memory_order_relaxed);
-    if (ncu == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
-            ncu = 256;
-        cus.store(ncu, std::memory_order_relaxed);
+    constexpr int kDevs = 64;
+    static std::atomic<int> cus[kDevs];  // CUs of each device, 0 until looked up
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) dev = 0;
+    if (dev < kDevs) ncu = cus[dev].load(std::memory_order_relaxed);
+    if (ncu <= 0) {
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
+        if (dev < kDevs) cus[dev].store(ncu, std::memory_order_relaxed);
     }
     const int64_t per = ((g->n[2] + 63) / 64) * ((g->n[1] - 2 + 15) / 16) * (int64_t)g->nshots;  // tiles per z-block
     const int64_t slots = (int64_t)ncu * (g->dtype == WFD_F64 ? 1 : 2);
