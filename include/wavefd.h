@@ -292,7 +292,7 @@ typedef struct wfd_batch_shot {
 } wfd_batch_shot;
 
 typedef struct wfd_batch_opts {
-    int64_t max_active_cells; /* 0 = 12 Mi cells (about 12 shots of 1024^2) */
+    int64_t max_active_cells; /* 0 = 16 Mi cells (about 16 shots of 1024^2) */
     int32_t blocks_per_cu;    /* persistent grid: resident blocks per CU, 0 = occupancy limit */
     int32_t mode;             /* 0 = one persistent launch; 1 = one launch per phase */
     int32_t timeout_ms;       /* bound of each grid-barrier wait, 0 = 2000 */

@@ -676,7 +676,9 @@ int make_schedule(wfd_batch* b, const std::vector<std::vector<HostOp>>& ops, Sch
             load[g] += work(s);
         }
     }
-    const int64_t total_target = b->opts.max_active_cells > 0 ? b->opts.max_active_cells : (int64_t)12 * 1024 * 1024;
+    // default: 16 Mi cells (64-shot batch of 512^2-1536^2 grids on one box: 12 Mi 3.260 s, 16 Mi
+    // 3.218 s, 20 Mi 3.213 s, 24 Mi 3.337 s -- beyond that the shots in flight leave the cache)
+    const int64_t total_target = b->opts.max_active_cells > 0 ? b->opts.max_active_cells : (int64_t)16 * 1024 * 1024;
     const int64_t target = std::max<int64_t>(1, total_target / G);
     std::vector<int64_t> local(S, 0), gphases(G, 0);
     for (int g = 0; g < G; g++) {
