@@ -266,6 +266,7 @@ struct Launch {
     int64_t gather_off;
     void* un2 = nullptr;  // two-step launch: u^{n+2} (un receives u^{n+1})
     int t2rows = 16;
+    bool img_group = false;  // 2D imaging of several shots: one partial per IMG_GROUP shots
 };
 
 template <typename T>
@@ -303,6 +304,7 @@ int launch_t(const Launch& L, hipStream_t s)
     if (L.img) {
         a.uf = static_cast<const T*>(L.uf) + off;
         a.g = static_cast<T*>(L.gpart) + off;
+        a.img_group = L.img_group && g->ndim == 2 && g->nshots > 1 ? IMG_GROUP : 0;
     }
     a.shot_stride = g->elems;
     a.ld = g->ld;
@@ -605,9 +607,10 @@ int wfd::check_medium_args(const wfd_grid* g, const wfd_medium* med)
 
 int wfd::step_launch(const wfd_grid* g, const void* up, const void* uc, void* un, const wfd_medium* med,
                      const wfd_points* src, const wfd_points* rec, int64_t step, int64_t gather_off, const void* uf,
-                     void* gpart, bool img, hipStream_t s)
+                     void* gpart, bool img, hipStream_t s, bool img_group)
 {
     Launch L{g, up, uc, un, med, src, rec, step, nullptr, uf, gpart, img, default_cfg(g), gather_off};
+    L.img_group = img_group;
     return do_launch(L, s);
 }
 

@@ -62,6 +62,10 @@ struct StepArgs {
     uint32_t gx, gy;     // stencil block grid (x fastest); the launch is 1D: ngb + gx*gy*gz blocks
     int32_t nshots;
     int32_t shot_fast;  // 2D step launches: tile index -> (shot fastest, then column, then row block)
+    // 2D imaging launches of several shots: IMG_GROUP = the 4 waves of a tile take the same rows of
+    // 4 consecutive shots, and shot group j's imaging partial (field j of a.g) accumulates their
+    // terms in shot order; 0 = one partial per shot
+    int32_t img_group;
     // Two-step launches: un receives u^{n+1}, un2 u^{n+2}; their gather blocks record up (u^{n-1})
     // and uc (u^n) into trace rows step + gather_off - 1 and step + gather_off.
     T* un2;
@@ -82,6 +86,8 @@ __device__ __forceinline__ void lds_barrier()
 // s_waitcnt immediate (gfx9 encoding) that waits for every outstanding vector memory access of
 // the wave and for nothing else: vmcnt 0, expcnt 7, lgkmcnt 15.
 constexpr int WAIT_VMCNT0 = 0x0F70;
+
+constexpr int IMG_GROUP = 4;  // shots per imaging partial in grouped imaging launches
 
 // Position of this block in a step launch: a gather block or a stencil tile.  The hardware hands
 // block b to XCD b % 8 (each XCD has its own L2) and starts an XCD's blocks in order b / 8.  The
@@ -188,7 +194,9 @@ int check_medium_args(const wfd_grid* g, const wfd_medium* med);
 // public entry points and the gradient plan.
 int step_launch(const wfd_grid* g, const void* up, const void* uc, void* un, const wfd_medium* med,
                 const wfd_points* src, const wfd_points* rec, int64_t step, int64_t gather_off, const void* uf,
-                void* gpart, bool img, hipStream_t s);
+                void* gpart, bool img, hipStream_t s, bool img_group = false);
+// Shot groups of a grouped imaging launch on g (0: one partial per shot).
+inline int img_groups(const wfd_grid* g) { return g->ndim == 2 && g->nshots > 1 ? (g->nshots + IMG_GROUP - 1) / IMG_GROUP : 0; }
 int gather_launch(const wfd_grid* g, const void* field, const wfd_points* rec, int64_t row, hipStream_t s);
 
 }  // namespace wfd

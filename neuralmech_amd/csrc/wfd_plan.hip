@@ -10,7 +10,9 @@
 //             segment slots (s1 = min(kC+C, N)); then adjoint steps m = s1..kC+1 read u^{m-1}.
 // The imaging partial (grid dtype) is flushed into the float64 gradient whenever
 // (m-1) % flush == 0, independently of C, so the gradient bits do not depend on the
-// checkpoint spacing.
+// checkpoint spacing.  2D grids of several shots keep one partial per group of IMG_GROUP shots
+// (the imaging launch adds the group's terms in shot order; the gradient is their sum anyway),
+// flushed in group order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,6 +43,17 @@ struct wfd_plan {
 namespace {
 
 int plan_fail(int code, const char* msg) { return wfd::set_error(code, msg); }
+
+// G += the imaging partials (one per shot, or one per shot group), which are cleared
+int flush_partials(const wfd_plan* p, double* G, hipStream_t st)
+{
+    const int groups = wfd::img_groups(&p->g);
+    const int parts = groups ? groups : p->g.nshots;
+    const hipError_t e = p->g.dtype == WFD_F64
+                             ? wfd::launch_flush<double>((double*)p->gpart, G, p->g.elems, parts, st)
+                             : wfd::launch_flush<float>((float*)p->gpart, G, p->g.elems, parts, st);
+    return e == hipSuccess ? WFD_OK : plan_fail(WFD_EHIP, "wfd_plan_backward: flush kernel failed");
+}
 
 size_t fields_needed(int64_t N, int64_t C)
 {
@@ -241,11 +254,11 @@ int wfd_plan_backward(wfd_plan* p, const wfd_medium* med, const wfd_points* src,
             // adjoint step m (signal row m-1) makes mu^m; its launch records mu^{m+1} (its current
             // level) into row m of src_rec
             int rc = wfd::step_launch(&p->g, muP, muC, muP, med, adj, src_rec, m - 1, +1, field_at(p, m - 1, s0, k),
-                                      p->gpart, true, st);
+                                      p->gpart, true, st, true);
             if (rc) return rc;
             std::swap(muP, muC);  // muC = mu^m, muP = mu^{m+1}
             if ((m - 1) % p->flush == 0) {
-                rc = wfd_grad_flush(&p->g, p->gpart, G, stream);
+                rc = flush_partials(p, G, st);
                 if (rc) return rc;
             }
         }

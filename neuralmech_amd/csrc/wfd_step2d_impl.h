@@ -144,13 +144,25 @@ __device__ __forceinline__ T lap_elem(const T wc, const T* wr, const T* wcol, co
     return lap;
 }
 
+// Grouped imaging: the imaging terms (new adjoint value, Laplacian of the forward field) of each
+// wave's rows, exchanged through LDS so that one wave sums the group's shots in shot order.
+template <typename T, int RI>
+__device__ __forceinline__ C4<T> (&img_exchange())[2][IMG_GROUP][RI][64]
+{
+    __shared__ C4<T> x[2][IMG_GROUP][RI][64];
+    return x;
+}
+
 // One stencil tile of both variants: column block tx, row block ty of shot `shot`.  XL selects
 // the lane-exchange halo; VROW = false is the 1D case (a single row, no vertical stencil).  LA is
 // how many rows ahead the per-row operands are loaded (see the row loop); NT selects streaming
-// access to u_prev, kc and u_next.
+// access to u_prev, kc and u_next.  Grouped imaging launches (a.img_group, 2D) pass the shot group
+// as `shot`: wave w takes shot IMG_GROUP * group + w on the tile's RI rows, and after the march
+// the group's partial row q is updated by wave q % 4 as p = fma(mu_s, L u_s, p) for s in shot
+// order (only the order of the float additions differs from one partial per shot).
 template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, int LA = 0,
-          bool NT = false>
-__device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, uint32_t ty, int shot, int64_t step)
+          bool NT = false, bool GRP = false>
+__device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, uint32_t ty, int shot_in, int64_t step)
 {
     constexpr int KR = VROW ? K : 0;
     constexpr int NW = RI + 2 * KR;
@@ -159,8 +171,13 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
     const int wave = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) & 3);  // wave of the 4-wave tile
     const int64_t base = (int64_t)tx * COLS - (XL ? 4 : 0);  // column of lane 0's chunk
     const int64_t c = base + lane * 4;
-    const int64_t r0 = a.row_lo + ((int64_t)ty * 4 + wave) * RI;
-    if (r0 >= a.row_hi) return;  // whole wave
+    constexpr bool grp = GRP && IMG && VROW;  // grouped imaging instance (a.img_group > 1)
+    // grouped: a wave of a shot past the last one marches the group's first shot and stores nothing
+    const int gshot = grp ? shot_in * IMG_GROUP + wave : shot_in;
+    const bool live = !grp || gshot < a.nshots;
+    const int shot = live ? gshot : shot_in * IMG_GROUP;
+    const int64_t r0 = a.row_lo + (grp ? (int64_t)ty * RI : ((int64_t)ty * 4 + wave) * RI);
+    if (r0 >= a.row_hi) return;  // whole wave (grouped: the whole tile, its waves share the rows)
     const int lane_lo = XL ? 1 : 0, lane_hi = XL ? 62 : 63;
     const bool load_ok = c < a.ld;  // no wrap of a lane's chunk into the next row
     const bool own = lane >= lane_lo && lane <= lane_hi && c < a.nc;
@@ -220,7 +237,7 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
         const int64_t o = r * a.ld + c;
         Pv[q] = ld ? sld4<NT>(up + o) : zero4<T>();
         if constexpr (VARC) Kv[q] = ld ? sld4<NT>(coef + o) : zero4<T>();
-        if constexpr (IMG) Gv[q] = ld ? ld4(a.g + sb + o) : zero4<T>();
+        if constexpr (IMG) Gv[q] = ld && !grp ? ld4(a.g + sb + o) : zero4<T>();
         if constexpr (SPONGE) Bv[q] = VROW ? a.beta_r[r] : T(0);
         if constexpr (!XL) {
             Lv[q] = load_ok ? ld4(uc + o - 4) : zero4<T>();
@@ -291,9 +308,16 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
             for (int e = 0; e < 4; e++) gv[e] = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, wf, q, Lf, Rf, e);
         }
         if constexpr (LA > 0) __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
-        if (own) sst4<NT>(un + o, val);
+        if (own && live) sst4<NT>(un + o, val);
         if constexpr (IMG) {
-            if (own) {
+            if constexpr (grp) {
+                auto& xch = img_exchange<T, RI>();
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    xch[0][wave][q][lane].e[e] = val[e];
+                    xch[1][wave][q][lane].e[e] = gv[e];
+                }
+            } else if (own) {
                 T* gp = a.g + sb;
                 const C4<T>& G = Gv[IMG ? q : 0];
 #pragma unroll
@@ -310,13 +334,50 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
             }
         }
     }
+    if constexpr (IMG) {
+        if constexpr (grp) {
+            // every wave of the tile reaches this point (same rows, same loop); the group's partial
+            // row q is updated by wave q % 4 from the four waves' terms, in shot order
+            auto& xch = img_exchange<T, RI>();
+            const int nlive = min(IMG_GROUP, a.nshots - shot_in * IMG_GROUP);
+            T* gp = a.g + (int64_t)shot_in * a.shot_stride;
+            C4<T> G[(RI + 3) / 4];
+#pragma unroll
+            for (int j = 0; j < (RI + 3) / 4; j++) {
+                const int q = wave + 4 * j;
+                const int64_t r = r0 + q;
+                G[j] = q < RI && r < a.row_hi && own ? ld4(gp + r * a.ld + c) : zero4<T>();
+            }
+            lds_barrier();
+#pragma unroll
+            for (int j = 0; j < (RI + 3) / 4; j++) {
+                const int q = wave + 4 * j;
+                const int64_t r = r0 + q;
+                if (q >= RI || r >= a.row_hi) break;  // whole wave
+                T p[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) p[e] = G[j].e[e];
+                for (int s = 0; s < nlive; s++) {
+                    const C4<T> v = xch[0][s][q][lane], l = xch[1][s][q][lane];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) p[e] = fmaT(v.e[e], l.e[e], p[e]);
+                }
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    if (!valid[e]) p[e] = T(0);
+                if (own) st4(gp + r * a.ld + c, p);
+            }
+        }
+    }
 }
 
 // A step launch: gather blocks copy receiver values, the other blocks update one tile each.
-template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, bool NT>
+template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, bool NT, bool GRP = false>
 __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
 {
-    const StepBlock bp = step_block(a.ngb, a.gx, a.gy, (uint32_t)a.nshots, a.shot_fast != 0);
+    // grouped imaging launches have one tile column per shot group instead of per shot
+    const uint32_t nz = GRP ? (uint32_t)((a.nshots + IMG_GROUP - 1) / IMG_GROUP) : (uint32_t)a.nshots;
+    const StepBlock bp = step_block(a.ngb, a.gx, a.gy, nz, a.shot_fast != 0);
     const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
     if (bp.gather) {
         gather_points(a, bp.g, step);
@@ -326,36 +387,39 @@ __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
     // registers cost a wave per SIMD (2 and 8 rows keep their 3 and 2 waves).  Loading a wave's
     // operands up front (LA = RI - 1) measured slower for lack of waves.
     constexpr int LA = RI > 1 && !(IMG && RI == 4) ? 1 : 0;
-    step2d_tile<T, K, VROW, VARC, SPONGE, IMG, RI, XL, LA, NT>(a, bp.x, bp.y, (int)bp.z, step);
+    step2d_tile<T, K, VROW, VARC, SPONGE, IMG, RI, XL, LA, NT, GRP>(a, bp.x, bp.y, (int)bp.z, step);
 }
 
-template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool NT>
+template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool NT, bool GRP = false>
 __global__ __launch_bounds__(256) void step2d_kernel(const StepArgs<T> a)
 {
-    step2d_body<T, K, VROW, VARC, SPONGE, IMG, RI, false, NT>(a);
+    step2d_body<T, K, VROW, VARC, SPONGE, IMG, RI, false, NT, GRP>(a);
 }
 
 constexpr int XL_COLS = 248;
 
-template <typename T, int K, bool VARC, bool SPONGE, bool IMG, int RI, bool NT>
+template <typename T, int K, bool VARC, bool SPONGE, bool IMG, int RI, bool NT, bool GRP = false>
 __global__ __launch_bounds__(256) void step2d_xl_kernel(const StepArgs<T> a)
 {
-    step2d_body<T, K, true, VARC, SPONGE, IMG, RI, true, NT>(a);
+    step2d_body<T, K, true, VARC, SPONGE, IMG, RI, true, NT, GRP>(a);
 }
 
 // ------------------------------------------------------------------------------ dispatch
 // variant bit 0: lane-exchange halo (XL); bit 1: streaming access (NT)
-template <typename T, int K, bool VARC, bool SPONGE, bool IMG, bool NT>
+template <typename T, int K, bool VARC, bool SPONGE, bool IMG, bool NT, bool GRP = false>
 static hipError_t launch2d_kn(const StepArgs<T>& a0, bool vrow, int variant, int ri, int nshots, hipStream_t s)
 {
     const dim3 block(256);
     const int64_t rows = a0.row_hi - a0.row_lo;
     StepArgs<T> a = a0;
-    // 1D launch: a.ngb gather blocks, then gx * gy * nshots stencil tiles
+    // 1D launch: a.ngb gather blocks, then gx * gy * nshots stencil tiles (grouped imaging: tiles of
+    // r rows, one per shot group)
+    constexpr bool grp = GRP;  // grouped imaging instance (2D only)
     auto grid = [&](int64_t cols, int r) {
         a.gx = (uint32_t)((a.nc + cols - 1) / cols);
-        a.gy = vrow ? (uint32_t)((rows + 4 * r - 1) / (4 * r)) : 1u;
-        return dim3(step_grid((uint32_t)((int64_t)a.gx * a.gy * nshots), a.ngb));
+        a.gy = vrow ? (uint32_t)(grp ? (rows + r - 1) / r : (rows + 4 * r - 1) / (4 * r)) : 1u;
+        const int64_t nz = grp ? (nshots + IMG_GROUP - 1) / IMG_GROUP : nshots;
+        return dim3(step_grid((uint32_t)((int64_t)a.gx * a.gy * nz), a.ngb));
     };
     if (!vrow) {
         const dim3 g = grid(256, 1);
@@ -365,27 +429,27 @@ static hipError_t launch2d_kn(const StepArgs<T>& a0, bool vrow, int variant, int
     if (variant & 1) {
         if constexpr (sizeof(T) == 4) {
             switch (ri) {
-                case 1: { const dim3 g = grid(XL_COLS, 1); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 1, NT>), g, block, 0, s, a); break; }
-                case 2: { const dim3 g = grid(XL_COLS, 2); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 2, NT>), g, block, 0, s, a); break; }
-                case 8: { const dim3 g = grid(XL_COLS, 8); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 8, NT>), g, block, 0, s, a); break; }
-                default: { const dim3 g = grid(XL_COLS, 4); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4, NT>), g, block, 0, s, a); break; }
+                case 1: { const dim3 g = grid(XL_COLS, 1); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 1, NT, GRP>), g, block, 0, s, a); break; }
+                case 2: { const dim3 g = grid(XL_COLS, 2); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 2, NT, GRP>), g, block, 0, s, a); break; }
+                case 8: { const dim3 g = grid(XL_COLS, 8); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 8, NT, GRP>), g, block, 0, s, a); break; }
+                default: { const dim3 g = grid(XL_COLS, 4); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4, NT, GRP>), g, block, 0, s, a); break; }
             }
         } else {
             const dim3 g = grid(XL_COLS, 4);
-            hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4, NT>), g, block, 0, s, a);
+            hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4, NT, GRP>), g, block, 0, s, a);
         }
         return hipGetLastError();
     }
     if constexpr (sizeof(T) == 4) {
         switch (ri) {
-            case 1: { const dim3 g = grid(256, 1); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 1, NT>), g, block, 0, s, a); break; }
-            case 2: { const dim3 g = grid(256, 2); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 2, NT>), g, block, 0, s, a); break; }
-            case 8: { const dim3 g = grid(256, 8); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 8, NT>), g, block, 0, s, a); break; }
-            default: { const dim3 g = grid(256, 4); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4, NT>), g, block, 0, s, a); break; }
+            case 1: { const dim3 g = grid(256, 1); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 1, NT, GRP>), g, block, 0, s, a); break; }
+            case 2: { const dim3 g = grid(256, 2); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 2, NT, GRP>), g, block, 0, s, a); break; }
+            case 8: { const dim3 g = grid(256, 8); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 8, NT, GRP>), g, block, 0, s, a); break; }
+            default: { const dim3 g = grid(256, 4); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4, NT, GRP>), g, block, 0, s, a); break; }
         }
     } else {
         const dim3 g = grid(256, 4);
-        hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4, NT>), g, block, 0, s, a);
+        hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4, NT, GRP>), g, block, 0, s, a);
     }
     return hipGetLastError();
 }
@@ -395,6 +459,14 @@ static hipError_t launch2d_flags(const StepArgs<T>& a, bool vrow, bool varc, boo
                                  int ri, int nshots, hipStream_t s)
 {
     const int f = (varc ? 4 : 0) | (sponge ? 2 : 0) | (img ? 1 : 0);
+    if (img && vrow && a.img_group > 1) {  // grouped imaging instances
+        switch (f) {
+            case 1: return launch2d_kn<T, K, false, false, true, NT, true>(a, vrow, variant, ri, nshots, s);
+            case 3: return launch2d_kn<T, K, false, true, true, NT, true>(a, vrow, variant, ri, nshots, s);
+            case 5: return launch2d_kn<T, K, true, false, true, NT, true>(a, vrow, variant, ri, nshots, s);
+            default: return launch2d_kn<T, K, true, true, true, NT, true>(a, vrow, variant, ri, nshots, s);
+        }
+    }
     switch (f) {
         case 0: return launch2d_kn<T, K, false, false, false, NT>(a, vrow, variant, ri, nshots, s);
         case 1: return launch2d_kn<T, K, false, false, true, NT>(a, vrow, variant, ri, nshots, s);

@@ -47,6 +47,15 @@ def lib():
             adj.argtypes = [ctypes.c_int, _i64p, _dp, ctypes.c_int, tp, ctypes.c_double, tp,
                             ctypes.c_int, _i64p, tp, ctypes.c_int, _i64p, tp, tp,
                             ctypes.c_int64, ctypes.c_int64, _dp, ctypes.c_int]
+            adjs = getattr(_lib, f"wfo_adjoint_shots_{sfx}")
+            adjs.restype = ctypes.c_int
+            adjs.argtypes = [ctypes.c_int, _i64p, _dp, ctypes.c_int, tp, ctypes.c_double, tp,
+                             ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64p, ctypes.c_int, tp,
+                             ctypes.c_int, _i64p, ctypes.c_int, tp, tp,
+                             ctypes.c_int64, ctypes.c_int64, _dp, tp, ctypes.c_int]
+            lap = getattr(_lib, f"wfo_lap_{sfx}")
+            lap.restype = ctypes.c_int
+            lap.argtypes = [ctypes.c_int, _i64p, _dp, ctypes.c_int, tp, tp]
             w = getattr(_lib, f"wfo_weights_{sfx}")
             w.restype = ctypes.c_int
             w.argtypes = [ctypes.c_int, _dp, ctypes.c_int, tp, tp]
@@ -174,6 +183,67 @@ def adjoint(kc, order, h, nsteps, adj_sig, u_hist, *, seg, kconst=0.0, beta_prof
     if rc:
         raise RuntimeError(f"oracle adjoint failed ({rc})")
     return G, mu_src[:, : len(src_nodes)]
+
+
+def adjoint_shots(kc, order, h, nsteps, adj_sig, u_hist, *, seg, group, beta_profiles=None,
+                  rec_nodes=(), src_nodes=(), keep_mu=False, nthreads=None):
+    """Adjoint sweeps of ``S`` shots over one medium; returns ``(G float64, mu_src[S, nsteps, P])``,
+    plus ``mu[S, nsteps, ...]`` (``mu[s, m-1] = mu_s^m``) when ``keep_mu``.
+
+    ``adj_sig[s, n, r]`` and ``u_hist[s, n] = u_s^n`` per shot.  ``src_nodes`` / ``rec_nodes`` are
+    one node list for every shot, or ``S`` lists (one per shot).  The imaging partials are one per
+    group of ``group`` consecutive shots, each summing its shots' terms in shot order at every
+    step, flushed into ``G`` in group order whenever ``(m-1) % seg == 0``.
+    """
+    kc = np.ascontiguousarray(kc)
+    dtype = kc.dtype
+    ct = _ct(dtype)
+    shape = kc.shape
+    sig = np.ascontiguousarray(adj_sig, dtype)
+    hist = np.ascontiguousarray(u_hist, dtype)
+    nshots = sig.shape[0]
+    assert sig.shape[1] == nsteps and hist.shape == (nshots, nsteps) + shape
+
+    def node_list(nodes):
+        per_shot = len(nodes) == nshots and len(nodes) > 0 and isinstance(nodes[0], (list, tuple)) \
+            and len(nodes[0]) > 0 and isinstance(nodes[0][0], (list, tuple))
+        lists = list(nodes) if per_shot else [nodes]
+        cnt = len(lists[0])
+        flat = np.concatenate([_nodes(l, shape)[: max(cnt, 1)] for l in lists]) if cnt else np.zeros(1, np.int64)
+        return np.ascontiguousarray(flat, np.int64), cnt, int(per_shot)
+
+    src, nsrc, src_ps = node_list(src_nodes)
+    rec, nrec, rec_ps = node_list(rec_nodes)
+    assert sig.shape[2] == nrec
+    beta = None
+    if beta_profiles is not None:
+        beta = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype) for p in beta_profiles]))
+    mu_src = np.zeros((nshots, nsteps, max(nsrc, 1)), dtype)
+    G = np.zeros(shape, np.float64)
+    mu = np.zeros((nshots, nsteps) + shape, dtype) if keep_mu else None
+    nthreads = nthreads or os.cpu_count() or 1
+    rc = getattr(lib(), f"wfo_adjoint_shots_{_sfx(dtype)}")(
+        len(shape), _ptr(np.asarray(shape, np.int64), ctypes.c_int64),
+        _ptr(np.ascontiguousarray(h, np.float64), ctypes.c_double), order, _ptr(kc, ct), 0.0,
+        _ptr(beta, ct), nshots, group, nrec, _ptr(rec, ctypes.c_int64), rec_ps, _ptr(sig, ct),
+        nsrc, _ptr(src, ctypes.c_int64), src_ps, _ptr(mu_src, ct), _ptr(hist, ct), nsteps, seg,
+        _ptr(G, ctypes.c_double), _ptr(mu, ct), nthreads)
+    if rc:
+        raise RuntimeError(f"oracle adjoint failed ({rc})")
+    return (G, mu_src[:, :, :nsrc], mu) if keep_mu else (G, mu_src[:, :, :nsrc])
+
+
+def lap(u, order, h):
+    """``L u`` at the interior nodes exactly as the oracle's imaging sum computes it."""
+    u = np.ascontiguousarray(u)
+    out = np.zeros_like(u)
+    ct = _ct(u.dtype)
+    rc = getattr(lib(), f"wfo_lap_{_sfx(u.dtype)}")(
+        u.ndim, _ptr(np.asarray(u.shape, np.int64), ctypes.c_int64),
+        _ptr(np.ascontiguousarray(h, np.float64), ctypes.c_double), order, _ptr(u, ct), _ptr(out, ct))
+    if rc:
+        raise RuntimeError(f"oracle lap failed ({rc})")
+    return out
 
 
 def history(kc, order, h, nsteps, **kw):

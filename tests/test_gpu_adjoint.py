@@ -256,3 +256,58 @@ def test_shot_fastest_tile_order_bitexact(variant, rows):
     (tr0, g0), (tr1, g1) = out[variant & 3], out[variant]
     assert torch.isfinite(g0).all() and g0.abs().max() > 0
     assert torch.equal(tr0, tr1) and torch.equal(g0, g1)
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+@pytest.mark.parametrize("shape,order,nshots,group", [((33, 51), 8, 6, 4), ((40, 36), 4, 3, 4),
+                                                      ((29, 130), 8, 4, 4), ((12, 20, 23), 8, 3, 1),
+                                                      ((60,), 2, 3, 1)])
+def test_multishot_imaging_sum_bitexact_vs_oracle(shape, order, nshots, group, dtype):
+    """Gradient plan over several shots in one grid: every shot's traces and adjoint source
+    records, and the imaging sum of all shots, bit-exact against the oracle's multi-shot adjoint.
+    2D grids keep one partial per group of 4 shots (terms added in shot order, the last group
+    with fewer live shots when 4 does not divide the count, under every launch shape); 1D and 3D
+    one per shot."""
+    from neuralmech_amd import solver
+
+    print(f"\n[adjoint shots] shape={shape} order={order} shots={nshots} {dtype}", flush=True)
+    npdt, tdt = (np.float32, torch.float32) if dtype == "f32" else (np.float64, torch.float64)
+    N = 37
+    case = make_case(shape, order, seed=43, nsteps=N, nsrc=1, nrec=4)
+    rng = np.random.default_rng(44)
+    srcs = [[tuple(int(rng.integers(1, n - 1)) for n in shape)] for _ in range(nshots)]
+    sig = rng.standard_normal((nshots, N, 1))
+    obs = rng.standard_normal((nshots, N, 4)) * 1e-3
+    g, med, _, _ = gpu_objects(case, tdt, nshots=nshots)
+    src = solver.PointSet(g, srcs, per_shot=True)
+    rec = solver.PointSet(g, case["rec"])
+    plan = solver.GradientPlan(g, N, seg=5)
+    tr = plan.forward(med, src, torch.as_tensor(sig), rec)
+    resid = tr - torch.as_tensor(obs, device=tr.device, dtype=tr.dtype)
+    plan.backward(med, src, rec, resid)
+    torch.cuda.synchronize()
+    kc = case["kc"].astype(npdt)
+    hist = []
+    for s in range(nshots):
+        tr_o, h_s, _, _ = c_oracle.history(kc, order, case["h"], N, beta_profiles=case["prof"], src_nodes=srcs[s],
+                                           src_sig=sig[s].astype(npdt), rec_nodes=case["rec"])
+        assert np.array_equal(tr[s].cpu().numpy(), tr_o)
+        hist.append(h_s)
+    G, mu_src = c_oracle.adjoint_shots(kc, order, case["h"], N, -resid.cpu().numpy(), np.stack(hist), seg=32,
+                                       group=group, beta_profiles=case["prof"], rec_nodes=case["rec"],
+                                       src_nodes=srcs)
+    assert np.array_equal(plan.last_mu_src.cpu().numpy(), mu_src)
+    assert np.array_equal(plan.last_G.cpu().numpy(), G)
+    if len(shape) != 2:
+        return
+    shapes = [(0, 2), (3, 8), (6, 4)] if dtype == "f32" else [(0, 4), (3, 4), (6, 4)]  # f64 marches 4 rows
+    for v, rows in shapes:  # every 2D launch shape groups the same way
+        g.set_launch(v, rows)
+        try:
+            plan = solver.GradientPlan(g, N, seg=5)
+            plan.forward(med, src, torch.as_tensor(sig), rec)
+            plan.backward(med, src, rec, resid)
+            torch.cuda.synchronize()
+        finally:
+            g.set_launch(0, 0)
+        assert np.array_equal(plan.last_G.cpu().numpy(), G), (v, rows)

@@ -345,3 +345,82 @@ def test_golden_ricker256_order2_fixture():
     assert rel_l2(tr[::10], gold["traces_every_10"]) < 1e-12
     assert rel_l2(uc.sum(1), gold["row_sums"]) < 1e-12 and rel_l2(uc.sum(0), gold["col_sums"]) < 1e-12
     assert abs(np.linalg.norm(uc) / float(gold["field_l2"]) - 1) < 1e-12
+
+
+def _round_f32(v: F) -> np.float32:
+    """The float32 nearest to the exact rational ``v`` (ties to even)."""
+    x = np.float32(float(v))
+    cands = [np.nextafter(x, np.float32(-np.inf)), x, np.nextafter(x, np.float32(np.inf))]
+    return min(cands, key=lambda c: (abs(F(float(c)) - v), int(np.asarray(c).view(np.uint32)) & 1))
+
+
+def _fma_exact(a, b, c, dtype):
+    v = F(float(a)) * F(float(b)) + F(float(c))
+    return _round_f32(v) if dtype == np.float32 else np.float64(float(v))  # int/int division rounds once
+
+
+def _imaging_exact(mu, lapu, seg, groups, dtype):
+    """G of the multi-shot imaging sum in exact arithmetic, one correctly rounded fma at a time:
+    per step m (descending) and shot group (a list of shot indices, in the order their terms are
+    added), p = fma(mu_s^m, L u_s^{m-1}, p) in ``dtype``; every (m-1) % seg == 0 the partials are
+    added into the float64 G in group order and cleared."""
+    nshots, nsteps = mu.shape[:2]
+    shape = mu.shape[2:]
+    G = np.zeros(shape, np.float64)
+    P = [np.zeros(shape, dtype) for _ in groups]
+    inner = list(np.ndindex(*[n - 2 for n in shape]))
+    for m in range(nsteps, 0, -1):
+        for q, grp in enumerate(groups):
+            for s in grp:
+                for idx in inner:
+                    x = tuple(i + 1 for i in idx)
+                    P[q][x] = _fma_exact(mu[s, m - 1][x], lapu[s, m - 1][x], P[q][x], dtype)
+        if (m - 1) % seg == 0:
+            for q in range(len(groups)):
+                G += P[q].astype(np.float64)
+                P[q][...] = 0
+    return G
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("group", [1, 2, 4])
+def test_multishot_imaging_groups_in_exact_arithmetic(dtype, group):
+    """The oracle's multi-shot adjoint: each shot's adjoint is the single-shot one, and the imaging
+    partials (one per group of consecutive shots, terms added in shot order, flushed in group
+    order) equal an exact-arithmetic accumulation of the same terms; the reversed shot order
+    inside a group gives other bits, so the check sees the order."""
+    case = make_case((10, 12), 4, seed=41, nsteps=9, nsrc=1, nrec=3)
+    S, N = 5, case["nsteps"]
+    rng = np.random.default_rng(17)
+    kc = case["kc"].astype(dtype)
+    srcs = [[(int(rng.integers(1, 9)), int(rng.integers(1, 11)))] for _ in range(S)]
+    sigs = rng.standard_normal((S, N, 1)).astype(dtype)
+    hist = np.stack([c_oracle.history(kc, 4, case["h"], N, beta_profiles=case["prof"], src_nodes=srcs[s],
+                                      src_sig=sigs[s], rec_nodes=case["rec"])[1] for s in range(S)])
+    adj = rng.standard_normal((S, N, 3)).astype(dtype)
+    G, mu_src, mu = c_oracle.adjoint_shots(kc, 4, case["h"], N, adj, hist, seg=4, group=group,
+                                           beta_profiles=case["prof"], rec_nodes=case["rec"], src_nodes=srcs,
+                                           keep_mu=True)
+    for s in range(S):
+        G1, m1 = c_oracle.adjoint(kc, 4, case["h"], N, adj[s], hist[s], seg=4, beta_profiles=case["prof"],
+                                  rec_nodes=case["rec"], src_nodes=srcs[s])
+        assert np.array_equal(mu_src[s], m1)
+    lapu = np.stack([np.stack([c_oracle.lap(hist[s, n], 4, case["h"]) for n in range(N)]) for s in range(S)])
+    groups = [list(range(q, min(q + group, S))) for q in range(0, S, group)]
+    assert np.array_equal(G, _imaging_exact(mu, lapu, 4, groups, dtype))
+    G_one, _ = c_oracle.adjoint_shots(kc, 4, case["h"], N, adj, hist, seg=4, group=1, beta_profiles=case["prof"],
+                                      rec_nodes=case["rec"], src_nodes=srcs)
+    assert rel_l2(G, G_one) < (1e-6 if dtype == np.float32 else 1e-14)
+    if group > 1:
+        assert not np.array_equal(G, _imaging_exact(mu, lapu, 4, [g[::-1] for g in groups], dtype))
+
+
+@pytest.mark.parametrize("order", [2, 8])
+def test_oracle_laplacian_export_matches_slice_restatement(order):
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((14, 17))
+    got = c_oracle.lap(u, order, (0.8, 0.9))
+    want = slice_ref.laplacian(u, order, (0.8, 0.9))
+    inner = (slice(1, -1), slice(1, -1))
+    assert np.allclose(got[inner], want[inner], rtol=1e-12, atol=1e-12)
+    assert not got[0].any() and not got[-1].any() and not got[:, 0].any() and not got[:, -1].any()
