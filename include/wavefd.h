@@ -254,7 +254,11 @@ int wfd_plan_forward(wfd_plan* plan, const wfd_medium* med, const wfd_points* sr
                      const wfd_points* rec, void* u_last, wfd_stream_t stream);
 /* Backward: recompute each segment from its checkpoint, run the adjoint with `adj` injected,
  * `src_rec` recording mu at the source nodes, and accumulate G = sum_n mu^{n+1} (L u^n)
- * (float64, one field summed over shots, overwritten). */
+ * (float64, one field summed over shots, overwritten).  The terms are summed in grid-dtype
+ * partials flushed into G every 32 adjoint steps (at (m-1) % 32 == 0); a 2D grid of several
+ * shots keeps one partial per group of 4 consecutive shots (terms added in shot order, groups
+ * flushed in order), other grids one per shot, so G is deterministic whatever the launch shape
+ * and checkpoint spacing. */
 int wfd_plan_backward(wfd_plan* plan, const wfd_medium* med, const wfd_points* src,
                       const wfd_points* adj, const wfd_points* src_rec, double* G,
                       wfd_stream_t stream);
