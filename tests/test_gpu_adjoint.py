@@ -260,8 +260,8 @@ def test_shot_fastest_tile_order_bitexact(variant, rows):
 
 @pytest.mark.parametrize("dtype", ["f32", "f64"])
 @pytest.mark.parametrize("shape,order,nshots,group", [((33, 51), 8, 6, 4), ((40, 36), 4, 3, 4),
-                                                      ((29, 130), 8, 4, 4), ((12, 20, 23), 8, 3, 1),
-                                                      ((60,), 2, 3, 1)])
+                                                      ((29, 130), 8, 4, 4), ((22, 300), 6, 9, 4),
+                                                      ((12, 20, 23), 8, 3, 1), ((60,), 2, 3, 1)])
 def test_multishot_imaging_sum_bitexact_vs_oracle(shape, order, nshots, group, dtype):
     """Gradient plan over several shots in one grid: every shot's traces and adjoint source
     records, and the imaging sum of all shots, bit-exact against the oracle's multi-shot adjoint.
