@@ -267,6 +267,7 @@ struct Launch {
     void* un2 = nullptr;  // two-step launch: u^{n+2} (un receives u^{n+1})
     int t2rows = 16;
     bool img_group = false;  // 2D imaging of several shots: one partial per IMG_GROUP shots
+    void* ucopy = nullptr;   // 2D forward: second copy of u_next (see StepArgs::ucopy)
 };
 
 template <typename T>
@@ -279,6 +280,7 @@ int launch_t(const Launch& L, hipStream_t s)
     a.up = L.up ? static_cast<const T*>(L.up) + off : nullptr;
     a.uc = static_cast<const T*>(L.uc) + off;
     a.un = L.un ? static_cast<T*>(L.un) + off : nullptr;
+    a.ucopy = L.ucopy && !L.img && g->ndim == 2 ? static_cast<T*>(L.ucopy) + off : nullptr;
     const wfd_medium* m = L.med;
     const bool varc = m && m->coef;
     const bool sponge = m && m->beta;
@@ -607,10 +609,11 @@ int wfd::check_medium_args(const wfd_grid* g, const wfd_medium* med)
 
 int wfd::step_launch(const wfd_grid* g, const void* up, const void* uc, void* un, const wfd_medium* med,
                      const wfd_points* src, const wfd_points* rec, int64_t step, int64_t gather_off, const void* uf,
-                     void* gpart, bool img, hipStream_t s, bool img_group)
+                     void* gpart, bool img, hipStream_t s, bool img_group, void* ucopy)
 {
     Launch L{g, up, uc, un, med, src, rec, step, nullptr, uf, gpart, img, default_cfg(g), gather_off};
     L.img_group = img_group;
+    L.ucopy = ucopy;
     return do_launch(L, s);
 }
 

@@ -69,6 +69,9 @@ struct StepArgs {
     // Two-step launches: un receives u^{n+1}, un2 u^{n+2}; their gather blocks record up (u^{n-1})
     // and uc (u^n) into trace rows step + gather_off - 1 and step + gather_off.
     T* un2;
+    // 2D forward launches of the gradient plan: a second copy of u^{n+1} (the level kept for the
+    // adjoint), written with the nontemporal hint while un rotates in place with up
+    T* ucopy;
 };
 
 // Block barrier that orders LDS only: each wave waits for its own LDS accesses (lgkmcnt), then
@@ -194,7 +197,7 @@ int check_medium_args(const wfd_grid* g, const wfd_medium* med);
 // public entry points and the gradient plan.
 int step_launch(const wfd_grid* g, const void* up, const void* uc, void* un, const wfd_medium* med,
                 const wfd_points* src, const wfd_points* rec, int64_t step, int64_t gather_off, const void* uf,
-                void* gpart, bool img, hipStream_t s, bool img_group = false);
+                void* gpart, bool img, hipStream_t s, bool img_group = false, void* ucopy = nullptr);
 // Shot groups of a grouped imaging launch on g (0: one partial per shot).
 inline int img_groups(const wfd_grid* g) { return g->ndim == 2 && g->nshots > 1 ? (g->nshots + IMG_GROUP - 1) / IMG_GROUP : 0; }
 int gather_launch(const wfd_grid* g, const void* field, const wfd_points* rec, int64_t row, hipStream_t s);

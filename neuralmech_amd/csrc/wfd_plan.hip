@@ -30,6 +30,10 @@ extern "C" int wfd_adjoint_step(const wfd_grid* g, const void* mu_prev, const vo
 struct wfd_plan {
     wfd_grid g;
     int64_t nsteps = 0, seg = 0, nckpt = 0, flush = 32;
+    // 2D grids: the forward and the recompute run in the two work fields (the stencil's fields stay
+    // in place, as in a plain run) and store each kept level a second time with the nontemporal
+    // hint; other grids step into the kept fields directly
+    bool copy_out = false;
     size_t bytes = 0;
     size_t batch_bytes = 0;  // one field batch (nshots fields)
     char* mem = nullptr;
@@ -122,6 +126,7 @@ int wfd_plan_create(wfd_plan** out, const wfd_grid* g, int64_t nsteps, int64_t s
     p->g = *g;
     p->nsteps = nsteps;
     p->seg = seg;
+    p->copy_out = g->ndim == 2;
     p->nckpt = (nsteps + seg - 1) / seg;
     p->batch_bytes = batch;
     const size_t nf = fields_needed(nsteps, seg);
@@ -198,17 +203,21 @@ int wfd_plan_forward(wfd_plan* p, const wfd_medium* med, const wfd_points* src, 
     void* Cu = p->ck_cur[0];
     for (int64_t n = 0; n < N; n++) {
         const int64_t m = n + 1;
-        void* D = nullptr;
+        void* keep = nullptr;  // the checkpoint field that keeps u^m, if any
         if (C > 1 && m % C == C - 1 && (m + 1) / C < p->nckpt)
-            D = p->ck_prev[(m + 1) / C];
+            keep = p->ck_prev[(m + 1) / C];
         else if (m % C == 0 && m / C < p->nckpt)
-            D = p->ck_cur[m / C];
+            keep = p->ck_cur[m / C];
+        void* D = nullptr;
+        if (keep && !p->copy_out)
+            D = keep;
         else if (P == p->work[0] || P == p->work[1])
             D = P;  // in place over the oldest level
         else
             D = Cu == p->work[0] ? p->work[1] : p->work[0];
         // the launch also records u^n (its current level) into trace row n-1
-        rc = wfd::step_launch(&p->g, P, Cu, D, med, src, rec, n, -1, nullptr, nullptr, false, st);
+        rc = wfd::step_launch(&p->g, P, Cu, D, med, src, rec, n, -1, nullptr, nullptr, false, st, false,
+                              p->copy_out ? keep : nullptr);
         if (rc) return rc;
         P = Cu;
         Cu = D;
@@ -244,11 +253,27 @@ int wfd_plan_backward(wfd_plan* p, const wfd_medium* med, const wfd_points* src,
     void* muC = p->mu[1];  // mu^{m+1}
     for (int64_t k = p->nckpt - 1; k >= 0; k--) {
         const int64_t s0 = k * C, s1 = std::min(s0 + C, N);
-        for (int64_t n = s0; n + 1 < s1; n++) {
-            const void* prev = n == s0 ? p->ck_prev[k] : field_at(p, n - 1, s0, k);
-            int rc = wfd::step_launch(&p->g, prev, field_at(p, n, s0, k), field_at(p, n + 1, s0, k), med, src,
-                                      nullptr, n, 0, nullptr, nullptr, false, st);
-            if (rc) return rc;
+        if (p->copy_out) {
+            // recompute in the two work fields (in place, like the forward), each level also
+            // stored into its segment slot
+            const void* P = p->ck_prev[k];
+            const void* Cu = p->ck_cur[k];
+            for (int64_t n = s0; n + 1 < s1; n++) {
+                void* D = P == p->work[0] || P == p->work[1] ? const_cast<void*>(P)
+                                                              : (Cu == p->work[0] ? p->work[1] : p->work[0]);
+                int rc = wfd::step_launch(&p->g, P, Cu, D, med, src, nullptr, n, 0, nullptr, nullptr, false, st,
+                                          false, field_at(p, n + 1, s0, k));
+                if (rc) return rc;
+                P = Cu;
+                Cu = D;
+            }
+        } else {
+            for (int64_t n = s0; n + 1 < s1; n++) {
+                const void* prev = n == s0 ? p->ck_prev[k] : field_at(p, n - 1, s0, k);
+                int rc = wfd::step_launch(&p->g, prev, field_at(p, n, s0, k), field_at(p, n + 1, s0, k), med, src,
+                                          nullptr, n, 0, nullptr, nullptr, false, st);
+                if (rc) return rc;
+            }
         }
         for (int64_t m = s1; m > s0; m--) {
             // adjoint step m (signal row m-1) makes mu^m; its launch records mu^{m+1} (its current

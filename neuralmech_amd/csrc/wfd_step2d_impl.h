@@ -159,7 +159,9 @@ __device__ __forceinline__ C4<T> (&img_exchange())[2][IMG_GROUP][RI][64]
 // access to u_prev, kc and u_next.  Grouped imaging launches (a.img_group, 2D) pass the shot group
 // as `shot`: wave w takes shot IMG_GROUP * group + w on the tile's RI rows, and after the march
 // the group's partial row q is updated by wave q % 4 as p = fma(mu_s, L u_s, p) for s in shot
-// order (only the order of the float additions differs from one partial per shot).
+// order (only the order of the float additions differs from one partial per shot).  On a forward
+// instance (IMG false) GRP instead stores u_next a second time, into a.ucopy with the nontemporal
+// hint (the gradient plan's kept levels, while u_next rotates in place with u_prev).
 template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, int LA = 0,
           bool NT = false, bool GRP = false>
 __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, uint32_t ty, int shot_in, int64_t step)
@@ -172,6 +174,7 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
     const int64_t base = (int64_t)tx * COLS - (XL ? 4 : 0);  // column of lane 0's chunk
     const int64_t c = base + lane * 4;
     constexpr bool grp = GRP && IMG && VROW;  // grouped imaging instance (a.img_group > 1)
+    constexpr bool cpy = GRP && !IMG && VROW;  // forward instance storing a second copy (a.ucopy)
     // grouped: a wave of a shot past the last one marches the group's first shot and stores nothing
     const int gshot = grp ? shot_in * IMG_GROUP + wave : shot_in;
     const bool live = !grp || gshot < a.nshots;
@@ -308,7 +311,10 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
             for (int e = 0; e < 4; e++) gv[e] = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, wf, q, Lf, Rf, e);
         }
         if constexpr (LA > 0) __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
-        if (own && live) sst4<NT>(un + o, val);
+        if (own && live) {
+            sst4<NT>(un + o, val);
+            if constexpr (cpy) sst4<true>(a.ucopy + sb + o, val);
+        }
         if constexpr (IMG) {
             if constexpr (grp) {
                 auto& xch = img_exchange<T, RI>();
@@ -376,7 +382,7 @@ template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI
 __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
 {
     // grouped imaging launches have one tile column per shot group instead of per shot
-    const uint32_t nz = GRP ? (uint32_t)((a.nshots + IMG_GROUP - 1) / IMG_GROUP) : (uint32_t)a.nshots;
+    const uint32_t nz = GRP && IMG ? (uint32_t)((a.nshots + IMG_GROUP - 1) / IMG_GROUP) : (uint32_t)a.nshots;
     const StepBlock bp = step_block(a.ngb, a.gx, a.gy, nz, a.shot_fast != 0);
     const int64_t step = a.step_dev ? *a.step_dev + a.step : a.step;
     if (bp.gather) {
@@ -414,7 +420,7 @@ static hipError_t launch2d_kn(const StepArgs<T>& a0, bool vrow, int variant, int
     StepArgs<T> a = a0;
     // 1D launch: a.ngb gather blocks, then gx * gy * nshots stencil tiles (grouped imaging: tiles of
     // r rows, one per shot group)
-    constexpr bool grp = GRP;  // grouped imaging instance (2D only)
+    constexpr bool grp = GRP && IMG;  // grouped imaging instance (2D only)
     auto grid = [&](int64_t cols, int r) {
         a.gx = (uint32_t)((a.nc + cols - 1) / cols);
         a.gy = vrow ? (uint32_t)(grp ? (rows + r - 1) / r : (rows + 4 * r - 1) / (4 * r)) : 1u;
@@ -465,6 +471,14 @@ static hipError_t launch2d_flags(const StepArgs<T>& a, bool vrow, bool varc, boo
             case 3: return launch2d_kn<T, K, false, true, true, NT, true>(a, vrow, variant, ri, nshots, s);
             case 5: return launch2d_kn<T, K, true, false, true, NT, true>(a, vrow, variant, ri, nshots, s);
             default: return launch2d_kn<T, K, true, true, true, NT, true>(a, vrow, variant, ri, nshots, s);
+        }
+    }
+    if (!img && vrow && a.ucopy) {  // forward instances that also store the kept copy of u_next
+        switch (f) {
+            case 0: return launch2d_kn<T, K, false, false, false, NT, true>(a, vrow, variant, ri, nshots, s);
+            case 2: return launch2d_kn<T, K, false, true, false, NT, true>(a, vrow, variant, ri, nshots, s);
+            case 4: return launch2d_kn<T, K, true, false, false, NT, true>(a, vrow, variant, ri, nshots, s);
+            default: return launch2d_kn<T, K, true, true, false, NT, true>(a, vrow, variant, ri, nshots, s);
         }
     }
     switch (f) {
