@@ -86,7 +86,7 @@ int wfd_plan_create(wfd_plan** out, const wfd_grid* g, int64_t nsteps, int64_t s
         return plan_fail(WFD_EINVAL, "wfd_plan_create: seg must be in [0, nsteps] or WFD_SEG_FASTEST");
     const size_t batch = (size_t)g->elems * (size_t)g->nshots * (g->dtype == WFD_F64 ? 8 : 4);
     if (seg == WFD_SEG_FASTEST) {
-        // every level kept (no recompute: 16 + 28 instead of 16 + 20 + 28 bytes per cell-step)
+        // every level kept (no recompute: 20 + 28 instead of 16 + 20 + 28 bytes per cell-step)
         // when that store fits the budget
         size_t budget = max_bytes;
         if (!budget) {

@@ -48,8 +48,8 @@ def observed(w):
 
 def batched_gradient(w, obs, v=None, seg=0):
     """J and dJ/dv of all shots with sqrt(n) checkpointing (seg 0); ``seg=solver.SEG_FASTEST``
-    keeps every forward level when it fits (no recompute in the backward pass: 16 + 28 bytes per
-    cell-step instead of 16 + 20 + 28), with the same gradient bits."""
+    keeps every forward level when it fits (no recompute in the backward pass: 20 + 22 bytes per
+    cell-step instead of 16 + 20 + 22), with the same gradient bits."""
     v = w["start"] if v is None else v
     return misfit_gradient(torch.as_tensor(v), w["grid"], w["src"], w["sig"], w["rec"], obs, w["beta"], seg=seg)
 
