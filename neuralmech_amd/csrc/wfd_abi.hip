@@ -267,7 +267,7 @@ struct Launch {
     void* un2 = nullptr;  // two-step launch: u^{n+2} (un receives u^{n+1})
     int t2rows = 16;
     bool img_group = false;  // 2D imaging of several shots: one partial per IMG_GROUP shots
-    void* ucopy = nullptr;   // 2D forward: second copy of u_next (see StepArgs::ucopy)
+    void* ucopy = nullptr;   // 2D forward: receives L u_cur (see StepArgs::ucopy)
 };
 
 template <typename T>

@@ -69,8 +69,8 @@ struct StepArgs {
     // Two-step launches: un receives u^{n+1}, un2 u^{n+2}; their gather blocks record up (u^{n-1})
     // and uc (u^n) into trace rows step + gather_off - 1 and step + gather_off.
     T* un2;
-    // 2D forward launches of the gradient plan: a second copy of u^{n+1} (the level kept for the
-    // adjoint), written with the nontemporal hint while un rotates in place with up
+    // 2D forward launches of the gradient plan: receives L u^n (the Laplacian of uc that the update
+    // uses), written with the nontemporal hint, for the adjoint's imaging; un rotates in place
     T* ucopy;
 };
 

@@ -30,10 +30,13 @@ extern "C" int wfd_adjoint_step(const wfd_grid* g, const void* mu_prev, const vo
 struct wfd_plan {
     wfd_grid g;
     int64_t nsteps = 0, seg = 0, nckpt = 0, flush = 32;
-    // 2D grids: the forward and the recompute run in the two work fields (the stencil's fields stay
-    // in place, as in a plain run) and store each kept level a second time with the nontemporal
-    // hint; other grids step into the kept fields directly
-    bool copy_out = false;
+    // 2D grids of several shots: the adjoint's imaging reads L u^{m-1} from a stored field instead
+    // of taking the Laplacian of u^{m-1}.  The recompute (every level kept: the forward) steps in
+    // place in the two work fields, as a plain run does, and stores the Laplacian of its current
+    // level as a second, nontemporal output of the launch: C slots per segment (L u^{s0} ..
+    // L u^{s1-1}) instead of C - 1, one more recompute step per segment.  Other grids step into
+    // the kept fields directly and the adjoint reads u.
+    bool lu = false;
     size_t bytes = 0;
     size_t batch_bytes = 0;  // one field batch (nshots fields)
     char* mem = nullptr;
@@ -59,12 +62,14 @@ int flush_partials(const wfd_plan* p, double* G, hipStream_t st)
     return e == hipSuccess ? WFD_OK : plan_fail(WFD_EHIP, "wfd_plan_backward: flush kernel failed");
 }
 
-size_t fields_needed(int64_t N, int64_t C)
+size_t fields_needed(int64_t N, int64_t C, bool lu)
 {
     const int64_t nck = (N + C - 1) / C;
     // C == 1: checkpoint k's older level is checkpoint k-1's newer one; only u^{-1} is extra
+    // (with lu, checkpoint field k holds L u^k)
     const int64_t ck_fields = C == 1 ? nck + 1 : 2 * nck;
-    return (size_t)(ck_fields + (C - 1) + 2 + 2 + 1);
+    const int64_t slots = C == 1 ? 0 : (lu ? C : C - 1);
+    return (size_t)(ck_fields + slots + 2 + 2 + 1);
 }
 
 void* field_at(const wfd_plan* p, int64_t n, int64_t s0, int64_t k)
@@ -85,8 +90,9 @@ int wfd_plan_create(wfd_plan** out, const wfd_grid* g, int64_t nsteps, int64_t s
     if ((seg < 0 && seg != WFD_SEG_FASTEST) || seg > nsteps)
         return plan_fail(WFD_EINVAL, "wfd_plan_create: seg must be in [0, nsteps] or WFD_SEG_FASTEST");
     const size_t batch = (size_t)g->elems * (size_t)g->nshots * (g->dtype == WFD_F64 ? 8 : 4);
+    const bool lu = wfd::img_groups(g) > 0;
     if (seg == WFD_SEG_FASTEST) {
-        // every level kept (no recompute: 20 + 28 instead of 16 + 20 + 28 bytes per cell-step)
+        // every level kept (no recompute: the forward and the adjoint pass only)
         // when that store fits the budget
         size_t budget = max_bytes;
         if (!budget) {
@@ -94,13 +100,13 @@ int wfd_plan_create(wfd_plan** out, const wfd_grid* g, int64_t nsteps, int64_t s
             if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return plan_fail(WFD_EHIP, "wfd_plan_create: hipMemGetInfo failed");
             budget = (size_t)(0.8 * (double)free_b);
         }
-        seg = fields_needed(nsteps, 1) * batch <= budget ? 1 : 0;
+        seg = fields_needed(nsteps, 1, lu) * batch <= budget ? 1 : 0;
     }
     if (seg == 0) {
         int64_t best = 1;
-        size_t best_f = fields_needed(nsteps, 1);
+        size_t best_f = fields_needed(nsteps, 1, lu);
         for (int64_t C = 2; C <= nsteps; C++) {
-            const size_t f = fields_needed(nsteps, C);
+            const size_t f = fields_needed(nsteps, C, lu);
             if (f < best_f) {
                 best_f = f;
                 best = C;
@@ -110,10 +116,10 @@ int wfd_plan_create(wfd_plan** out, const wfd_grid* g, int64_t nsteps, int64_t s
     }
     if (max_bytes) {
         // shrink toward the memory-optimal spacing until the store fits
-        if (fields_needed(nsteps, seg) * batch > max_bytes) {
+        if (fields_needed(nsteps, seg, lu) * batch > max_bytes) {
             int64_t best = -1;
             for (int64_t C = 1; C <= nsteps; C++)
-                if (fields_needed(nsteps, C) * batch <= max_bytes) {
+                if (fields_needed(nsteps, C, lu) * batch <= max_bytes) {
                     best = C;
                     break;
                 }
@@ -126,10 +132,10 @@ int wfd_plan_create(wfd_plan** out, const wfd_grid* g, int64_t nsteps, int64_t s
     p->g = *g;
     p->nsteps = nsteps;
     p->seg = seg;
-    p->copy_out = g->ndim == 2;
+    p->lu = lu;
     p->nckpt = (nsteps + seg - 1) / seg;
     p->batch_bytes = batch;
-    const size_t nf = fields_needed(nsteps, seg);
+    const size_t nf = fields_needed(nsteps, seg, lu);
     p->bytes = nf * batch;
     hipError_t e = hipMalloc(&p->mem, p->bytes);
     if (e != hipSuccess) {
@@ -155,7 +161,7 @@ int wfd_plan_create(wfd_plan** out, const wfd_grid* g, int64_t nsteps, int64_t s
     for (int64_t k = 0; k < p->nckpt; k++) p->ck_cur[k] = take();
     for (int64_t k = 0; k < p->nckpt; k++) p->ck_prev[k] = seg == 1 ? (k ? p->ck_cur[k - 1] : nullptr) : take();
     if (seg == 1) p->ck_prev[0] = take();  // u^{-1}
-    p->slot.resize(seg > 1 ? seg - 1 : 0);
+    p->slot.resize(seg > 1 ? (lu ? seg : seg - 1) : 0);
     for (auto& s : p->slot) s = take();
     p->work[0] = take();
     p->work[1] = take();
@@ -196,20 +202,25 @@ int wfd_plan_forward(wfd_plan* p, const wfd_medium* med, const wfd_points* src, 
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     const int64_t N = p->nsteps, C = p->seg;
-    hipError_t e = wfd::launch_zero(p->ck_prev[0], (int64_t)p->batch_bytes, st);
-    if (e == hipSuccess) e = wfd::launch_zero(p->ck_cur[0], (int64_t)p->batch_bytes, st);
+    // every level kept with stored Laplacians: the levels rotate in the work fields and step n
+    // stores L u^n into checkpoint field n; otherwise checkpoints receive u directly
+    const bool lu_all = p->lu && C == 1;
+    void* P = lu_all ? p->work[0] : p->ck_prev[0];
+    void* Cu = lu_all ? p->work[1] : p->ck_cur[0];
+    hipError_t e = wfd::launch_zero(P, (int64_t)p->batch_bytes, st);
+    if (e == hipSuccess) e = wfd::launch_zero(Cu, (int64_t)p->batch_bytes, st);
     if (e != hipSuccess) return plan_fail(WFD_EHIP, "wfd_plan_forward: zero fill failed");
-    void* P = p->ck_prev[0];
-    void* Cu = p->ck_cur[0];
     for (int64_t n = 0; n < N; n++) {
         const int64_t m = n + 1;
         void* keep = nullptr;  // the checkpoint field that keeps u^m, if any
-        if (C > 1 && m % C == C - 1 && (m + 1) / C < p->nckpt)
-            keep = p->ck_prev[(m + 1) / C];
-        else if (m % C == 0 && m / C < p->nckpt)
-            keep = p->ck_cur[m / C];
+        if (!lu_all) {
+            if (C > 1 && m % C == C - 1 && (m + 1) / C < p->nckpt)
+                keep = p->ck_prev[(m + 1) / C];
+            else if (m % C == 0 && m / C < p->nckpt)
+                keep = p->ck_cur[m / C];
+        }
         void* D = nullptr;
-        if (keep && !p->copy_out)
+        if (keep)
             D = keep;
         else if (P == p->work[0] || P == p->work[1])
             D = P;  // in place over the oldest level
@@ -217,7 +228,7 @@ int wfd_plan_forward(wfd_plan* p, const wfd_medium* med, const wfd_points* src, 
             D = Cu == p->work[0] ? p->work[1] : p->work[0];
         // the launch also records u^n (its current level) into trace row n-1
         rc = wfd::step_launch(&p->g, P, Cu, D, med, src, rec, n, -1, nullptr, nullptr, false, st, false,
-                              p->copy_out ? keep : nullptr);
+                              lu_all ? p->ck_cur[n] : nullptr);
         if (rc) return rc;
         P = Cu;
         Cu = D;
@@ -253,16 +264,16 @@ int wfd_plan_backward(wfd_plan* p, const wfd_medium* med, const wfd_points* src,
     void* muC = p->mu[1];  // mu^{m+1}
     for (int64_t k = p->nckpt - 1; k >= 0; k--) {
         const int64_t s0 = k * C, s1 = std::min(s0 + C, N);
-        if (p->copy_out) {
-            // recompute in the two work fields (in place, like the forward), each level also
-            // stored into its segment slot
+        if (p->lu) {
+            // recompute in the two work fields (in place, like the forward); step n stores L u^n
+            // into slot n - s0 (C == 1: the forward stored them)
             const void* P = p->ck_prev[k];
             const void* Cu = p->ck_cur[k];
-            for (int64_t n = s0; n + 1 < s1; n++) {
+            for (int64_t n = s0; C > 1 && n < s1; n++) {
                 void* D = P == p->work[0] || P == p->work[1] ? const_cast<void*>(P)
                                                               : (Cu == p->work[0] ? p->work[1] : p->work[0]);
                 int rc = wfd::step_launch(&p->g, P, Cu, D, med, src, nullptr, n, 0, nullptr, nullptr, false, st,
-                                          false, field_at(p, n + 1, s0, k));
+                                          false, p->slot[n - s0]);
                 if (rc) return rc;
                 P = Cu;
                 Cu = D;
@@ -278,8 +289,8 @@ int wfd_plan_backward(wfd_plan* p, const wfd_medium* med, const wfd_points* src,
         for (int64_t m = s1; m > s0; m--) {
             // adjoint step m (signal row m-1) makes mu^m; its launch records mu^{m+1} (its current
             // level) into row m of src_rec
-            int rc = wfd::step_launch(&p->g, muP, muC, muP, med, adj, src_rec, m - 1, +1, field_at(p, m - 1, s0, k),
-                                      p->gpart, true, st, true);
+            const void* F = !p->lu ? field_at(p, m - 1, s0, k) : C == 1 ? p->ck_cur[m - 1] : p->slot[m - 1 - s0];
+            int rc = wfd::step_launch(&p->g, muP, muC, muP, med, adj, src_rec, m - 1, +1, F, p->gpart, true, st, true);
             if (rc) return rc;
             std::swap(muP, muC);  // muC = mu^m, muP = mu^{m+1}
             if ((m - 1) % p->flush == 0) {

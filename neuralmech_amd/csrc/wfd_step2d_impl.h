@@ -159,9 +159,10 @@ __device__ __forceinline__ C4<T> (&img_exchange())[2][IMG_GROUP][RI][64]
 // access to u_prev, kc and u_next.  Grouped imaging launches (a.img_group, 2D) pass the shot group
 // as `shot`: wave w takes shot IMG_GROUP * group + w on the tile's RI rows, and after the march
 // the group's partial row q is updated by wave q % 4 as p = fma(mu_s, L u_s, p) for s in shot
-// order (only the order of the float additions differs from one partial per shot).  On a forward
-// instance (IMG false) GRP instead stores u_next a second time, into a.ucopy with the nontemporal
-// hint (the gradient plan's kept levels, while u_next rotates in place with u_prev).
+// order (only the order of the float additions differs from one partial per shot); a.uf then holds
+// the Laplacian L u of the forward field (stored by the recompute), read pointwise.  On a forward
+// instance (IMG false) GRP stores that Laplacian of u_cur (the same lap_elem values the update
+// uses) into a.ucopy with the nontemporal hint, while u_next rotates in place with u_prev.
 template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, int LA = 0,
           bool NT = false, bool GRP = false>
 __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, uint32_t ty, int shot_in, int64_t step)
@@ -197,8 +198,9 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
         const int64_t r = r0 - KR + t;
         win[t] = (r < a.row_hi + KR && load_ok) ? ld4(uc + r * a.ld + c) : zero4<T>();
     }
-    C4<T> wf[IMG ? NW : 1];
-    if constexpr (IMG) {
+    // the forward field's window for its Laplacian (grouped instances read that Laplacian instead)
+    C4<T> wf[IMG && !grp ? NW : 1];
+    if constexpr (IMG && !grp) {
         const T* __restrict__ uf = a.uf + sb;
 #pragma unroll
         for (int t = 0; t < NW; t++) {
@@ -230,7 +232,7 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
     // waiting for the store).  u_next may alias u_prev only element for element, and a row's
     // operands are loaded before its own store: the values read are those of a row-by-row update.
     const T* __restrict__ ufp = IMG ? a.uf + sb : nullptr;
-    constexpr int NH = XL ? 1 : RI, NHF = (XL || !IMG) ? 1 : RI;
+    constexpr int NH = XL ? 1 : RI, NHF = (XL || !IMG || grp) ? 1 : RI;
     C4<T> Pv[RI], Kv[VARC ? RI : 1], Gv[IMG ? RI : 1], Lv[NH], Rv[NH], Lfv[NHF], Rfv[NHF];
     T Bv[SPONGE ? RI : 1];
     auto issue = [&](int q) {
@@ -240,12 +242,14 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
         const int64_t o = r * a.ld + c;
         Pv[q] = ld ? sld4<NT>(up + o) : zero4<T>();
         if constexpr (VARC) Kv[q] = ld ? sld4<NT>(coef + o) : zero4<T>();
-        if constexpr (IMG) Gv[q] = ld && !grp ? ld4(a.g + sb + o) : zero4<T>();
+        // Gv: the imaging partial, or for grouped instances the stored Laplacian of the forward
+        // field (read once, with the nontemporal hint)
+        if constexpr (IMG) Gv[q] = !ld ? zero4<T>() : grp ? sld4<true>(ufp + o) : ld4(a.g + sb + o);
         if constexpr (SPONGE) Bv[q] = VROW ? a.beta_r[r] : T(0);
         if constexpr (!XL) {
             Lv[q] = load_ok ? ld4(uc + o - 4) : zero4<T>();
             Rv[q] = load_ok ? ld4(uc + o + 4) : zero4<T>();
-            if constexpr (IMG) {
+            if constexpr (IMG && !grp) {
                 Lfv[q] = load_ok ? ld4(ufp + o - 4) : zero4<T>();
                 Rfv[q] = load_ok ? ld4(ufp + o + 4) : zero4<T>();
             }
@@ -263,14 +267,14 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
         if constexpr (XL) {
             L = from_lane_left(win[q + KR]);
             R = from_lane_right(win[q + KR]);
-            if constexpr (IMG) {
+            if constexpr (IMG && !grp) {
                 Lf = from_lane_left(wf[q + KR]);
                 Rf = from_lane_right(wf[q + KR]);
             }
         } else {
             L = Lv[XL ? 0 : q];
             R = Rv[XL ? 0 : q];
-            if constexpr (IMG) {
+            if constexpr (IMG && !grp) {
                 Lf = Lfv[NHF > 1 ? q : 0];
                 Rf = Rfv[NHF > 1 ? q : 0];
             }
@@ -288,11 +292,12 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
                 damp |= b[e] != T(0);
             }
         }
-        T val[4];
+        T val[4], lapc[4];
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const T ctr = win[q + KR].e[e];
             const T lap = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, win, q, L, R, e);
+            lapc[e] = lap;
             const T t = damp ? T(2) * ctr - (T(1) - b[e]) * P.e[e] : T(2) * ctr - P.e[e];
             val[e] = fmaT(kc[e], lap, t);
         }
@@ -306,14 +311,17 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
         for (int e = 0; e < 4; e++)
             if (!valid[e]) val[e] = T(0);
         T gv[4];
-        if constexpr (IMG) {
+        if constexpr (IMG && grp) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) gv[e] = Gv[q].e[e];
+        } else if constexpr (IMG) {
 #pragma unroll
             for (int e = 0; e < 4; e++) gv[e] = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, wf, q, Lf, Rf, e);
         }
         if constexpr (LA > 0) __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
         if (own && live) {
             sst4<NT>(un + o, val);
-            if constexpr (cpy) sst4<true>(a.ucopy + sb + o, val);
+            if constexpr (cpy) sst4<true>(a.ucopy + sb + o, lapc);
         }
         if constexpr (IMG) {
             if constexpr (grp) {

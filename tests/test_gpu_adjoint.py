@@ -300,6 +300,12 @@ def test_multishot_imaging_sum_bitexact_vs_oracle(shape, order, nshots, group, d
     assert np.array_equal(plan.last_G.cpu().numpy(), G)
     if len(shape) != 2:
         return
+    for seg in (solver.SEG_FASTEST, 1, N, 7):  # every level kept, one segment, a ragged last segment
+        plan = solver.GradientPlan(g, N, seg=seg)
+        plan.forward(med, src, torch.as_tensor(sig), rec)
+        plan.backward(med, src, rec, resid)
+        torch.cuda.synchronize()
+        assert np.array_equal(plan.last_G.cpu().numpy(), G), seg
     shapes = [(0, 2), (3, 8), (6, 4)] if dtype == "f32" else [(0, 4), (3, 4), (6, 4)]  # f64 marches 4 rows
     for v, rows in shapes:  # every 2D launch shape groups the same way
         g.set_launch(v, rows)
