@@ -146,27 +146,38 @@ __device__ __forceinline__ uint32_t hw_xcc_id() { return __builtin_amdgcn_s_getr
 // 2 and halve the adjoint tiles per phase: 3.350 -> 3.258 s for the 64-shot batch, though the
 // kernel then spills 3 VGPRs (2 slots per CU with 8-row adjoint tiles and no spills: 3.368 s).
 constexpr int ADJ_ROWS = 4;
-template <typename T, int K, bool IMG>
-__device__ __forceinline__ void batch_tile_k(const StepArgs<T>& a, int cfg, uint32_t tx, uint32_t ty, int64_t step)
+// Adjoint tiles read the Laplacian L u^{m-1} that the recompute stored (LUF); recompute tiles
+// (a.ucopy set) store it as a second output while their levels rotate in the work fields.
+template <typename T, int K, bool IMG, bool CPY>
+__device__ __forceinline__ void batch_tile_kc(const StepArgs<T>& a, int cfg, uint32_t tx, uint32_t ty, int64_t step)
 {
     if constexpr (sizeof(T) == 8) {
         if (cfg_variant(cfg) == 1)
-            step2d_tile<T, K, true, true, true, IMG, 2, true, 0>(a, tx, ty, 0, step);
+            step2d_tile<T, K, true, true, true, IMG, 2, true, 0, false, CPY, IMG>(a, tx, ty, 0, step);
         else
-            step2d_tile<T, K, true, true, true, IMG, 2, false, 0>(a, tx, ty, 0, step);
+            step2d_tile<T, K, true, true, true, IMG, 2, false, 0, false, CPY, IMG>(a, tx, ty, 0, step);
     } else if constexpr (IMG) {
         if (cfg_variant(cfg) == 1)
-            step2d_tile<T, K, true, true, true, IMG, ADJ_ROWS, true, 0>(a, tx, ty, 0, step);
+            step2d_tile<T, K, true, true, true, IMG, ADJ_ROWS, true, 0, false, false, true>(a, tx, ty, 0, step);
         else
-            step2d_tile<T, K, true, true, true, IMG, ADJ_ROWS, false, 0>(a, tx, ty, 0, step);
+            step2d_tile<T, K, true, true, true, IMG, ADJ_ROWS, false, 0, false, false, true>(a, tx, ty, 0, step);
     } else {
         switch (cfg) {
-            case 0: step2d_tile<T, K, true, true, true, IMG, 8, true, 2>(a, tx, ty, 0, step); break;
-            case 1: step2d_tile<T, K, true, true, true, IMG, 2, true, 0>(a, tx, ty, 0, step); break;
-            case 2: step2d_tile<T, K, true, true, true, IMG, 8, false, 2>(a, tx, ty, 0, step); break;
-            default: step2d_tile<T, K, true, true, true, IMG, 2, false, 0>(a, tx, ty, 0, step); break;
+            case 0: step2d_tile<T, K, true, true, true, IMG, 8, true, 2, false, CPY>(a, tx, ty, 0, step); break;
+            case 1: step2d_tile<T, K, true, true, true, IMG, 2, true, 0, false, CPY>(a, tx, ty, 0, step); break;
+            case 2: step2d_tile<T, K, true, true, true, IMG, 8, false, 2, false, CPY>(a, tx, ty, 0, step); break;
+            default: step2d_tile<T, K, true, true, true, IMG, 2, false, 0, false, CPY>(a, tx, ty, 0, step); break;
         }
     }
+}
+
+template <typename T, int K, bool IMG>
+__device__ __forceinline__ void batch_tile_k(const StepArgs<T>& a, int cfg, uint32_t tx, uint32_t ty, int64_t step)
+{
+    if (!IMG && a.ucopy)
+        batch_tile_kc<T, K, IMG, !IMG>(a, cfg, tx, ty, step);
+    else
+        batch_tile_kc<T, K, IMG, false>(a, cfg, tx, ty, step);
 }
 
 template <typename T, bool IMG>
@@ -196,6 +207,7 @@ __device__ __forceinline__ void batch_item(const BShot<T>& sh, const BItem& it, 
         a.src = img ? sh.adj : sh.src;
         if (!(it.flags & F_INJECT)) a.src.count = 0;
         const uint32_t tx = local % sh.gx, ty = local / sh.gx;
+        a.ucopy = !img && it.uf >= 0 ? F + it.uf * fs : nullptr;  // recompute: L u^n into field uf
         if (img) {
             a.uf = F + it.uf * fs;
             a.g = F + it.gp * fs;
@@ -354,14 +366,15 @@ struct HostOp {
     int32_t phase;  // phase of the shot's schedule the op runs in (ops of one phase run concurrently)
 };
 
-// Fields of a shot's store: checkpoints (as in the gradient plan), two sets of segment slots
-// (the backward recomputes segment k-1 into one set while the adjoint of segment k reads the
-// other), two work fields, two adjoint fields and the imaging partial.
+// Fields of a shot's store: checkpoints (as in the gradient plan), two sets of C segment slots
+// holding L u^n for the imaging (the backward recomputes segment k-1 into one set while the
+// adjoint of segment k reads the other), two work fields, two adjoint fields and the imaging
+// partial.  With C = 1 checkpoint field n holds L u^n.
 size_t batch_fields(int64_t N, int64_t C)
 {
     const int64_t nck = (N + C - 1) / C;
     const int64_t ck_fields = C == 1 ? nck + 1 : 2 * nck;
-    return (size_t)(ck_fields + (nck >= 2 ? 2 : 1) * (C - 1) + 5);
+    return (size_t)(ck_fields + (C == 1 ? 0 : (nck >= 2 ? 2 : 1) * C) + 5);
 }
 
 int64_t optimal_seg(int64_t N)
@@ -379,16 +392,19 @@ int64_t optimal_seg(int64_t N)
 }
 
 // The ops of one shot over field indices: the forward run of wfd_plan_forward (steps writing
-// straight into checkpoint storage), then the backward of wfd_plan_backward with the recompute of
-// segment k-1 running in the same phases as the adjoint of segment k (they touch disjoint fields:
-// segments alternate between the two slot sets).  Field 0 (checkpoint 0's newer level, the rest
-// state, never written) doubles as the zero field the adjoint starts from.  Every value, and the
-// order of every floating-point operation, is the plan's.  Returns the number of fields.
+// straight into checkpoint storage; with C = 1 the levels rotate in the work fields and each step
+// stores L u^n into checkpoint field n), then the backward of wfd_plan_backward with the recompute
+// of segment k-1 running in the same phases as the adjoint of segment k (they touch disjoint
+// fields: segments alternate between the two slot sets).  The recompute steps in place in the
+// work fields and stores L u^n into the segment's slot n - s0; the adjoint reads it pointwise.  A
+// checkpoint field that is never written (checkpoint 0's newer level, or with C = 1 u^{-1})
+// doubles as the zero field the adjoint starts from.  Every value, and the order of every
+// floating-point operation, is the plan's.  Returns the number of fields.
 int shot_ops(int64_t N, int64_t C, bool has_src, bool has_rec, std::vector<HostOp>& ops)
 {
     const int64_t nck = (N + C - 1) / C;
     int nf = 0;
-    std::vector<int> ck_cur(nck), ck_prev(nck), slot_a(C > 1 ? C - 1 : 0), slot_b(C > 1 && nck >= 2 ? C - 1 : 0);
+    std::vector<int> ck_cur(nck), ck_prev(nck), slot_a(C > 1 ? C : 0), slot_b(C > 1 && nck >= 2 ? C : 0);
     for (int64_t k = 0; k < nck; k++) ck_cur[k] = nf++;
     if (C == 1) {
         ck_prev[0] = nf++;
@@ -399,7 +415,7 @@ int shot_ops(int64_t N, int64_t C, bool has_src, bool has_rec, std::vector<HostO
     for (auto& f : slot_a) f = nf++;
     for (auto& f : slot_b) f = nf++;
     const int w0 = nf++, w1 = nf++, mu0 = nf++, mu1 = nf++, gpart = nf++;
-    const int Z = ck_cur[0];
+    const int Z = C == 1 ? ck_prev[0] : ck_cur[0];
     ops.clear();
     int32_t ph = 0;
     auto mk = [&](int kind, int flags, int up, int uc, int un, int uf, int gp, int gf, int64_t step, int64_t grow) {
@@ -410,34 +426,39 @@ int shot_ops(int64_t N, int64_t C, bool has_src, bool has_rec, std::vector<HostO
         ops.push_back(o);
     };
     // forward
-    int P = ck_prev[0], Cu = ck_cur[0];
+    int P = ck_prev[0], Cu = C == 1 ? Z : ck_cur[0];  // u^{-1} = u^0 = 0
     for (int64_t n = 0; n < N; n++) {
         const int64_t m = n + 1;
         int D;
         if (C > 1 && m % C == C - 1 && (m + 1) / C < nck)
             D = ck_prev[(m + 1) / C];
-        else if (m % C == 0 && m / C < nck)
+        else if (C > 1 && m % C == 0 && m / C < nck)
             D = ck_cur[m / C];
         else if (P == w0 || P == w1)
             D = P;
         else
             D = Cu == w0 ? w1 : w0;
         const int fl = (has_src ? (int)F_INJECT : 0) | (has_rec && n >= 1 ? (int)F_GREC : 0);
-        emit(mk(K_STEP, fl, P, Cu, D, -1, -1, Cu, n, n - 1));
+        emit(mk(K_STEP, fl, P, Cu, D, C == 1 ? ck_cur[n] : -1, -1, Cu, n, n - 1));
         P = Cu;
         Cu = D;
     }
     const int last = Cu;  // u^N: receiver row N-1 is gathered from it
     // backward
     auto slots = [&](int64_t k) -> const std::vector<int>& { return ((nck - 1 - k) & 1) ? slot_b : slot_a; };
-    auto field_at = [&](int64_t n, int64_t s0, int64_t k) { return n == s0 ? ck_cur[k] : slots(k)[n - s0 - 1]; };
+    // L u^n of a segment's level n (C = 1: checkpoint field n)
+    auto lu_at = [&](int64_t n, int64_t s0, int64_t k) { return C == 1 ? ck_cur[n] : slots(k)[n - s0]; };
     auto recompute = [&](int64_t k) {
         std::vector<HostOp> r;
+        if (C == 1) return r;
         const int64_t s0 = k * C, s1 = std::min(s0 + C, N);
-        for (int64_t n = s0; n + 1 < s1; n++) {
-            const int prev = n == s0 ? ck_prev[k] : field_at(n - 1, s0, k);
-            r.push_back(mk(K_STEP, has_src ? F_INJECT : 0, prev, field_at(n, s0, k), field_at(n + 1, s0, k), -1, -1, -1,
-                           n, -1));
+        int P = ck_prev[k], Cu = ck_cur[k];
+        for (int64_t n = s0; n < s1; n++) {
+            // in place over the oldest work level; never into u^N (gathered in the first phase)
+            const int D = (P == w0 || P == w1) ? P : (Cu == w0 || Cu == w1) ? (Cu == w0 ? w1 : w0) : (last == w0 ? w1 : w0);
+            r.push_back(mk(K_STEP, has_src ? F_INJECT : 0, P, Cu, D, lu_at(n, s0, k), -1, -1, n, -1));
+            P = Cu;
+            Cu = D;
         }
         return r;
     };
@@ -453,7 +474,7 @@ int shot_ops(int64_t N, int64_t C, bool has_src, bool has_rec, std::vector<HostO
                 fl |= F_FLUSH | (first ? (int)F_FIRST : 0);
                 first = false;
             }
-            a.push_back(mk(K_ADJ, fl, muP, muC, dest, field_at(m - 1, s0, k), gpart, muC, m - 1, m));
+            a.push_back(mk(K_ADJ, fl, muP, muC, dest, lu_at(m - 1, s0, k), gpart, muC, m - 1, m));
             muP = muC;
             muC = dest;
         }

@@ -160,11 +160,12 @@ __device__ __forceinline__ C4<T> (&img_exchange())[2][IMG_GROUP][RI][64]
 // as `shot`: wave w takes shot IMG_GROUP * group + w on the tile's RI rows, and after the march
 // the group's partial row q is updated by wave q % 4 as p = fma(mu_s, L u_s, p) for s in shot
 // order (only the order of the float additions differs from one partial per shot); a.uf then holds
-// the Laplacian L u of the forward field (stored by the recompute), read pointwise.  On a forward
+// the Laplacian L u of the forward field (stored by the recompute), read pointwise, as it is in
+// LUF imaging instances (one partial per shot; the uneven batch).  On a forward
 // instance (IMG false) GRP stores that Laplacian of u_cur (the same lap_elem values the update
 // uses) into a.ucopy with the nontemporal hint, while u_next rotates in place with u_prev.
 template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, int LA = 0,
-          bool NT = false, bool GRP = false>
+          bool NT = false, bool GRP = false, bool LUF = false>
 __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, uint32_t ty, int shot_in, int64_t step)
 {
     constexpr int KR = VROW ? K : 0;
@@ -175,6 +176,7 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
     const int64_t base = (int64_t)tx * COLS - (XL ? 4 : 0);  // column of lane 0's chunk
     const int64_t c = base + lane * 4;
     constexpr bool grp = GRP && IMG && VROW;  // grouped imaging instance (a.img_group > 1)
+    constexpr bool luf = IMG && (grp || LUF);  // a.uf holds L u of the forward field
     constexpr bool cpy = GRP && !IMG && VROW;  // forward instance storing a second copy (a.ucopy)
     // grouped: a wave of a shot past the last one marches the group's first shot and stores nothing
     const int gshot = grp ? shot_in * IMG_GROUP + wave : shot_in;
@@ -199,8 +201,8 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
         win[t] = (r < a.row_hi + KR && load_ok) ? ld4(uc + r * a.ld + c) : zero4<T>();
     }
     // the forward field's window for its Laplacian (grouped instances read that Laplacian instead)
-    C4<T> wf[IMG && !grp ? NW : 1];
-    if constexpr (IMG && !grp) {
+    C4<T> wf[IMG && !luf ? NW : 1];
+    if constexpr (IMG && !luf) {
         const T* __restrict__ uf = a.uf + sb;
 #pragma unroll
         for (int t = 0; t < NW; t++) {
@@ -232,8 +234,8 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
     // waiting for the store).  u_next may alias u_prev only element for element, and a row's
     // operands are loaded before its own store: the values read are those of a row-by-row update.
     const T* __restrict__ ufp = IMG ? a.uf + sb : nullptr;
-    constexpr int NH = XL ? 1 : RI, NHF = (XL || !IMG || grp) ? 1 : RI;
-    C4<T> Pv[RI], Kv[VARC ? RI : 1], Gv[IMG ? RI : 1], Lv[NH], Rv[NH], Lfv[NHF], Rfv[NHF];
+    constexpr int NH = XL ? 1 : RI, NHF = (XL || !IMG || luf) ? 1 : RI;
+    C4<T> Pv[RI], Kv[VARC ? RI : 1], Gv[IMG ? RI : 1], Lv[NH], Rv[NH], Lfv[NHF], Rfv[NHF], Fv[luf && !grp ? RI : 1];
     T Bv[SPONGE ? RI : 1];
     auto issue = [&](int q) {
         const int64_t r = r0 + q;
@@ -245,11 +247,12 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
         // Gv: the imaging partial, or for grouped instances the stored Laplacian of the forward
         // field (read once, with the nontemporal hint)
         if constexpr (IMG) Gv[q] = !ld ? zero4<T>() : grp ? sld4<true>(ufp + o) : ld4(a.g + sb + o);
+        if constexpr (luf && !grp) Fv[q] = ld ? sld4<true>(ufp + o) : zero4<T>();
         if constexpr (SPONGE) Bv[q] = VROW ? a.beta_r[r] : T(0);
         if constexpr (!XL) {
             Lv[q] = load_ok ? ld4(uc + o - 4) : zero4<T>();
             Rv[q] = load_ok ? ld4(uc + o + 4) : zero4<T>();
-            if constexpr (IMG && !grp) {
+            if constexpr (IMG && !luf) {
                 Lfv[q] = load_ok ? ld4(ufp + o - 4) : zero4<T>();
                 Rfv[q] = load_ok ? ld4(ufp + o + 4) : zero4<T>();
             }
@@ -267,14 +270,14 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
         if constexpr (XL) {
             L = from_lane_left(win[q + KR]);
             R = from_lane_right(win[q + KR]);
-            if constexpr (IMG && !grp) {
+            if constexpr (IMG && !luf) {
                 Lf = from_lane_left(wf[q + KR]);
                 Rf = from_lane_right(wf[q + KR]);
             }
         } else {
             L = Lv[XL ? 0 : q];
             R = Rv[XL ? 0 : q];
-            if constexpr (IMG && !grp) {
+            if constexpr (IMG && !luf) {
                 Lf = Lfv[NHF > 1 ? q : 0];
                 Rf = Rfv[NHF > 1 ? q : 0];
             }
@@ -314,6 +317,9 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
         if constexpr (IMG && grp) {
 #pragma unroll
             for (int e = 0; e < 4; e++) gv[e] = Gv[q].e[e];
+        } else if constexpr (luf) {
+#pragma unroll
+            for (int e = 0; e < 4; e++) gv[e] = Fv[q].e[e];
         } else if constexpr (IMG) {
 #pragma unroll
             for (int e = 0; e < 4; e++) gv[e] = lap_elem<T, K, VROW>(a.wc, a.wr, a.wcol, wf, q, Lf, Rf, e);

@@ -1,5 +1,6 @@
 """CPU checks of the uneven batch's host side: the per-shot op list that the persistent kernel
-runs, replayed symbolically (which time level every field holds), and argument validation."""
+runs, replayed symbolically (which time level, or which level's Laplacian, every field holds),
+and argument validation."""
 
 import ctypes
 
@@ -35,12 +36,14 @@ def replay(N, seg, has_src, has_rec):
         group = ops[ops[:, 10] == ph].tolist()
         assert 1 <= len(group) <= 2
         # ops of one phase run concurrently: no op writes what another op of the phase touches
-        writes = [o[4] for o in group if o[0] != K_GATHER]
+        writes = [w for o in group if o[0] != K_GATHER for w in ((o[4], o[5]) if o[0] == K_STEP else (o[4],)) if w >= 0]
         touched = [[f for f in (o[2], o[3], o[4], o[5], o[7]) if f >= 0] for o in group]
-        for i, w in enumerate(writes):
+        for i, o in enumerate(group):
+            own = [w for w in ((o[4], o[5]) if o[0] == K_STEP else (o[4],)) if w >= 0] if o[0] != K_GATHER else []
             for j, t in enumerate(touched):
                 if j != i:
-                    assert w not in t, (ph, group)
+                    assert not set(own) & set(t), (ph, group)
+        assert len(writes) == len(set(writes))
         pending = []
         for kind, flags, up, uc, un, uf, gp, gf, step, row, _ in group:
             # gather and stencil of one op run concurrently: the gathered field is not written
@@ -57,13 +60,16 @@ def replay(N, seg, has_src, has_rec):
                 assert _holds(tag[up], "u", n - 1, N) and _holds(tag[uc], "u", n, N), (n, tag[up], tag[uc])
                 assert bool(flags & F_INJECT) == has_src
                 pending.append((un, ("u", n + 1)))
+                if uf >= 0:  # also stores L u^n, the Laplacian of its current level
+                    assert uf not in (up, uc, un)
+                    pending.append((uf, ("Lu", n)))
                 if n == N - 1:
                     fwd_done = True
             elif kind == K_ADJ:
                 m = step + 1
                 assert fwd_done and un != uc
                 assert _holds(tag[up], "mu", m + 2, N) and _holds(tag[uc], "mu", m + 1, N), (m, tag[up], tag[uc])
-                assert _holds(tag[uf], "u", m - 1, N), (m, tag[uf])
+                assert tag[uf] == ("Lu", m - 1), (m, tag[uf])
                 assert bool(flags & F_INJECT) == has_rec
                 assert not adj_seen or m == adj_seen[-1] - 1
                 assert gpart is None or gp == gpart
@@ -107,17 +113,19 @@ def test_schedule_length_and_fields(N, seg):
 
     def fields(C):
         nck = -(-N // C)
-        return (nck + 1 if C == 1 else 2 * nck) + (2 if nck >= 2 else 1) * (C - 1) + 5
+        return (nck + 1 if C == 1 else 2 * nck) + (0 if C == 1 else (2 if nck >= 2 else 1) * C) + 5
 
     C = seg or 1 + int(np.argmin([fields(c) for c in range(1, N + 1)]))
     nck = -(-N // C)
     last = N - (nck - 1) * C  # steps of the last segment
-    # ops: forward + lone gather (unless merged into a recompute) + recompute + adjoint + final gather
-    assert len(ops) == N + (0 if last > 1 else 1) + (N - nck) + N + 1
+    # ops: forward + lone gather (C = 1: no recompute to merge it into) + recompute (every level
+    # once, storing its Laplacian) + adjoint + final gather
+    assert len(ops) == N + (1 if C == 1 else 0) + (0 if C == 1 else N) + N + 1
     assert nf == fields(C)
     # phases: forward, the last segment's recompute alone, then each adjoint segment together with
     # the previous segment's recompute, then the final gather
-    phases = N + max(last - 1, 1) + sum(max(min(C, N - k * C), C - 1 if k else 0) for k in range(nck)) + 1
+    rec = C if C > 1 else 0
+    phases = N + (last if C > 1 else 1) + sum(max(min(C, N - k * C), rec if k else 0) for k in range(nck)) + 1
     assert ops[-1, 10] + 1 == phases
 
 
