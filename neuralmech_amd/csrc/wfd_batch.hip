@@ -139,7 +139,8 @@ __device__ __forceinline__ uint32_t hw_xcc_id() { return __builtin_amdgcn_s_getr
 
 // Forward / recompute tiles march cfg_rows(cfg) rows per wave (8: fewer re-reads of the window
 // rows and fewer tiles per phase; 3.44 s against 3.53 s with 4 rows for the 64-shot batch);
-// float adjoint tiles (two register windows) march ADJ_ROWS = 4 and float64 tiles 2, which keeps
+// float adjoint tiles march ADJ_ROWS = 4 (8 once they read the stored Laplacian instead of a
+// second window: 3.165 s against 3.148 s) and float64 tiles 2, which keeps
 // the kernel at 3 waves per SIMD.  At that occupancy the 8-row tiles load their rows' operands 2
 // rows ahead, the others row by row (a look-ahead on 2-row adjoint tiles spills 13 VGPRs, on
 // 4-row ones 41).  4-row adjoint tiles re-read 12 window rows per 4 output rows instead of 10 per
