@@ -258,8 +258,8 @@ int wfd_plan_forward(wfd_plan* plan, const wfd_medium* med, const wfd_points* sr
  * partials flushed into G every 32 adjoint steps (at (m-1) % 32 == 0); a 2D grid of several
  * shots keeps one partial per group of 4 consecutive shots (terms added in shot order, groups
  * flushed in order), other grids one per shot, so G is deterministic whatever the launch shape
- * and checkpoint spacing.  On such grids the recompute also stores L u^n for the imaging (one
- * more field per checkpoint segment in the plan's store). */
+ * and checkpoint spacing.  On 2D grids the recompute also stores L u^n for the imaging (one more
+ * field per checkpoint segment in the plan's store). */
 int wfd_plan_backward(wfd_plan* plan, const wfd_medium* med, const wfd_points* src,
                       const wfd_points* adj, const wfd_points* src_rec, double* G,
                       wfd_stream_t stream);

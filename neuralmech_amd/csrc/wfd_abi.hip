@@ -266,7 +266,7 @@ struct Launch {
     int64_t gather_off;
     void* un2 = nullptr;  // two-step launch: u^{n+2} (un receives u^{n+1})
     int t2rows = 16;
-    bool img_group = false;  // 2D imaging of several shots: one partial per IMG_GROUP shots
+    bool img_group = false;  // 2D plan imaging: uf holds L u; several shots share a partial per IMG_GROUP
     void* ucopy = nullptr;   // 2D forward: receives L u_cur (see StepArgs::ucopy)
 };
 
@@ -306,7 +306,7 @@ int launch_t(const Launch& L, hipStream_t s)
     if (L.img) {
         a.uf = static_cast<const T*>(L.uf) + off;
         a.g = static_cast<T*>(L.gpart) + off;
-        a.img_group = L.img_group && g->ndim == 2 && g->nshots > 1 ? IMG_GROUP : 0;
+        a.img_group = L.img_group && g->ndim == 2 ? (g->nshots > 1 ? IMG_GROUP : 1) : 0;
     }
     a.shot_stride = g->elems;
     a.ld = g->ld;

@@ -62,9 +62,10 @@ struct StepArgs {
     uint32_t gx, gy;     // stencil block grid (x fastest); the launch is 1D: ngb + gx*gy*gz blocks
     int32_t nshots;
     int32_t shot_fast;  // 2D step launches: tile index -> (shot fastest, then column, then row block)
-    // 2D imaging launches of several shots: IMG_GROUP = the 4 waves of a tile take the same rows of
-    // 4 consecutive shots, and shot group j's imaging partial (field j of a.g) accumulates their
-    // terms in shot order; 0 = one partial per shot
+    // 2D imaging launches of the gradient plan (a.uf holds L u of the forward field): IMG_GROUP =
+    // the 4 waves of a tile take the same rows of 4 consecutive shots, and shot group j's imaging
+    // partial (field j of a.g) accumulates their terms in shot order; 1 = one shot, one partial;
+    // 0 = a.uf holds u (its Laplacian is taken in the launch), one partial per shot
     int32_t img_group;
     // Two-step launches: un receives u^{n+1}, un2 u^{n+2}; their gather blocks record up (u^{n-1})
     // and uc (u^n) into trace rows step + gather_off - 1 and step + gather_off.

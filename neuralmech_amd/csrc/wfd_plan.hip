@@ -30,11 +30,11 @@ extern "C" int wfd_adjoint_step(const wfd_grid* g, const void* mu_prev, const vo
 struct wfd_plan {
     wfd_grid g;
     int64_t nsteps = 0, seg = 0, nckpt = 0, flush = 32;
-    // 2D grids of several shots: the adjoint's imaging reads L u^{m-1} from a stored field instead
+    // 2D grids: the adjoint's imaging reads L u^{m-1} from a stored field instead
     // of taking the Laplacian of u^{m-1}.  The recompute (every level kept: the forward) steps in
     // place in the two work fields, as a plain run does, and stores the Laplacian of its current
     // level as a second, nontemporal output of the launch: C slots per segment (L u^{s0} ..
-    // L u^{s1-1}) instead of C - 1, one more recompute step per segment.  Other grids step into
+    // L u^{s1-1}) instead of C - 1, one more recompute step per segment.  1D and 3D grids step into
     // the kept fields directly and the adjoint reads u.
     bool lu = false;
     size_t bytes = 0;
@@ -90,7 +90,7 @@ int wfd_plan_create(wfd_plan** out, const wfd_grid* g, int64_t nsteps, int64_t s
     if ((seg < 0 && seg != WFD_SEG_FASTEST) || seg > nsteps)
         return plan_fail(WFD_EINVAL, "wfd_plan_create: seg must be in [0, nsteps] or WFD_SEG_FASTEST");
     const size_t batch = (size_t)g->elems * (size_t)g->nshots * (g->dtype == WFD_F64 ? 8 : 4);
-    const bool lu = wfd::img_groups(g) > 0;
+    const bool lu = g->ndim == 2;
     if (seg == WFD_SEG_FASTEST) {
         // every level kept (no recompute: the forward and the adjoint pass only)
         // when that store fits the budget

@@ -392,7 +392,8 @@ __device__ __forceinline__ void step2d_tile(const StepArgs<T>& a, uint32_t tx, u
 }
 
 // A step launch: gather blocks copy receiver values, the other blocks update one tile each.
-template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, bool NT, bool GRP = false>
+template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool XL, bool NT, bool GRP = false,
+          bool LUF = false>
 __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
 {
     // grouped imaging launches have one tile column per shot group instead of per shot
@@ -407,26 +408,26 @@ __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
     // registers cost a wave per SIMD (2 and 8 rows keep their 3 and 2 waves).  Loading a wave's
     // operands up front (LA = RI - 1) measured slower for lack of waves.
     constexpr int LA = RI > 1 && !(IMG && RI == 4) ? 1 : 0;
-    step2d_tile<T, K, VROW, VARC, SPONGE, IMG, RI, XL, LA, NT, GRP>(a, bp.x, bp.y, (int)bp.z, step);
+    step2d_tile<T, K, VROW, VARC, SPONGE, IMG, RI, XL, LA, NT, GRP, LUF>(a, bp.x, bp.y, (int)bp.z, step);
 }
 
-template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool NT, bool GRP = false>
+template <typename T, int K, bool VROW, bool VARC, bool SPONGE, bool IMG, int RI, bool NT, bool GRP = false, bool LUF = false>
 __global__ __launch_bounds__(256) void step2d_kernel(const StepArgs<T> a)
 {
-    step2d_body<T, K, VROW, VARC, SPONGE, IMG, RI, false, NT, GRP>(a);
+    step2d_body<T, K, VROW, VARC, SPONGE, IMG, RI, false, NT, GRP, LUF>(a);
 }
 
 constexpr int XL_COLS = 248;
 
-template <typename T, int K, bool VARC, bool SPONGE, bool IMG, int RI, bool NT, bool GRP = false>
+template <typename T, int K, bool VARC, bool SPONGE, bool IMG, int RI, bool NT, bool GRP = false, bool LUF = false>
 __global__ __launch_bounds__(256) void step2d_xl_kernel(const StepArgs<T> a)
 {
-    step2d_body<T, K, true, VARC, SPONGE, IMG, RI, true, NT, GRP>(a);
+    step2d_body<T, K, true, VARC, SPONGE, IMG, RI, true, NT, GRP, LUF>(a);
 }
 
 // ------------------------------------------------------------------------------ dispatch
 // variant bit 0: lane-exchange halo (XL); bit 1: streaming access (NT)
-template <typename T, int K, bool VARC, bool SPONGE, bool IMG, bool NT, bool GRP = false>
+template <typename T, int K, bool VARC, bool SPONGE, bool IMG, bool NT, bool GRP = false, bool LUF = false>
 static hipError_t launch2d_kn(const StepArgs<T>& a0, bool vrow, int variant, int ri, int nshots, hipStream_t s)
 {
     const dim3 block(256);
@@ -449,27 +450,27 @@ static hipError_t launch2d_kn(const StepArgs<T>& a0, bool vrow, int variant, int
     if (variant & 1) {
         if constexpr (sizeof(T) == 4) {
             switch (ri) {
-                case 1: { const dim3 g = grid(XL_COLS, 1); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 1, NT, GRP>), g, block, 0, s, a); break; }
-                case 2: { const dim3 g = grid(XL_COLS, 2); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 2, NT, GRP>), g, block, 0, s, a); break; }
-                case 8: { const dim3 g = grid(XL_COLS, 8); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 8, NT, GRP>), g, block, 0, s, a); break; }
-                default: { const dim3 g = grid(XL_COLS, 4); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4, NT, GRP>), g, block, 0, s, a); break; }
+                case 1: { const dim3 g = grid(XL_COLS, 1); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 1, NT, GRP, LUF>), g, block, 0, s, a); break; }
+                case 2: { const dim3 g = grid(XL_COLS, 2); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 2, NT, GRP, LUF>), g, block, 0, s, a); break; }
+                case 8: { const dim3 g = grid(XL_COLS, 8); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 8, NT, GRP, LUF>), g, block, 0, s, a); break; }
+                default: { const dim3 g = grid(XL_COLS, 4); hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4, NT, GRP, LUF>), g, block, 0, s, a); break; }
             }
         } else {
             const dim3 g = grid(XL_COLS, 4);
-            hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4, NT, GRP>), g, block, 0, s, a);
+            hipLaunchKernelGGL((step2d_xl_kernel<T, K, VARC, SPONGE, IMG, 4, NT, GRP, LUF>), g, block, 0, s, a);
         }
         return hipGetLastError();
     }
     if constexpr (sizeof(T) == 4) {
         switch (ri) {
-            case 1: { const dim3 g = grid(256, 1); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 1, NT, GRP>), g, block, 0, s, a); break; }
-            case 2: { const dim3 g = grid(256, 2); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 2, NT, GRP>), g, block, 0, s, a); break; }
-            case 8: { const dim3 g = grid(256, 8); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 8, NT, GRP>), g, block, 0, s, a); break; }
-            default: { const dim3 g = grid(256, 4); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4, NT, GRP>), g, block, 0, s, a); break; }
+            case 1: { const dim3 g = grid(256, 1); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 1, NT, GRP, LUF>), g, block, 0, s, a); break; }
+            case 2: { const dim3 g = grid(256, 2); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 2, NT, GRP, LUF>), g, block, 0, s, a); break; }
+            case 8: { const dim3 g = grid(256, 8); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 8, NT, GRP, LUF>), g, block, 0, s, a); break; }
+            default: { const dim3 g = grid(256, 4); hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4, NT, GRP, LUF>), g, block, 0, s, a); break; }
         }
     } else {
         const dim3 g = grid(256, 4);
-        hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4, NT, GRP>), g, block, 0, s, a);
+        hipLaunchKernelGGL((step2d_kernel<T, K, true, VARC, SPONGE, IMG, 4, NT, GRP, LUF>), g, block, 0, s, a);
     }
     return hipGetLastError();
 }
@@ -487,7 +488,15 @@ static hipError_t launch2d_flags(const StepArgs<T>& a, bool vrow, bool varc, boo
             default: return launch2d_kn<T, K, true, true, true, NT, true>(a, vrow, variant, ri, nshots, s);
         }
     }
-    if (!img && vrow && a.ucopy) {  // forward instances that also store the kept copy of u_next
+    if (img && vrow && a.img_group == 1) {  // one partial per shot, L u of the forward field read pointwise
+        switch (f) {
+            case 1: return launch2d_kn<T, K, false, false, true, NT, false, true>(a, vrow, variant, ri, nshots, s);
+            case 3: return launch2d_kn<T, K, false, true, true, NT, false, true>(a, vrow, variant, ri, nshots, s);
+            case 5: return launch2d_kn<T, K, true, false, true, NT, false, true>(a, vrow, variant, ri, nshots, s);
+            default: return launch2d_kn<T, K, true, true, true, NT, false, true>(a, vrow, variant, ri, nshots, s);
+        }
+    }
+    if (!img && vrow && a.ucopy) {  // forward instances that also store L u_cur
         switch (f) {
             case 0: return launch2d_kn<T, K, false, false, false, NT, true>(a, vrow, variant, ri, nshots, s);
             case 2: return launch2d_kn<T, K, false, true, false, NT, true>(a, vrow, variant, ri, nshots, s);
