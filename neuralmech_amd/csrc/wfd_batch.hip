@@ -142,8 +142,9 @@ __device__ __forceinline__ uint32_t hw_xcc_id() { return __builtin_amdgcn_s_getr
 // float adjoint tiles march ADJ_ROWS = 4 (8 once they read the stored Laplacian instead of a
 // second window: 3.165 s against 3.148 s) and float64 tiles 2, which keeps
 // the kernel at 3 waves per SIMD.  At that occupancy the 8-row tiles load their rows' operands 2
-// rows ahead, the others row by row (a look-ahead on 2-row adjoint tiles spills 13 VGPRs, on
-// 4-row ones 41).  4-row adjoint tiles re-read 12 window rows per 4 output rows instead of 10 per
+// rows ahead and the float adjoint tiles 1 row ahead (157 VGPRs, no spills, since they read the
+// stored Laplacian: 3.149 -> 3.119 s; with the forward field's window it had spilled 41 VGPRs),
+// the others row by row.  4-row adjoint tiles re-read 12 window rows per 4 output rows instead of 10 per
 // 2 and halve the adjoint tiles per phase: 3.350 -> 3.258 s for the 64-shot batch, though the
 // kernel then spills 3 VGPRs (2 slots per CU with 8-row adjoint tiles and no spills: 3.368 s).
 constexpr int ADJ_ROWS = 4;
@@ -159,9 +160,9 @@ __device__ __forceinline__ void batch_tile_kc(const StepArgs<T>& a, int cfg, uin
             step2d_tile<T, K, true, true, true, IMG, 2, false, 0, false, CPY, IMG>(a, tx, ty, 0, step);
     } else if constexpr (IMG) {
         if (cfg_variant(cfg) == 1)
-            step2d_tile<T, K, true, true, true, IMG, ADJ_ROWS, true, 0, false, false, true>(a, tx, ty, 0, step);
+            step2d_tile<T, K, true, true, true, IMG, ADJ_ROWS, true, 1, false, false, true>(a, tx, ty, 0, step);
         else
-            step2d_tile<T, K, true, true, true, IMG, ADJ_ROWS, false, 0, false, false, true>(a, tx, ty, 0, step);
+            step2d_tile<T, K, true, true, true, IMG, ADJ_ROWS, false, 1, false, false, true>(a, tx, ty, 0, step);
     } else {
         switch (cfg) {
             case 0: step2d_tile<T, K, true, true, true, IMG, 8, true, 2, false, CPY>(a, tx, ty, 0, step); break;
