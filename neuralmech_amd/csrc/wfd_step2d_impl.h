@@ -404,10 +404,12 @@ __device__ __forceinline__ void step2d_body(const StepArgs<T>& a)
         gather_points(a, bp.g, step);
         return;
     }
-    // one row ahead, except imaging tiles of 4 rows (two windows), where the look-ahead's
-    // registers cost a wave per SIMD (2 and 8 rows keep their 3 and 2 waves).  Loading a wave's
-    // operands up front (LA = RI - 1) measured slower for lack of waves.
-    constexpr int LA = RI > 1 && !(IMG && RI == 4) ? 1 : 0;
+    // one row ahead, except imaging tiles of 4 rows that take the forward field's Laplacian
+    // themselves (two windows), where the look-ahead's registers cost a wave per SIMD (2 and 8
+    // rows keep their 3 and 2 waves); the instances reading the stored Laplacian stay at 4 waves
+    // per SIMD with it (112 / 122 VGPRs; 16 x 1024^2 gradient step 138.3 -> 135.0 us).  Loading a
+    // wave's operands up front (LA = RI - 1) measured slower for lack of waves.
+    constexpr int LA = RI > 1 && !(IMG && RI == 4 && !GRP && !LUF) ? 1 : 0;
     step2d_tile<T, K, VROW, VARC, SPONGE, IMG, RI, XL, LA, NT, GRP, LUF>(a, bp.x, bp.y, (int)bp.z, step);
 }
 
